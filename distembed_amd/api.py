@@ -1,0 +1,219 @@
+"""Python view of the native library (test and benchmark harness).
+
+The product path is C++ host code plus HIP kernels behind the C ABI in
+include/force_embed.h; this module only marshals numpy arrays through ctypes.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib
+from ._lib import FeError, fe_graph, fe_tuning
+
+FP32 = _lib.FE_GATHER_FP32
+BF16 = _lib.FE_GATHER_BF16
+QUIRKS_REFERENCE = _lib.FE_QUIRKS_REFERENCE
+
+
+def _lib_or_raise():
+    return _lib.load()
+
+
+def _ptr(a: np.ndarray, ctype):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _host_err() -> str:
+    return _lib_or_raise().fe_last_error(None).decode()
+
+
+@dataclass
+class Graph:
+    """Symmetric CSR (row_ptr int64[n+1], col_idx int32[nnz])."""
+    n: int
+    row_ptr: np.ndarray
+    col_idx: np.ndarray
+
+    @property
+    def nnz(self) -> int:
+        return int(self.col_idx.shape[0])
+
+    @staticmethod
+    def _take(g: fe_graph) -> "Graph":
+        lib = _lib_or_raise()
+        n, nnz = int(g.n), int(g.nnz)
+        rp = np.ctypeslib.as_array(g.row_ptr, shape=(n + 1,)).copy()
+        ci = (np.ctypeslib.as_array(g.col_idx, shape=(nnz,)).copy() if nnz > 0
+              else np.zeros(0, dtype=np.int32))
+        lib.fe_free_graph(C.byref(g))
+        return Graph(n, rp, ci)
+
+    @staticmethod
+    def rmat(scale: int, edge_factor: int = 16, a: float = 0.57, b: float = 0.19,
+             c: float = 0.19, seed: int = 0) -> "Graph":
+        lib = _lib_or_raise()
+        g = fe_graph()
+        _lib.check(lib.fe_gen_rmat(scale, edge_factor, a, b, c, seed, C.byref(g)), _host_err)
+        return Graph._take(g)
+
+    @staticmethod
+    def sbm(n: int, blocks: int, p_in: float, p_out: float, seed: int = 0) -> "Graph":
+        lib = _lib_or_raise()
+        g = fe_graph()
+        _lib.check(lib.fe_gen_sbm(n, blocks, p_in, p_out, seed, C.byref(g)), _host_err)
+        return Graph._take(g)
+
+    @staticmethod
+    def from_edges(n: int, u, v) -> "Graph":
+        lib = _lib_or_raise()
+        u = np.ascontiguousarray(u, dtype=np.int64)
+        v = np.ascontiguousarray(v, dtype=np.int64)
+        g = fe_graph()
+        _lib.check(lib.fe_csr_from_edges(n, _ptr(u, C.c_int64), _ptr(v, C.c_int64), u.shape[0],
+                                          C.byref(g)), _host_err)
+        return Graph._take(g)
+
+    @staticmethod
+    def read_mtx(path: str) -> "Graph":
+        lib = _lib_or_raise()
+        g = fe_graph()
+        _lib.check(lib.fe_read_mtx(str(path).encode(), C.byref(g)), _host_err)
+        return Graph._take(g)
+
+
+def init_uniform(n: int, d: int, lo: float = -0.5, hi: float = 0.5, seed: int = 0) -> np.ndarray:
+    lib = _lib_or_raise()
+    x = np.empty((n, d), dtype=np.float32)
+    _lib.check(lib.fe_init_uniform(_ptr(x, C.c_float), n * d, lo, hi, seed), _host_err)
+    return x
+
+
+def reference_negatives(n: int, n_lists: int, ns: int, policy: int = _lib.FE_NEG_REDRAW):
+    """Lists l = 0..n_lists-1 of generate_random_numbers(0, n, l, ns)
+    (cpp/core/common.cpp:37-53); returns (int32 [n_lists, ns], draws equal to n)."""
+    lib = _lib_or_raise()
+    out = np.empty((n_lists, ns), dtype=np.int32)
+    hits = C.c_int64(0)
+    _lib.check(lib.fe_reference_negatives(n, n_lists, ns, policy, _ptr(out, C.c_int32),
+                                          C.byref(hits)), _host_err)
+    return out, int(hits.value)
+
+
+def eval_negative(seed: int, r: int, k: int, n: int) -> int:
+    return int(_lib_or_raise().fe_eval_negative(seed, r, k, n))
+
+
+def write_embedding(path: str, x: np.ndarray) -> None:
+    lib = _lib_or_raise()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    _lib.check(lib.fe_write_embedding(str(path).encode(), _ptr(x, C.c_float), x.shape[0],
+                                      x.shape[1]), _host_err)
+
+
+class ForceEmbed:
+    """One graph's device-resident embedding state (fe_ctx)."""
+
+    def __init__(self, n: int, d: int, gather: int = FP32, quirks: int = QUIRKS_REFERENCE,
+                 device: int = 0):
+        self._lib = _lib_or_raise()
+        self._ctx = C.c_void_p()
+        st = self._lib.fe_create(C.byref(self._ctx), device, n, d, gather, quirks)
+        if st != _lib.FE_OK:
+            raise FeError(st, self._lib.fe_last_error(None).decode())
+        self.n, self.d, self.gather, self.quirks = n, d, gather, quirks
+
+    def _err(self) -> str:
+        return self._lib.fe_last_error(self._ctx).decode()
+
+    def _check(self, st: int) -> None:
+        _lib.check(st, self._err)
+
+    def close(self) -> None:
+        if self._ctx:
+            self._lib.fe_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload_csr(self, g: Graph) -> None:
+        rp = np.ascontiguousarray(g.row_ptr, dtype=np.int64)
+        ci = np.ascontiguousarray(g.col_idx, dtype=np.int32)
+        self._check(self._lib.fe_upload_csr(self._ctx, _ptr(rp, C.c_int64), _ptr(ci, C.c_int32),
+                                            ci.shape[0]))
+
+    def upload_x(self, x: np.ndarray) -> None:
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        assert x.shape == (self.n, self.d)
+        self._check(self._lib.fe_upload_x(self._ctx, _ptr(x, C.c_float)))
+
+    def download_x(self) -> np.ndarray:
+        x = np.empty((self.n, self.d), dtype=np.float32)
+        self._check(self._lib.fe_download_x(self._ctx, _ptr(x, C.c_float)))
+        return x
+
+    def set_negatives(self, lists: np.ndarray) -> None:
+        lists = np.ascontiguousarray(lists, dtype=np.int32)
+        n_lists, ns = (lists.shape if lists.ndim == 2 else (0, 0))
+        self._check(self._lib.fe_set_negatives(self._ctx, _ptr(lists, C.c_int32), n_lists, ns))
+
+    def configure(self, batch_size: int, lr: float = 0.02, max_bound: float = 5.0) -> None:
+        self._check(self._lib.fe_configure(self._ctx, batch_size, lr, max_bound))
+
+    def set_tuning(self, chunk=0, max_chunks=0, unroll=0, waves_per_block=0,
+                   items_per_wave=0) -> None:
+        t = fe_tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave)
+        self._check(self._lib.fe_set_tuning(self._ctx, C.byref(t)))
+
+    def get_tuning(self) -> dict:
+        t = fe_tuning()
+        self._check(self._lib.fe_get_tuning(self._ctx, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in fe_tuning._fields_}
+
+    def step(self, epoch: int, batch: int) -> None:
+        self._check(self._lib.fe_step(self._ctx, epoch, batch))
+
+    def step_delta(self, epoch: int, batch: int, rows: int) -> np.ndarray:
+        out = np.empty((rows, self.d), dtype=np.float32)
+        self._check(self._lib.fe_step_delta(self._ctx, epoch, batch, _ptr(out, C.c_float)))
+        return out
+
+    def train(self, iterations: int, batch_size: int, ns: int, lr: float,
+              use_graph: bool = True) -> None:
+        self._check(self._lib.fe_train(self._ctx, iterations, batch_size, ns, lr,
+                                       1 if use_graph else 0))
+
+    def synchronize(self) -> None:
+        self._check(self._lib.fe_synchronize(self._ctx))
+
+    def loss(self, eval_seed: int = 0, eval_ns: int = 5):
+        a, r = C.c_double(), C.c_double()
+        self._check(self._lib.fe_loss(self._ctx, eval_seed, eval_ns, C.byref(a), C.byref(r)))
+        return a.value, r.value
+
+    def epoch_units(self, ns: int):
+        e, n0, n1 = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self._lib.fe_epoch_units(self._ctx, ns, C.byref(e), C.byref(n0), C.byref(n1)))
+        return e.value, n0.value, n1.value
+
+    def set_timing(self, enable: bool) -> None:
+        self._check(self._lib.fe_set_timing(self._ctx, 1 if enable else 0))
+
+    def kernel_times(self):
+        f, x, a, n = C.c_double(), C.c_double(), C.c_double(), C.c_int64()
+        self._check(self._lib.fe_kernel_times(self._ctx, C.byref(f), C.byref(x), C.byref(a),
+                                              C.byref(n)))
+        return {"force_ms": f.value, "fixup_ms": x.value, "apply_ms": a.value,
+                "launches": n.value}

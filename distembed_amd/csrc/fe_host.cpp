@@ -1,0 +1,443 @@
+// Host-side helpers of the force-embedding library: the reference negative
+// sampler, graph generators, CSR construction, MatrixMarket ingest and the
+// embedding writer. Nothing here touches the GPU.
+#include <algorithm>
+#include <cerrno>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "force_embed.h"
+#include "fe_host.hpp"
+
+namespace fe {
+
+thread_local std::string g_host_error;
+
+void set_host_error(const std::string &s) { g_host_error = s; }
+const char *host_error() { return g_host_error.c_str(); }
+
+uint64_t splitmix64_next(uint64_t &state) {
+  uint64_t z = (state += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+uint64_t splitmix64_mix(uint64_t z) {
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// ---- reference sampler (cpp/core/common.cpp:37-53) --------------------------
+//
+// std::minstd_rand is the linear congruential engine x <- 48271 x mod (2^31-1)
+// (C++ [rand.predef]); constructing it with seed s sets x = s mod m, or 1 if
+// that is 0. Its outputs lie in [1, 2^31-2]. libstdc++'s
+// uniform_int_distribution<int>(lo, hi) downscales such a generator, whose
+// range (2^31-3) is neither 2^32-1 nor 2^64-1, by rejection:
+//   urange = hi - lo, erange = urange + 1, scaling = (2^31-3) / erange,
+//   past = erange * scaling; draw ret = x - 1 until ret < past;
+//   return ret / scaling + lo.
+// A CPU test checks this restatement against std:: compiled with g++.
+struct MinStd {
+  uint64_t x;
+  explicit MinStd(uint64_t seed) {
+    x = seed % 2147483647ull;
+    if (x == 0) x = 1;
+  }
+  uint64_t next() {
+    x = (x * 48271ull) % 2147483647ull;
+    return x;
+  }
+};
+
+static inline int64_t minstd_uniform(MinStd &g, int64_t lo, int64_t hi) {
+  const uint64_t urngrange = 2147483646ull - 1ull;
+  const uint64_t urange = (uint64_t)(hi - lo);
+  const uint64_t erange = urange + 1;
+  const uint64_t scaling = urngrange / erange;
+  const uint64_t past = erange * scaling;
+  uint64_t ret;
+  do {
+    ret = g.next() - 1ull;
+  } while (ret >= past);
+  return (int64_t)(ret / scaling) + lo;
+}
+
+} // namespace fe
+
+using namespace fe;
+
+extern "C" fe_status fe_reference_negatives(int64_t n, int n_lists, int ns, int policy,
+                                            int32_t *out, int64_t *n_hits) {
+  if (n <= 0 || n >= 2147483647LL || n_lists < 0 || ns < 0 || (n_lists > 0 && ns > 0 && !out) ||
+      (policy != FE_NEG_STRICT && policy != FE_NEG_REDRAW)) {
+    set_host_error("fe_reference_negatives: invalid argument");
+    return FE_ERR_INVALID_ARG;
+  }
+  // The reference range [0, n] must fit uniform_int_distribution<int>'s
+  // downscaling branch: n < 2^31 - 3.
+  if (n + 1 > 2147483645LL) {
+    set_host_error("fe_reference_negatives: n too large for the reference sampler");
+    return FE_ERR_INVALID_ARG;
+  }
+  int64_t hits = 0;
+  for (int l = 0; l < n_lists; ++l) {
+    MinStd g((uint64_t)(uint32_t)l);
+    for (int k = 0; k < ns; ++k) {
+      int64_t v = minstd_uniform(g, 0, n);
+      if (v == n) {
+        ++hits;
+        if (policy == FE_NEG_REDRAW) {
+          while (v == n) v = minstd_uniform(g, 0, n);
+        }
+      }
+      out[(size_t)l * ns + k] = (int32_t)v;
+    }
+  }
+  if (n_hits) *n_hits = hits;
+  if (hits > 0 && policy == FE_NEG_STRICT) {
+    set_host_error("fe_reference_negatives: a draw equals n (out of range) under FE_NEG_STRICT");
+    return FE_ERR_INVALID_ARG;
+  }
+  return FE_OK;
+}
+
+extern "C" int64_t fe_eval_negative(uint64_t seed, int64_t r, int k, int64_t n) {
+  if (n <= 0) return 0;
+  const uint64_t x = seed + 0x9E3779B97F4A7C15ull * ((uint64_t)r * 1048576ull + (uint64_t)k + 1ull);
+  return (int64_t)(splitmix64_mix(x) % (uint64_t)n);
+}
+
+// ---- CSR construction ----------------------------------------------------------
+
+namespace fe {
+
+// Builds the CSR of the directed entries (u[i], v[i]); with `symmetrize`
+// every entry is also added reversed. Columns are sorted and duplicates
+// merged; self-loops are dropped when `drop_self`.
+fe_status build_csr(int64_t n, const int64_t *u, const int64_t *v, int64_t m, bool symmetrize,
+                    bool drop_self, fe_graph *out) {
+  if (n <= 0 || n >= 2147483647LL || m < 0 || !out || (m > 0 && (!u || !v))) {
+    set_host_error("build_csr: invalid argument");
+    return FE_ERR_INVALID_ARG;
+  }
+  for (int64_t i = 0; i < m; ++i) {
+    if (u[i] < 0 || u[i] >= n || v[i] < 0 || v[i] >= n) {
+      set_host_error("build_csr: vertex id out of range");
+      return FE_ERR_INVALID_ARG;
+    }
+  }
+  std::vector<int64_t> deg(n + 1, 0);
+  for (int64_t i = 0; i < m; ++i) {
+    if (drop_self && u[i] == v[i]) continue;
+    deg[u[i]]++;
+    if (symmetrize) deg[v[i]]++;
+  }
+  std::vector<int64_t> start(n + 1, 0);
+  for (int64_t r = 0; r < n; ++r) start[r + 1] = start[r] + deg[r];
+  const int64_t total = start[n];
+  std::vector<int32_t> tmp;
+  try {
+    tmp.resize((size_t)total);
+  } catch (...) {
+    set_host_error("build_csr: out of host memory");
+    return FE_ERR_OOM;
+  }
+  std::vector<int64_t> pos(start.begin(), start.end() - 1);
+  for (int64_t i = 0; i < m; ++i) {
+    if (drop_self && u[i] == v[i]) continue;
+    tmp[pos[u[i]]++] = (int32_t)v[i];
+    if (symmetrize) tmp[pos[v[i]]++] = (int32_t)u[i];
+  }
+  std::vector<int64_t> newdeg(n, 0);
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t r = 0; r < n; ++r) {
+    int32_t *b = tmp.data() + start[r];
+    int32_t *e = tmp.data() + start[r + 1];
+    std::sort(b, e);
+    newdeg[r] = std::unique(b, e) - b;
+  }
+  int64_t *rp = (int64_t *)malloc(sizeof(int64_t) * (size_t)(n + 1));
+  if (!rp) {
+    set_host_error("build_csr: out of host memory");
+    return FE_ERR_OOM;
+  }
+  rp[0] = 0;
+  for (int64_t r = 0; r < n; ++r) rp[r + 1] = rp[r] + newdeg[r];
+  const int64_t nnz = rp[n];
+  if (nnz >= 2147483647LL) {
+    free(rp);
+    set_host_error("build_csr: more than 2^31-1 entries");
+    return FE_ERR_INVALID_ARG;
+  }
+  int32_t *ci = (int32_t *)malloc(sizeof(int32_t) * (size_t)std::max<int64_t>(nnz, 1));
+  if (!ci) {
+    free(rp);
+    set_host_error("build_csr: out of host memory");
+    return FE_ERR_OOM;
+  }
+#pragma omp parallel for schedule(dynamic, 4096)
+  for (int64_t r = 0; r < n; ++r)
+    std::memcpy(ci + rp[r], tmp.data() + start[r], sizeof(int32_t) * (size_t)newdeg[r]);
+  out->n = n;
+  out->nnz = nnz;
+  out->row_ptr = rp;
+  out->col_idx = ci;
+  return FE_OK;
+}
+
+} // namespace fe
+
+extern "C" fe_status fe_csr_from_edges(int64_t n, const int64_t *u, const int64_t *v, int64_t m,
+                                       fe_graph *out) {
+  return build_csr(n, u, v, m, true, true, out);
+}
+
+extern "C" void fe_free_graph(fe_graph *g) {
+  if (!g) return;
+  free(g->row_ptr);
+  free(g->col_idx);
+  g->row_ptr = nullptr;
+  g->col_idx = nullptr;
+  g->n = g->nnz = 0;
+}
+
+// ---- generators ---------------------------------------------------------------
+
+// R-MAT (Chakrabarti, Zhan, Faloutsos 2004): each of edge_factor * 2^scale
+// edges descends `scale` levels of the adjacency matrix, choosing the
+// quadrant (row bit, col bit) = (0,0) w.p. a, (0,1) w.p. b, (1,0) w.p. c,
+// (1,1) w.p. 1-a-b-c, most significant bit first. Edge e draws its
+// uniforms from splitmix64 started at seed-derived state + e, so the output
+// does not depend on the thread count. No vertex relabelling.
+extern "C" fe_status fe_gen_rmat(int scale, int edge_factor, double a, double b, double c,
+                                 uint64_t seed, fe_graph *out) {
+  if (scale < 1 || scale > 30 || edge_factor < 1 || a < 0 || b < 0 || c < 0 ||
+      a + b + c > 1.0 || !out) {
+    set_host_error("fe_gen_rmat: invalid argument");
+    return FE_ERR_INVALID_ARG;
+  }
+  const int64_t n = (int64_t)1 << scale;
+  const int64_t m = (int64_t)edge_factor * n;
+  std::vector<int64_t> u, v;
+  try {
+    u.resize((size_t)m);
+    v.resize((size_t)m);
+  } catch (...) {
+    set_host_error("fe_gen_rmat: out of host memory");
+    return FE_ERR_OOM;
+  }
+  const double two32 = 4294967296.0;
+  const uint64_t ta = (uint64_t)(a * two32);
+  const uint64_t tab = (uint64_t)((a + b) * two32);
+  const uint64_t tabc = (uint64_t)((a + b + c) * two32);
+  const uint64_t base = splitmix64_mix(seed ^ 0x52A7A11C0FFEEull);
+#pragma omp parallel for schedule(static, 65536)
+  for (int64_t e = 0; e < m; ++e) {
+    uint64_t st = base + (uint64_t)e * 0xD1B54A32D192ED03ull;
+    int64_t row = 0, col = 0;
+    uint64_t word = 0;
+    for (int lv = 0; lv < scale; ++lv) {
+      uint64_t r32;
+      if ((lv & 1) == 0) {
+        word = splitmix64_next(st);
+        r32 = word & 0xFFFFFFFFull;
+      } else {
+        r32 = word >> 32;
+      }
+      const int64_t bit = (int64_t)1 << (scale - 1 - lv);
+      if (r32 < ta) {
+      } else if (r32 < tab) {
+        col |= bit;
+      } else if (r32 < tabc) {
+        row |= bit;
+      } else {
+        row |= bit;
+        col |= bit;
+      }
+    }
+    u[e] = row;
+    v[e] = col;
+  }
+  return build_csr(n, u.data(), v.data(), m, true, true, out);
+}
+
+// Stochastic block model with geometric skipping over the candidate pairs of
+// each block pair (Batagelj and Brandes 2005): block pair (bi, bj), bi <= bj,
+// draws its own splitmix64 stream from (seed, bi, bj).
+extern "C" fe_status fe_gen_sbm(int64_t n, int blocks, double p_in, double p_out, uint64_t seed,
+                                fe_graph *out) {
+  if (n <= 0 || n >= 2147483647LL || blocks < 1 || blocks > n || p_in < 0 || p_in > 1 ||
+      p_out < 0 || p_out > 1 || !out) {
+    set_host_error("fe_gen_sbm: invalid argument");
+    return FE_ERR_INVALID_ARG;
+  }
+  std::vector<int64_t> bstart(blocks + 1, 0);
+  for (int i = 0; i < blocks; ++i)
+    bstart[i + 1] = bstart[i] + n / blocks + (i < n % blocks ? 1 : 0);
+  const int64_t npairs = (int64_t)blocks * (blocks + 1) / 2;
+  std::vector<std::vector<int64_t>> eu(npairs), ev(npairs);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int64_t pidx = 0; pidx < npairs; ++pidx) {
+    // decode pidx -> (bi, bj), bi <= bj
+    int64_t bi = 0, rem = pidx;
+    while (rem >= blocks - bi) {
+      rem -= blocks - bi;
+      ++bi;
+    }
+    const int64_t bj = bi + rem;
+    const double p = (bi == bj) ? p_in : p_out;
+    if (p <= 0.0) continue;
+    const int64_t si = bstart[bi + 1] - bstart[bi];
+    const int64_t sj = bstart[bj + 1] - bstart[bj];
+    const int64_t cand = si * sj; // (i, j) grid; within a block only i < j is kept
+    uint64_t st = splitmix64_mix(seed ^ 0x5B3C0DEull) ^ splitmix64_mix((uint64_t)bi * 1000003ull + (uint64_t)bj + 1);
+    const double lq = (p < 1.0) ? std::log1p(-p) : 0.0;
+    int64_t idx = -1;
+    while (true) {
+      if (p >= 1.0) {
+        idx += 1;
+      } else {
+        const double uni = ((splitmix64_next(st) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+        const double skip = std::floor(std::log(uni) / lq);
+        if (skip > (double)cand) break;
+        idx += 1 + (int64_t)skip;
+      }
+      if (idx >= cand) break;
+      const int64_t i = idx / sj, j = idx % sj;
+      if (bi == bj && i >= j) continue;
+      eu[pidx].push_back(bstart[bi] + i);
+      ev[pidx].push_back(bstart[bj] + j);
+    }
+  }
+  std::vector<int64_t> u, v;
+  for (int64_t pidx = 0; pidx < npairs; ++pidx) {
+    u.insert(u.end(), eu[pidx].begin(), eu[pidx].end());
+    v.insert(v.end(), ev[pidx].begin(), ev[pidx].end());
+  }
+  return build_csr(n, u.data(), v.data(), (int64_t)u.size(), true, true, out);
+}
+
+extern "C" fe_status fe_init_uniform(float *x, int64_t count, float lo, float hi, uint64_t seed) {
+  if (count < 0 || (count > 0 && !x) || !(hi >= lo)) {
+    set_host_error("fe_init_uniform: invalid argument");
+    return FE_ERR_INVALID_ARG;
+  }
+  const uint64_t base = splitmix64_mix(seed ^ 0x1A17C0DEull);
+  const double span = (double)hi - (double)lo;
+#pragma omp parallel for schedule(static, 65536)
+  for (int64_t i = 0; i < count; ++i) {
+    const uint64_t z = splitmix64_mix(base + (uint64_t)i * 0x9E3779B97F4A7C15ull);
+    const double u01 = (double)(z >> 40) * (1.0 / 16777216.0); // 24-bit uniform in [0,1)
+    x[i] = (float)((double)lo + span * u01);
+  }
+  return FE_OK;
+}
+
+// ---- MatrixMarket ingest (cpp/io/parrallel_IO.hpp:31-88) ------------------------
+//
+// The reference reads through CombBLAS ParallelReadMM(file, onebased=true,
+// maximum<>): 1-based ids, duplicates merged. A `symmetric` header stores
+// one triangle and is expanded to both directions; `general` entries are
+// kept as given. Values are ignored (the reference replaces them with the
+// column id, :64-75, and the embedding path never reads them). Self-loops
+// are kept: they add a zero force, as in the reference.
+extern "C" fe_status fe_read_mtx(const char *path, fe_graph *out) {
+  if (!path || !out) {
+    set_host_error("fe_read_mtx: invalid argument");
+    return FE_ERR_INVALID_ARG;
+  }
+  FILE *f = fopen(path, "r");
+  if (!f) {
+    set_host_error(std::string("fe_read_mtx: cannot open ") + path + ": " + strerror(errno));
+    return FE_ERR_INVALID_ARG;
+  }
+  char line[4096];
+  if (!fgets(line, sizeof line, f)) {
+    fclose(f);
+    set_host_error("fe_read_mtx: empty file");
+    return FE_ERR_INVALID_ARG;
+  }
+  std::string hdr(line);
+  for (auto &ch : hdr) ch = (char)tolower((unsigned char)ch);
+  if (hdr.rfind("%%matrixmarket", 0) != 0 || hdr.find("coordinate") == std::string::npos) {
+    fclose(f);
+    set_host_error("fe_read_mtx: not a MatrixMarket coordinate file");
+    return FE_ERR_INVALID_ARG;
+  }
+  const bool symmetric = hdr.find("symmetric") != std::string::npos ||
+                         hdr.find("hermitian") != std::string::npos;
+  if (hdr.find("skew-symmetric") != std::string::npos) {
+    fclose(f);
+    set_host_error("fe_read_mtx: skew-symmetric matrices are not supported");
+    return FE_ERR_INVALID_ARG;
+  }
+  long long rows = -1, cols = -1, nnz = -1;
+  while (fgets(line, sizeof line, f)) {
+    if (line[0] == '%' || line[0] == '\n' || line[0] == '\r') continue;
+    if (sscanf(line, "%lld %lld %lld", &rows, &cols, &nnz) != 3) {
+      fclose(f);
+      set_host_error("fe_read_mtx: bad size line");
+      return FE_ERR_INVALID_ARG;
+    }
+    break;
+  }
+  if (rows <= 0 || rows != cols || nnz < 0) {
+    fclose(f);
+    set_host_error("fe_read_mtx: the matrix must be square with a valid size line");
+    return FE_ERR_INVALID_ARG;
+  }
+  std::vector<int64_t> u, v;
+  u.reserve((size_t)nnz);
+  v.reserve((size_t)nnz);
+  long long read = 0;
+  while (read < nnz && fgets(line, sizeof line, f)) {
+    if (line[0] == '%' || line[0] == '\n' || line[0] == '\r') continue;
+    long long i, j;
+    if (sscanf(line, "%lld %lld", &i, &j) != 2 || i < 1 || j < 1 || i > rows || j > cols) {
+      fclose(f);
+      set_host_error("fe_read_mtx: bad entry line");
+      return FE_ERR_INVALID_ARG;
+    }
+    u.push_back(i - 1);
+    v.push_back(j - 1);
+    ++read;
+  }
+  fclose(f);
+  if (read != nnz) {
+    set_host_error("fe_read_mtx: fewer entries than announced");
+    return FE_ERR_INVALID_ARG;
+  }
+  if (symmetric) return build_csr(rows, u.data(), v.data(), (int64_t)u.size(), true, false, out);
+  return build_csr(rows, u.data(), v.data(), (int64_t)u.size(), false, false, out);
+}
+
+// Embedding writer (cpp/io/parrallel_IO.hpp:90-146 at world size 1).
+extern "C" fe_status fe_write_embedding(const char *path, const float *x, int64_t n, int d) {
+  if (!path || (!x && n > 0) || n < 0 || d <= 0) {
+    set_host_error("fe_write_embedding: invalid argument");
+    return FE_ERR_INVALID_ARG;
+  }
+  FILE *f = fopen(path, "w");
+  if (!f) {
+    set_host_error(std::string("fe_write_embedding: cannot open ") + path);
+    return FE_ERR_INVALID_ARG;
+  }
+  for (int64_t i = 0; i < n; ++i) {
+    fprintf(f, "%lu", (unsigned long)(i + 1));
+    for (int j = 0; j < d; ++j) fprintf(f, " %.5f", (double)x[i * d + j]);
+    fputc('\n', f);
+  }
+  const bool ok = fclose(f) == 0;
+  if (!ok) {
+    set_host_error("fe_write_embedding: write failed");
+    return FE_ERR_INVALID_ARG;
+  }
+  return FE_OK;
+}

@@ -1,0 +1,50 @@
+// Internal host-side declarations shared by the library's translation units.
+#pragma once
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "force_embed.h"
+
+namespace fe {
+
+void set_host_error(const std::string &s);
+const char *host_error();
+uint64_t splitmix64_mix(uint64_t z);
+uint64_t splitmix64_next(uint64_t &state);
+fe_status build_csr(int64_t n, const int64_t *u, const int64_t *v, int64_t m, bool symmetrize,
+                    bool drop_self, fe_graph *out);
+
+// Work plan of one graph at one batch size: the rows of batch j
+// (cpp/algo/algo.hpp:394-397) cut into work items.
+//   item  {row, e0, cnt, pslot}: edges [e0, e0+cnt) of `row`; pslot < 0
+//         means the item is the whole row and writes the new row, otherwise
+//         it writes partial sum `pslot` of a heavy row.
+//   split {row, p0, k, 0}: heavy row whose partials p0..p0+k-1 the fixup
+//         kernel adds in index order.
+// A row of degree <= chunk is one item; a heavier row is cut into
+// k = ceil(deg / max(chunk, ceil(deg / max_chunks))) near-equal chunks.
+struct Plan {
+  int B = 0;
+  int batches = 0;
+  int last = 0;
+  int chunk = 0;
+  int max_chunks = 0;
+  std::vector<int32_t> items;
+  std::vector<int64_t> item_off;  // batches + 1, in items
+  std::vector<int32_t> splits;
+  std::vector<int64_t> split_off; // batches + 1, in splits
+  std::vector<int32_t> pslots;    // partial slots used by batch j
+  int max_pslots = 0;
+  std::vector<int64_t> edges_eff; // CSR entries with c < c_limit in batch j
+  int64_t edges_eff_total = 0;
+};
+
+fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int B, int chunk,
+                     int max_chunks, int64_t c_limit, Plan *out);
+
+// Repulsive row counts of batch j in epoch 0 and in later epochs
+// (cpp/algo/algo.hpp:135,167-169,594).
+void rep_rows(const Plan &p, uint32_t quirks, int j, int *rep0, int *rep1);
+
+} // namespace fe

@@ -1,0 +1,450 @@
+// Device state of one embedding job: uploads, work plan, step descriptors,
+// launches and the FP64 loss.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "fe_job.hpp"
+
+namespace fe {
+
+fe_status job_check(Job &j, hipError_t e, const char *what) {
+  if (e == hipSuccess) return FE_OK;
+  j.err = std::string(what) + ": " + hipGetErrorString(e);
+  if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return FE_ERR_OOM;
+  return FE_ERR_HIP;
+}
+
+#define FE_HIP(j, call, what)                                                                  \
+  do {                                                                                         \
+    fe_status _s = job_check((j), (call), (what));                                             \
+    if (_s != FE_OK) return _s;                                                                \
+  } while (0)
+
+template <typename T>
+static fe_status ensure_cap(Job &j, T *&ptr, size_t &cap, size_t count, const char *what) {
+  if (count <= cap && ptr) return FE_OK;
+  if (ptr) {
+    hipFree(ptr);
+    j.device_bytes -= (int64_t)(cap * sizeof(T));
+    ptr = nullptr;
+    cap = 0;
+  }
+  const size_t want = std::max<size_t>(count, 1);
+  void *p = nullptr;
+  hipError_t e = hipMalloc(&p, want * sizeof(T));
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    j.err = std::string(what) + ": hipMalloc of " + std::to_string(want * sizeof(T)) +
+            " bytes failed: " + hipGetErrorString(e);
+    return FE_ERR_OOM;
+  }
+  ptr = static_cast<T *>(p);
+  cap = want;
+  j.device_bytes += (int64_t)(want * sizeof(T));
+  return FE_OK;
+}
+
+int rows_of_batch(const Job &j, int b) {
+  const int64_t r0 = (int64_t)b * j.plan.B;
+  return (int)(std::min<int64_t>(r0 + j.plan.B, j.n) - r0);
+}
+
+fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks) {
+  if (n <= 0 || n >= 2147483647LL) {
+    j.err = "n must be in [1, 2^31-1)";
+    return FE_ERR_INVALID_ARG;
+  }
+  if (d < 8 || d > 512 || d % 8 != 0 || ((d / 8) & (d / 8 - 1)) != 0) {
+    j.err = "d must be 8 x a power of two in [8, 512]";
+    return FE_ERR_INVALID_ARG;
+  }
+  if (dtype != FE_GATHER_FP32 && dtype != FE_GATHER_BF16) {
+    j.err = "gather dtype must be FE_GATHER_FP32 or FE_GATHER_BF16";
+    return FE_ERR_INVALID_ARG;
+  }
+  if (quirks & ~FE_QUIRKS_REFERENCE) {
+    j.err = "unknown quirk bits";
+    return FE_ERR_INVALID_ARG;
+  }
+  j.n = n;
+  j.d = d;
+  j.G = d / 8;
+  j.dtype = dtype;
+  j.quirks = quirks;
+  j.c_limit = (quirks & FE_QUIRK_SKIP_LAST_COL) ? n - 1 : n;
+  size_t cap = 0;
+  fe_status s = ensure_cap(j, j.X, cap, (size_t)n * d, "X");
+  if (s != FE_OK) return s;
+  if (dtype == FE_GATHER_BF16) {
+    size_t cap2 = 0;
+    s = ensure_cap(j, j.Xs, cap2, (size_t)n * d, "BF16 shadow");
+    if (s != FE_OK) return s;
+  }
+  size_t cap3 = 0;
+  s = ensure_cap(j, j.epoch, cap3, 1, "epoch counter");
+  if (s != FE_OK) return s;
+  return FE_OK;
+}
+
+void job_free(Job &j) {
+  hipFree(j.X);
+  hipFree(j.Xs);
+  hipFree(j.col);
+  hipFree(j.items);
+  hipFree(j.splits);
+  hipFree(j.S);
+  hipFree(j.P);
+  hipFree(j.negs);
+  hipFree(j.epoch);
+  hipFree(j.steps);
+  hipFree(j.steps_delta);
+  hipFree(j.prefix);
+  j.X = nullptr;
+  j.Xs = nullptr;
+  j.col = nullptr;
+  j.items = j.splits = nullptr;
+  j.S = j.P = nullptr;
+  j.negs = nullptr;
+  j.epoch = nullptr;
+  j.steps = j.steps_delta = nullptr;
+  j.prefix = nullptr;
+  j.device_bytes = 0;
+}
+
+fe_status job_upload_csr(Job &j, const int64_t *row_ptr, const int32_t *col, int64_t nnz,
+                         hipStream_t st) {
+  if (!row_ptr || (nnz > 0 && !col) || nnz < 0 || nnz >= 2147483647LL) {
+    j.err = "fe_upload_csr: null pointer or nnz out of [0, 2^31-1)";
+    return FE_ERR_INVALID_ARG;
+  }
+  if (row_ptr[0] != 0 || row_ptr[j.n] != nnz) {
+    j.err = "fe_upload_csr: row_ptr[0] must be 0 and row_ptr[n] must equal nnz";
+    return FE_ERR_INVALID_ARG;
+  }
+  std::vector<int32_t> skip((size_t)j.n, 0);
+  for (int64_t r = 0; r < j.n; ++r) {
+    if (row_ptr[r + 1] < row_ptr[r]) {
+      j.err = "fe_upload_csr: row_ptr is not monotone";
+      return FE_ERR_INVALID_ARG;
+    }
+    int32_t sk = 0;
+    for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
+      if (col[e] < 0 || col[e] >= j.n) {
+        j.err = "fe_upload_csr: column id out of [0, n)";
+        return FE_ERR_INVALID_ARG;
+      }
+      sk += (col[e] >= j.c_limit) ? 1 : 0;
+    }
+    skip[r] = sk;
+  }
+  fe_status s = ensure_cap(j, j.col, j.cap_col, (size_t)nnz, "CSR columns");
+  if (s != FE_OK) return s;
+  if (nnz > 0)
+    FE_HIP(j, hipMemcpyAsync(j.col, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st),
+           "fe_upload_csr: copy");
+  FE_HIP(j, hipStreamSynchronize(st), "fe_upload_csr: sync");
+  j.row_ptr.assign(row_ptr, row_ptr + j.n + 1);
+  j.skip_deg.swap(skip);
+  j.nnz = nnz;
+  j.plan_dirty = true;
+  j.steps_dirty = true;
+  return FE_OK;
+}
+
+fe_status job_upload_x(Job &j, const float *x, hipStream_t st) {
+  if (!x) {
+    j.err = "fe_upload_x: null pointer";
+    return FE_ERR_INVALID_ARG;
+  }
+  const size_t count = (size_t)j.n * j.d;
+  FE_HIP(j, hipMemcpyAsync(j.X, x, sizeof(float) * count, hipMemcpyHostToDevice, st),
+         "fe_upload_x: copy");
+  if (j.dtype == FE_GATHER_BF16)
+    FE_HIP(j, launch_to_bf16(j.X, j.Xs, (int64_t)count, st), "fe_upload_x: shadow refresh");
+  FE_HIP(j, hipStreamSynchronize(st), "fe_upload_x: sync");
+  j.x_ready = true;
+  return FE_OK;
+}
+
+fe_status job_download_x(Job &j, float *x, hipStream_t st) {
+  if (!x) {
+    j.err = "fe_download_x: null pointer";
+    return FE_ERR_INVALID_ARG;
+  }
+  if (!j.x_ready) {
+    j.err = "fe_download_x: X was never uploaded";
+    return FE_ERR_STATE;
+  }
+  FE_HIP(j, hipMemcpyAsync(x, j.X, sizeof(float) * (size_t)j.n * j.d, hipMemcpyDeviceToHost, st),
+         "fe_download_x: copy");
+  FE_HIP(j, hipStreamSynchronize(st), "fe_download_x: sync");
+  return FE_OK;
+}
+
+fe_status job_set_negatives(Job &j, const int32_t *lists, int n_lists, int ns, hipStream_t st) {
+  if (n_lists < 0 || ns < 0 || ns > kMaxNs || (n_lists > 0 && ns > 0 && !lists)) {
+    j.err = "fe_set_negatives: invalid argument (ns must be in [0, 64])";
+    return FE_ERR_INVALID_ARG;
+  }
+  if ((int64_t)ns * j.d > kMaxLdsFloats) {
+    j.err = "fe_set_negatives: ns * d must be <= 8192";
+    return FE_ERR_INVALID_ARG;
+  }
+  const size_t count = (size_t)n_lists * ns;
+  for (size_t i = 0; i < count; ++i) {
+    if (lists[i] < 0 || lists[i] >= j.n) {
+      j.err = "fe_set_negatives: negative id out of [0, n)";
+      return FE_ERR_INVALID_ARG;
+    }
+  }
+  fe_status s = ensure_cap(j, j.negs, j.cap_negs, count, "negative lists");
+  if (s != FE_OK) return s;
+  if (count > 0)
+    FE_HIP(j, hipMemcpyAsync(j.negs, lists, sizeof(int32_t) * count, hipMemcpyHostToDevice, st),
+           "fe_set_negatives: copy");
+  FE_HIP(j, hipStreamSynchronize(st), "fe_set_negatives: sync");
+  j.ns = ns;
+  j.n_lists = n_lists;
+  j.steps_dirty = true;
+  return FE_OK;
+}
+
+fe_status job_configure(Job &j, int B, float lr, float bound) {
+  if (B <= 0 || !std::isfinite(lr) || !std::isfinite(bound) || bound < 0) {
+    j.err = "fe_configure: batch size must be > 0, lr and max_bound finite, max_bound >= 0";
+    return FE_ERR_INVALID_ARG;
+  }
+  if (B != j.B) j.plan_dirty = true;
+  if (B != j.B || lr != j.lr || bound != j.bound) j.steps_dirty = true;
+  j.B = B;
+  j.lr = lr;
+  j.bound = bound;
+  return FE_OK;
+}
+
+fe_status job_set_tuning(Job &j, const Tuning &t) {
+  auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+  if (t.chunk < 1 || t.chunk > 64 || t.max_chunks < 1 || !pow2(t.unroll) || t.unroll > 8 ||
+      !pow2(t.waves_per_block) || t.waves_per_block > 8 || t.items_per_wave < 1 ||
+      t.items_per_wave > 64) {
+    j.err = "tuning out of range";
+    return FE_ERR_INVALID_ARG;
+  }
+  if (t.chunk != j.tun.chunk || t.max_chunks != j.tun.max_chunks) j.plan_dirty = true;
+  j.tun = t;
+  j.steps_dirty = true;
+  return FE_OK;
+}
+
+static StepDesc make_desc(const Job &j, int b, bool delta_mode) {
+  StepDesc sd;
+  std::memset(&sd, 0, sizeof sd);
+  sd.X = j.X;
+  sd.Xw = j.X;
+  sd.Xs = j.Xs;
+  sd.Xsw = j.Xs;
+  sd.col = j.col;
+  sd.items = j.items + 4 * j.plan.item_off[b];
+  sd.splits = j.splits + 4 * j.plan.split_off[b];
+  sd.S = j.S;
+  sd.P = j.P;
+  sd.negs = j.negs;
+  sd.epoch = j.epoch;
+  sd.n_items = (int)(j.plan.item_off[b + 1] - j.plan.item_off[b]);
+  sd.n_splits = (int)(j.plan.split_off[b + 1] - j.plan.split_off[b]);
+  sd.row_begin = (int)((int64_t)b * j.plan.B);
+  sd.rows = rows_of_batch(j, b);
+  rep_rows(j.plan, j.quirks, b, &sd.rep0, &sd.rep1);
+  sd.batch = b;
+  sd.ns = j.ns;
+  sd.n_lists = j.n_lists;
+  sd.D = j.d;
+  sd.G = j.G;
+  sd.bf16 = j.dtype == FE_GATHER_BF16 ? 1 : 0;
+  sd.c_limit = (int)j.c_limit;
+  sd.delta_mode = delta_mode ? 1 : 0;
+  sd.last_batch = (b == j.plan.batches - 1) ? 1 : 0;
+  sd.lr = j.lr;
+  sd.bound = j.bound;
+  return sd;
+}
+
+fe_status job_prepare(Job &j, hipStream_t st) {
+  if (j.nnz < 0) {
+    j.err = "CSR not uploaded";
+    return FE_ERR_STATE;
+  }
+  if (!j.x_ready) {
+    j.err = "X not uploaded";
+    return FE_ERR_STATE;
+  }
+  if (j.B <= 0) {
+    j.err = "batch size not configured";
+    return FE_ERR_STATE;
+  }
+  if (j.plan_dirty) {
+    Plan p;
+    fe_status s = build_plan(j.n, j.row_ptr.data(), nullptr, j.B, j.tun.chunk,
+                             j.tun.max_chunks, j.c_limit, &p);
+    if (s != FE_OK) {
+      j.err = host_error();
+      return s;
+    }
+    // processed entries per batch from the per-row skip counts
+    p.edges_eff_total = 0;
+    for (int b = 0; b < p.batches; ++b) {
+      const int64_t r0 = (int64_t)b * p.B, r1 = std::min<int64_t>(r0 + p.B, j.n);
+      int64_t eff = j.row_ptr[r1] - j.row_ptr[r0];
+      for (int64_t r = r0; r < r1; ++r) eff -= j.skip_deg[r];
+      p.edges_eff[b] = eff;
+      p.edges_eff_total += eff;
+    }
+    j.plan = std::move(p);
+    const size_t n_items = j.plan.items.size() / 4, n_splits = j.plan.splits.size() / 4;
+    if ((s = ensure_cap(j, j.items, j.cap_items, 4 * n_items, "work items")) != FE_OK) return s;
+    if ((s = ensure_cap(j, j.splits, j.cap_splits, 4 * std::max<size_t>(n_splits, 1),
+                        "split rows")) != FE_OK)
+      return s;
+    if ((s = ensure_cap(j, j.S, j.cap_S, (size_t)j.B * j.d, "staging rows")) != FE_OK) return s;
+    if ((s = ensure_cap(j, j.P, j.cap_P, (size_t)std::max(j.plan.max_pslots, 1) * j.d,
+                        "partials")) != FE_OK)
+      return s;
+    FE_HIP(j, hipMemcpyAsync(j.items, j.plan.items.data(), sizeof(int32_t) * j.plan.items.size(),
+                             hipMemcpyHostToDevice, st),
+           "work items copy");
+    if (n_splits)
+      FE_HIP(j, hipMemcpyAsync(j.splits, j.plan.splits.data(),
+                               sizeof(int32_t) * j.plan.splits.size(), hipMemcpyHostToDevice, st),
+             "split rows copy");
+    FE_HIP(j, hipStreamSynchronize(st), "plan upload sync");
+    j.plan_dirty = false;
+    j.steps_dirty = true;
+  }
+  if (j.steps_dirty) {
+    const int nb = j.plan.batches;
+    fe_status s;
+    if ((s = ensure_cap(j, j.steps, j.cap_steps, (size_t)nb, "step descriptors")) != FE_OK)
+      return s;
+    if ((s = ensure_cap(j, j.steps_delta, j.cap_steps_d, (size_t)nb, "step descriptors")) != FE_OK)
+      return s;
+    if ((s = ensure_cap(j, j.prefix, j.cap_prefix, (size_t)nb * 6, "block prefixes")) != FE_OK)
+      return s;
+    std::vector<StepDesc> hs(nb), hd(nb);
+    std::vector<int> pf((size_t)nb * 6, 0);
+    j.launches.assign(nb, StepLaunch());
+    const int ipb = j.tun.waves_per_block * j.tun.items_per_wave;
+    for (int b = 0; b < nb; ++b) {
+      hs[b] = make_desc(j, b, false);
+      hd[b] = make_desc(j, b, true);
+      StepLaunch &L = j.launches[b];
+      L.force_blocks = (hs[b].n_items + ipb - 1) / ipb;
+      L.fixup_blocks = (hs[b].n_splits + j.tun.waves_per_block - 1) / j.tun.waves_per_block;
+      const int64_t q4 = (int64_t)hs[b].rows * j.d / 4;
+      L.apply_blocks = (int)std::min<int64_t>((q4 + 255) / 256, 4096);
+      pf[6 * b + 1] = L.force_blocks;
+      pf[6 * b + 3] = L.fixup_blocks;
+      pf[6 * b + 5] = L.apply_blocks;
+    }
+    FE_HIP(j, hipMemcpyAsync(j.steps, hs.data(), sizeof(StepDesc) * nb, hipMemcpyHostToDevice, st),
+           "descriptor copy");
+    FE_HIP(j, hipMemcpyAsync(j.steps_delta, hd.data(), sizeof(StepDesc) * nb,
+                             hipMemcpyHostToDevice, st),
+           "descriptor copy");
+    FE_HIP(j, hipMemcpyAsync(j.prefix, pf.data(), sizeof(int) * pf.size(), hipMemcpyHostToDevice,
+                             st),
+           "prefix copy");
+    FE_HIP(j, hipStreamSynchronize(st), "descriptor sync");
+    j.host_steps.swap(hs);
+    j.steps_dirty = false;
+  }
+  return FE_OK;
+}
+
+fe_status job_launch_batch(Job &j, int b, bool delta_mode, hipStream_t st, hipEvent_t *ev) {
+  const StepLaunch &L = j.launches[b];
+  const StepDesc *sd = (delta_mode ? j.steps_delta : j.steps) + b;
+  const int *pf = j.prefix + 6 * b;
+  const size_t lds = (size_t)j.ns * j.d * sizeof(float);
+  if (ev) FE_HIP(j, hipEventRecord(ev[0], st), "event");
+  FE_HIP(j, launch_force(sd, pf, 1, L.force_blocks, j.tun.waves_per_block, j.tun.items_per_wave,
+                         j.tun.unroll, lds, st),
+         "force kernel launch");
+  if (ev) FE_HIP(j, hipEventRecord(ev[1], st), "event");
+  if (L.fixup_blocks > 0)
+    FE_HIP(j, launch_fixup(sd, pf + 2, 1, L.fixup_blocks, j.tun.waves_per_block, 1, j.tun.unroll,
+                           lds, st),
+           "fixup kernel launch");
+  if (ev) FE_HIP(j, hipEventRecord(ev[2], st), "event");
+  if (!delta_mode) FE_HIP(j, launch_apply(sd, pf + 4, 1, L.apply_blocks, st), "apply kernel launch");
+  if (ev) FE_HIP(j, hipEventRecord(ev[3], st), "event");
+  return FE_OK;
+}
+
+fe_status job_loss(Job &j, uint64_t seed, int eval_ns, double *attr, double *rep,
+                   hipStream_t st) {
+  if (eval_ns < 0 || eval_ns >= 1048576 || !attr || !rep) {
+    j.err = "fe_loss: invalid argument";
+    return FE_ERR_INVALID_ARG;
+  }
+  LossDesc ld;
+  std::memset(&ld, 0, sizeof ld);
+  ld.X = j.X;
+  ld.col = j.col;
+  ld.items = j.items;
+  ld.n_items = (int)(j.plan.items.size() / 4);
+  ld.D = j.d;
+  ld.G = j.G;
+  ld.c_limit = (int)j.c_limit;
+  ld.n = j.n;
+  ld.eval_seed = seed;
+  ld.eval_ns = eval_ns;
+  const int ipw = 4, wpb = 4;
+  const int eblocks = std::max(1, (ld.n_items + ipw * wpb - 1) / (ipw * wpb));
+  const int NG = 64 / j.G;
+  const int ppg = 8;
+  const int64_t pairs = j.n * (int64_t)eval_ns;
+  const int64_t per_block = (int64_t)wpb * NG * ppg;
+  const int nblocks = (int)std::max<int64_t>(1, (pairs + per_block - 1) / per_block);
+  double *buf = nullptr;
+  FE_HIP(j, hipMalloc(&buf, sizeof(double) * ((size_t)eblocks + nblocks + 2)), "loss buffer");
+  fe_status s = FE_OK;
+  double out[2] = {0.0, 0.0};
+  do {
+    if ((s = job_check(j, launch_loss_edges(ld, buf + 2, eblocks, ipw, st), "loss edges")) != FE_OK)
+      break;
+    if ((s = job_check(j, launch_loss_negs(ld, buf + 2 + eblocks, nblocks, ppg, st),
+                       "loss negatives")) != FE_OK)
+      break;
+    if ((s = job_check(j, launch_reduce(buf + 2, eblocks, buf, st), "reduce")) != FE_OK) break;
+    if ((s = job_check(j, launch_reduce(buf + 2 + eblocks, nblocks, buf + 1, st), "reduce")) !=
+        FE_OK)
+      break;
+    if ((s = job_check(j, hipMemcpyAsync(out, buf, sizeof out, hipMemcpyDeviceToHost, st),
+                       "loss copy")) != FE_OK)
+      break;
+    s = job_check(j, hipStreamSynchronize(st), "loss sync");
+  } while (0);
+  hipFree(buf);
+  if (s != FE_OK) return s;
+  *attr = out[0];
+  *rep = eval_ns > 0 ? out[1] : 0.0;
+  if (!std::isfinite(out[0]) || !std::isfinite(*rep)) {
+    j.err = "fe_loss: the loss is not finite (NaN or Inf in X)";
+    return FE_ERR_NOT_FINITE;
+  }
+  return FE_OK;
+}
+
+int64_t job_bytes_estimate(const Job &j) {
+  const int64_t nd = j.n * (int64_t)j.d;
+  int64_t b = nd * 4 + (j.dtype == FE_GATHER_BF16 ? nd * 2 : 0);
+  b += std::max<int64_t>(j.nnz, 1) * 4;
+  b += (int64_t)j.plan.items.size() * 4 + (int64_t)std::max<size_t>(j.plan.splits.size(), 4) * 4;
+  b += (int64_t)j.plan.B * j.d * 4 + (int64_t)std::max(j.plan.max_pslots, 1) * j.d * 4;
+  b += (int64_t)j.n_lists * j.ns * 4;
+  b += (int64_t)j.plan.batches * (2 * (int64_t)sizeof(StepDesc) + 24) + 4;
+  return b;
+}
+
+} // namespace fe

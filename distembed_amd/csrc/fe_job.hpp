@@ -1,0 +1,111 @@
+// Device-resident state of one embedding job (one graph) and the host code
+// that fills it. Shared by the single-graph context (fe_capi.cpp) and the
+// multi-job engine (fe_engine.cpp).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "fe_device.hpp"
+#include "fe_host.hpp"
+
+namespace fe {
+
+// Launch wrappers (fe_kernels.hip).
+hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
+                        int wpb, int ipw, int unroll, size_t lds_bytes, hipStream_t st);
+hipError_t launch_fixup(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
+                        int wpb, int ipw, int unroll, size_t lds_bytes, hipStream_t st);
+hipError_t launch_apply(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
+                        hipStream_t st);
+hipError_t launch_set_int(int *p, int v, hipStream_t st);
+hipError_t launch_to_bf16(const float *x, uint16_t *xs, int64_t count, hipStream_t st);
+hipError_t launch_loss_edges(const LossDesc &ld, double *block_out, int n_blocks, int ipw,
+                             hipStream_t st);
+hipError_t launch_loss_negs(const LossDesc &ld, double *block_out, int n_blocks,
+                            int pairs_per_group, hipStream_t st);
+hipError_t launch_reduce(const double *in, int n, double *out, hipStream_t st);
+
+struct Tuning {
+  int chunk = 32;
+  int max_chunks = 256;
+  int unroll = 4;
+  int waves_per_block = 4;
+  int items_per_wave = 1;
+};
+
+// Block counts of the three kernels for one step.
+struct StepLaunch {
+  int force_blocks = 0;
+  int fixup_blocks = 0;
+  int apply_blocks = 0;
+};
+
+struct Job {
+  // shape
+  int64_t n = 0;
+  int d = 0;
+  int G = 0;
+  int dtype = FE_GATHER_FP32;
+  uint32_t quirks = FE_QUIRKS_REFERENCE;
+  int64_t c_limit = 0;
+  // host copies
+  std::vector<int64_t> row_ptr;
+  std::vector<int32_t> skip_deg; // per row: entries with c >= c_limit
+  int64_t nnz = -1;
+  bool x_ready = false;
+  // configuration
+  int B = 0;
+  float lr = 0.02f;
+  float bound = 5.0f;
+  int ns = 0;
+  int n_lists = 0;
+  Tuning tun;
+  Plan plan;
+  std::vector<StepLaunch> launches; // per batch
+  // device buffers
+  float *X = nullptr;
+  uint16_t *Xs = nullptr;
+  int32_t *col = nullptr;
+  int32_t *items = nullptr;
+  int32_t *splits = nullptr;
+  float *S = nullptr;
+  float *P = nullptr;
+  int32_t *negs = nullptr;
+  int *epoch = nullptr;
+  StepDesc *steps = nullptr;      // per batch (normal mode)
+  StepDesc *steps_delta = nullptr; // per batch (delta mode)
+  int *prefix = nullptr;          // per batch: 3 kernels x {0, blocks}
+  std::vector<StepDesc> host_steps;
+  size_t cap_items = 0, cap_splits = 0, cap_S = 0, cap_P = 0, cap_negs = 0, cap_steps = 0;
+  size_t cap_steps_d = 0, cap_prefix = 0, cap_col = 0;
+  bool plan_dirty = true;  // plan must be rebuilt (batch size, chunking, CSR)
+  bool steps_dirty = true; // descriptors must be rebuilt
+  int64_t device_bytes = 0;
+  std::string err;
+};
+
+// All return FE_OK or an error with job.err set. `st` is the stream used.
+fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks);
+void job_free(Job &j);
+fe_status job_upload_csr(Job &j, const int64_t *row_ptr, const int32_t *col, int64_t nnz,
+                         hipStream_t st);
+fe_status job_upload_x(Job &j, const float *x, hipStream_t st);
+fe_status job_download_x(Job &j, float *x, hipStream_t st);
+fe_status job_set_negatives(Job &j, const int32_t *lists, int n_lists, int ns, hipStream_t st);
+fe_status job_configure(Job &j, int B, float lr, float bound);
+fe_status job_set_tuning(Job &j, const Tuning &t);
+// Builds the plan / descriptors if dirty and uploads them.
+fe_status job_prepare(Job &j, hipStream_t st);
+// Launches batch `b` (force, fixup, apply) on `st`; the epoch counter must
+// already hold the epoch. delta_mode: skip apply, S holds the delta.
+fe_status job_launch_batch(Job &j, int b, bool delta_mode, hipStream_t st,
+                           hipEvent_t *ev /* 4 events or null */);
+fe_status job_loss(Job &j, uint64_t seed, int eval_ns, double *attr, double *rep, hipStream_t st);
+// Device bytes the job holds once prepared (plan must be built).
+int64_t job_bytes_estimate(const Job &j);
+fe_status job_check(Job &j, hipError_t e, const char *what);
+int rows_of_batch(const Job &j, int b);
+
+} // namespace fe

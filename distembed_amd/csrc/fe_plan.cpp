@@ -1,0 +1,75 @@
+// Work-plan construction (host). See Plan in fe_host.hpp.
+#include <algorithm>
+
+#include "fe_host.hpp"
+
+namespace fe {
+
+fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int B, int chunk,
+                     int max_chunks, int64_t c_limit, Plan *out) {
+  if (n <= 0 || !row_ptr || B <= 0 || chunk <= 0 || max_chunks <= 0 || !out) {
+    set_host_error("build_plan: invalid argument");
+    return FE_ERR_INVALID_ARG;
+  }
+  Plan p;
+  p.B = B;
+  p.chunk = chunk;
+  p.max_chunks = max_chunks;
+  // cpp/algo/algo.hpp:80-88 with proc_row_width = n at world size 1.
+  if (n % B == 0) {
+    p.batches = (int)(n / B);
+    p.last = B;
+  } else {
+    p.batches = (int)(n / B) + 1;
+    p.last = (int)(n - (int64_t)B * (p.batches - 1));
+  }
+  p.item_off.assign(p.batches + 1, 0);
+  p.split_off.assign(p.batches + 1, 0);
+  p.pslots.assign(p.batches, 0);
+  p.edges_eff.assign(p.batches, 0);
+  for (int j = 0; j < p.batches; ++j) {
+    const int64_t r0 = (int64_t)j * B;
+    const int64_t r1 = std::min<int64_t>(r0 + B, n);
+    int32_t pcount = 0;
+    int64_t eff = 0;
+    for (int64_t r = r0; r < r1; ++r) {
+      const int64_t e0 = row_ptr[r];
+      const int64_t deg = row_ptr[r + 1] - e0;
+      if (col) {
+        for (int64_t e = e0; e < e0 + deg; ++e) eff += (col[e] < c_limit) ? 1 : 0;
+      }
+      if (deg <= chunk) {
+        p.items.insert(p.items.end(), {(int32_t)r, (int32_t)e0, (int32_t)deg, -1});
+        continue;
+      }
+      const int64_t per = std::max<int64_t>(chunk, (deg + max_chunks - 1) / max_chunks);
+      const int64_t k = (deg + per - 1) / per;
+      for (int64_t i = 0; i < k; ++i) {
+        const int64_t lo = deg * i / k, hi = deg * (i + 1) / k;
+        p.items.insert(p.items.end(),
+                       {(int32_t)r, (int32_t)(e0 + lo), (int32_t)(hi - lo), (int32_t)(pcount + i)});
+      }
+      p.splits.insert(p.splits.end(), {(int32_t)r, pcount, (int32_t)k, 0});
+      pcount += (int32_t)k;
+    }
+    p.item_off[j + 1] = (int64_t)p.items.size() / 4;
+    p.split_off[j + 1] = (int64_t)p.splits.size() / 4;
+    p.pslots[j] = pcount;
+    p.max_pslots = std::max(p.max_pslots, (int)pcount);
+    p.edges_eff[j] = eff;
+    p.edges_eff_total += eff;
+  }
+  *out = std::move(p);
+  return FE_OK;
+}
+
+void rep_rows(const Plan &p, uint32_t quirks, int j, int *rep0, int *rep1) {
+  // considering_batch_size starts at B and becomes `last` at the final batch
+  // of every epoch; without the reset quirk it is B for the other batches.
+  const bool final_batch = (j == p.batches - 1);
+  *rep0 = final_batch ? p.last : p.B;
+  if (final_batch) *rep1 = p.last;
+  else *rep1 = (quirks & FE_QUIRK_STICKY_REP) ? p.last : p.B;
+}
+
+} // namespace fe

@@ -1,0 +1,206 @@
+/*
+ * force_embed.h -- C ABI of the MI355X force-model graph-embedding hot path.
+ *
+ * The reference exposes this path only as a header-only C++ class template,
+ *   EmbeddingAlgo<INDEX_TYPE, VALUE_TYPE, embedding_dim>
+ *     ctor                       cpp/algo/algo.hpp:54-63
+ *     algo_force2_vec_ns(...)    cpp/algo/algo.hpp:74-262
+ * called once from the driver (cpp/dist_embed.cpp:330-347), which then reads
+ * DenseMat::nCoordinates back (cpp/dist_embed.cpp:346). There is no FFI in the
+ * reference; the entry points below are what such an FFI would bind. Every
+ * function returns an fe_status; library code never aborts. Host pointers are
+ * borrowed for the duration of a call only; the context owns all device memory.
+ *
+ * Threading: calls on one context are serialised by the caller. Different
+ * contexts (or engines) may be used from different threads.
+ *
+ * The C++ mirror of the reference class (same constructor and method names and
+ * argument order) is include/distembed_amd/embedding_algo.hpp.
+ */
+#ifndef FORCE_EMBED_H
+#define FORCE_EMBED_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int fe_status;
+
+enum {
+  FE_OK = 0,
+  FE_ERR_INVALID_ARG = 1, /* bad size, null pointer, index out of range      */
+  FE_ERR_OOM = 2,         /* device or host allocation refused               */
+  FE_ERR_HIP = 3,         /* a HIP runtime call failed                       */
+  FE_ERR_OVER_BUDGET = 4, /* engine admission: job exceeds the HBM budget    */
+  FE_ERR_STATE = 5,       /* call out of order (e.g. step before upload)     */
+  FE_ERR_NO_DEVICE = 6,   /* no HIP device / device index out of range       */
+  FE_ERR_TIMEOUT = 7,     /* a bounded device-side wait expired              */
+  FE_ERR_NOT_FINITE = 8   /* loss reduction found a NaN or Inf               */
+};
+
+/* Dtype of the rows the gathers read. FP32: read the FP32 master copy.
+ * BF16: read a BF16 shadow copy, accumulate in FP32, update the FP32 master
+ * and refresh the shadow rows of each batch after its update.              */
+enum { FE_GATHER_FP32 = 0, FE_GATHER_BF16 = 1 };
+
+/* Behavioural quirks of the reference, reproduced by default.
+ * STICKY_REP   cpp/algo/algo.hpp:135,167-169 -- the repulsive row count is set
+ *              to the last batch's size at the final batch and never reset, so
+ *              from the second epoch on every batch repels only its first
+ *              `last` rows.
+ * SKIP_LAST_COL cpp/algo/algo.hpp:401-405,538 -- attractive neighbours are
+ *              accepted with `c < N-1`, so edges into vertex N-1 are skipped. */
+enum {
+  FE_QUIRK_STICKY_REP = 1u,
+  FE_QUIRK_SKIP_LAST_COL = 2u,
+  FE_QUIRKS_REFERENCE = 3u
+};
+
+/* How the reference-mode sampler treats a draw equal to N. The reference draws
+ * from the inclusive range [0, N] (cpp/core/common.cpp:44) and an id N then
+ * indexes past the end of its row cache (cpp/algo/algo.hpp:605-617), which is
+ * undefined behaviour. STRICT reports FE_ERR_INVALID_ARG if any list holds N;
+ * REDRAW keeps drawing from the same engine until the value is below N.   */
+enum { FE_NEG_STRICT = 0, FE_NEG_REDRAW = 1 };
+
+/* Tiles chosen by the autotuner. A zero field keeps the current value.
+ * chunk and max_chunks change the split of heavy rows and therefore the
+ * floating-point summation order; unroll and waves_per_block do not.      */
+typedef struct fe_tuning {
+  int chunk;           /* edges per work item for ordinary rows (<= 64)     */
+  int max_chunks;      /* most work items (partials) one heavy row may use  */
+  int unroll;          /* rows in flight per lane group: 1, 2, 4 or 8       */
+  int waves_per_block; /* 1, 2, 4 or 8                                      */
+  int items_per_wave;  /* work items a wave takes in turn (>= 1)            */
+} fe_tuning;
+
+typedef struct fe_ctx fe_ctx;
+
+/* ---- lifecycle -------------------------------------------------------- */
+
+/* Creates a context on HIP device `device` for an N x d embedding.
+ * d must be a multiple of 8 in [8, 512]; 0 < n < 2^31.                     */
+fe_status fe_create(fe_ctx **out, int device, int64_t n, int d,
+                    int gather_dtype, uint32_t quirks);
+void fe_destroy(fe_ctx *ctx);
+/* Message of the last failure on `ctx` (or of the last failed fe_create when
+ * ctx is NULL). Never NULL.                                                */
+const char *fe_last_error(const fe_ctx *ctx);
+
+/* ---- data (replaces SpMat/CSRLocal native CSR, cpp/core/csr_local.hpp:37-125,
+ *      and DenseMat::nCoordinates, cpp/core/dense_mat.hpp:56-64) ---------- */
+
+/* CSR of the local graph: row_ptr[n+1] (monotone, row_ptr[0] = 0,
+ * row_ptr[n] = nnz < 2^31), col_idx[nnz] in [0, n).                        */
+fe_status fe_upload_csr(fe_ctx *ctx, const int64_t *row_ptr,
+                        const int32_t *col_idx, int64_t nnz);
+/* Row-major n x d FP32 embedding (also refreshes the BF16 shadow).        */
+fe_status fe_upload_x(fe_ctx *ctx, const float *x);
+fe_status fe_download_x(fe_ctx *ctx, float *x);
+
+/* Negative lists: list l (l = epoch + batch, cpp/algo/algo.hpp:165) is
+ * lists[l*ns .. l*ns+ns-1]; every id must be in [0, n).                    */
+fe_status fe_set_negatives(fe_ctx *ctx, const int32_t *lists, int n_lists,
+                           int ns);
+
+/* Batch size, learning rate and clamp bound (MAX_BOUND, passed as 5 at
+ * cpp/dist_embed.cpp:341). Builds the per-batch work plan on first use or
+ * when the batch size or tiling changes.                                   */
+fe_status fe_configure(fe_ctx *ctx, int batch_size, float lr, float max_bound);
+fe_status fe_set_tuning(fe_ctx *ctx, const fe_tuning *t);
+fe_status fe_get_tuning(const fe_ctx *ctx, fe_tuning *t);
+
+/* ---- the hot path ------------------------------------------------------ */
+
+/* One minibatch (epoch i, batch j) of cpp/algo/algo.hpp:165-192: attractive
+ * forces over the CSR rows of the batch (:520-584), repulsive forces against
+ * negative list i+j (:586-649), then X[batch] += delta (:651-666).         */
+fe_status fe_step(fe_ctx *ctx, int epoch, int batch);
+/* Same forces as fe_step but X is left unchanged: the rows x d delta of the
+ * batch is copied to delta_out (host). rows = min(B, n - batch*B).        */
+fe_status fe_step_delta(fe_ctx *ctx, int epoch, int batch, float *delta_out);
+/* algo_force2_vec_ns(iterations, batch_size, ns, lr) (cpp/algo/algo.hpp:74).
+ * Uses the lists set with fe_set_negatives when they cover
+ * iterations + batches - 1 lists with `ns` ids each; otherwise generates the
+ * reference lists (fe_reference_negatives, FE_NEG_REDRAW). use_graph != 0
+ * replays one captured HIP graph per epoch.                                */
+fe_status fe_train(fe_ctx *ctx, int iterations, int batch_size, int ns,
+                   float lr, int use_graph);
+fe_status fe_synchronize(fe_ctx *ctx);
+
+/* Force-model loss at the current X, reduced in FP64 on the device:
+ *   attr = sum over CSR entries (r,c) taken by the attractive kernel of
+ *          log(1 + |x_r - x_c|^2)
+ *   rep  = 1/(1-1e-6) * sum over r in [0,n), k in [0,eval_ns) of
+ *          log((1 + s) / (s + 1e-6)),  s = |x_r - x_m|^2, m = fe_eval_negative
+ * Its negative gradient in x_r is the unclipped update of
+ * cpp/algo/algo.hpp:573 and :641. The reference computes no loss.          */
+fe_status fe_loss(fe_ctx *ctx, uint64_t eval_seed, int eval_ns,
+                  double *attr, double *rep);
+
+/* Work-unit counts of one epoch at the configured batch size: processed CSR
+ * entries, (row, negative) pairs in the first epoch and in later epochs.   */
+fe_status fe_epoch_units(const fe_ctx *ctx, int ns, int64_t *edges,
+                         int64_t *negs_epoch0, int64_t *negs_later);
+/* Average duration (ms) of the force kernel over the launches timed since
+ * the last reset, measured with HIP events on the context's stream.
+ * Timing is enabled with fe_set_timing(ctx, 1).                            */
+fe_status fe_set_timing(fe_ctx *ctx, int enable);
+fe_status fe_kernel_times(fe_ctx *ctx, double *force_ms, double *fixup_ms,
+                          double *apply_ms, int64_t *launches);
+
+/* ---- host helpers (no device needed) ---------------------------------- */
+
+/* generate_random_numbers(0, n, seed, ns) (cpp/core/common.cpp:37-53): a
+ * minimal-standard LCG (multiplier 48271, modulus 2^31-1) seeded with seed,
+ * downscaled to the inclusive range [0, n] as libstdc++'s
+ * uniform_int_distribution<int> does. Writes lists l = 0..n_lists-1 with
+ * seed l into out[l*ns ...]. Returns the number of draws equal to n (before
+ * redraw) in *n_hits when n_hits is not NULL.                              */
+fe_status fe_reference_negatives(int64_t n, int n_lists, int ns, int policy,
+                                 int32_t *out, int64_t *n_hits);
+/* Evaluation negative k of row r for the loss: splitmix64 of
+ * (seed, r * 2^20 + k) reduced modulo n (a definition of this library).   */
+int64_t fe_eval_negative(uint64_t seed, int64_t r, int k, int64_t n);
+
+/* Graph generators (deterministic in seed, independent of thread count).
+ * They return a symmetric CSR with sorted, de-duplicated columns and no
+ * self-loops. Arrays are malloc'ed by the library and released with
+ * fe_free_graph. Vertex order is the generator's natural order.           */
+typedef struct fe_graph {
+  int64_t n;
+  int64_t nnz;
+  int64_t *row_ptr; /* n + 1 */
+  int32_t *col_idx; /* nnz   */
+} fe_graph;
+fe_status fe_gen_rmat(int scale, int edge_factor, double a, double b, double c,
+                      uint64_t seed, fe_graph *out);
+/* Stochastic block model: n vertices in `blocks` near-equal contiguous
+ * blocks, pairs inside a block joined with p_in, across blocks with p_out. */
+fe_status fe_gen_sbm(int64_t n, int blocks, double p_in, double p_out,
+                     uint64_t seed, fe_graph *out);
+/* CSR from an edge list (u[i], v[i]) in [0, n): symmetrised, sorted,
+ * duplicates merged, self-loops dropped.                                   */
+fe_status fe_csr_from_edges(int64_t n, const int64_t *u, const int64_t *v,
+                            int64_t m, fe_graph *out);
+void fe_free_graph(fe_graph *g);
+/* X0 ~ U(lo, hi) from a counter-based generator (seed, element index).    */
+fe_status fe_init_uniform(float *x, int64_t count, float lo, float hi,
+                          uint64_t seed);
+
+/* MatrixMarket ingest (cpp/io/parrallel_IO.hpp:31-88): square coordinate
+ * matrix, 1-based ids; `symmetric` files are expanded to both directions,
+ * `general` entries are kept as given; columns sorted, duplicates merged,
+ * values ignored, self-loops kept (they add a zero force).                 */
+fe_status fe_read_mtx(const char *path, fe_graph *out);
+/* Embedding writer (cpp/io/parrallel_IO.hpp:90-146): one line per row,
+ * "%lu" 1-based id then " %.5f" per component, no header.                  */
+fe_status fe_write_embedding(const char *path, const float *x, int64_t n,
+                             int d);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FORCE_EMBED_H */

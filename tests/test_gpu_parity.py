@@ -1,0 +1,246 @@
+"""GPU parity of the HIP force step against the oracle (through the C ABI).
+
+Tolerance (FP32 path): per batch, on identical X and identical negative
+lists, the updated rows X' = X + delta match the FP32 oracle with
+|X'_gpu - X'_ref| <= 1e-5 * |X'_ref| + 1e-5 * rms(X'_ref), and the batch
+delta matches normwise, ||delta_gpu - delta_ref|| <= 1e-5 ||delta_ref||.
+The GPU reduces |v|^2 with a DPP tree and sums a row's edges in a fixed
+interleaved order; the reference sums serially, so results agree to
+rounding, not bitwise. Every GPU result is bitwise reproducible.
+"""
+import numpy as np
+import pytest
+
+import _oracle as orc
+from distembed_amd import api
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-5
+
+
+def _ctx(g, d, X, B, lists, gather=api.FP32, quirks=3, lr=0.02, tuning=None):
+    ctx = api.ForceEmbed(g.n, d, gather=gather, quirks=quirks)
+    ctx.upload_csr(g)
+    ctx.upload_x(X)
+    if lists is not None:
+        ctx.set_negatives(lists)
+    if tuning:
+        ctx.set_tuning(**tuning)
+    ctx.configure(B, lr, 5.0)
+    return ctx
+
+
+def _rows(n, B, j):
+    return min((j + 1) * B, n) - j * B
+
+
+def _check_rows(new_gpu, new_ref, delta_gpu, delta_ref, what=""):
+    scale = float(np.sqrt(np.mean(new_ref.astype(np.float64) ** 2)))
+    err = np.abs(new_gpu.astype(np.float64) - new_ref)
+    bound = RTOL * np.abs(new_ref) + RTOL * scale
+    assert np.all(err <= bound), f"{what}: max excess {(err - bound).max():.3e}"
+    num = np.linalg.norm((delta_gpu - delta_ref).astype(np.float64))
+    den = np.linalg.norm(delta_ref.astype(np.float64))
+    assert num <= RTOL * den + 1e-30, f"{what}: normwise delta error {num / den:.3e}"
+
+
+def _batch_parity(g, d, B, steps, ns=5, quirks=3, tuning=None, seed=0, lr=0.02):
+    X = api.init_uniform(g.n, d, -0.5, 0.5, seed=seed)
+    batches = -(-g.n // B)
+    n_lists = max(e for e, _ in steps) + batches
+    lists = api.reference_negatives(g.n, n_lists, ns)[0] if ns else None
+    ctx = _ctx(g, d, X, B, lists, quirks=quirks, tuning=tuning, lr=lr)
+    try:
+        for (epoch, j) in steps:
+            rows = _rows(g.n, B, j)
+            dg = ctx.step_delta(epoch, j, rows)
+            dr = orc.step(g, X.copy(), B, epoch, j, lists, lr=lr, quirks=quirks, apply=False,
+                          want_delta=True)
+            r0 = j * B
+            _check_rows(X[r0:r0 + rows] + dg, X[r0:r0 + rows] + dr, dg, dr,
+                        f"d={d} B={B} epoch={epoch} batch={j}")
+            # the applied step writes exactly X + delta into the batch rows
+            ctx.step(epoch, j)
+            Xg = ctx.download_x()
+            np.testing.assert_array_equal(Xg[r0:r0 + rows], X[r0:r0 + rows] + dg)
+            np.testing.assert_array_equal(np.delete(Xg, np.s_[r0:r0 + rows], 0),
+                                          np.delete(X, np.s_[r0:r0 + rows], 0))
+            ctx.upload_x(X)
+    finally:
+        ctx.close()
+
+
+@pytest.mark.parametrize("d", [128, 64, 32])
+def test_sbm_batches(d):
+    g = api.Graph.sbm(2000, 10, 18 / 199, 2 / 1800, seed=0)
+    # B=256: 8 batches, the last of 208 rows (sticky repulsive count active)
+    _batch_parity(g, d, 256, [(0, 0), (0, 3), (0, 7), (1, 0), (1, 7), (4, 2)])
+
+
+@pytest.mark.parametrize("d", [8, 16, 256])
+def test_other_widths(d):
+    g = api.Graph.sbm(700, 7, 0.05, 0.004, seed=3)
+    _batch_parity(g, d, 128, [(0, 0), (2, 5)], ns=5 if d <= 128 else 3)
+
+
+def test_hub_rows_split_into_chunks():
+    g = api.Graph.rmat(12, 16, seed=5)
+    # small chunks force heavy rows through partials + the fixup kernel
+    for tun in ({"chunk": 4, "max_chunks": 8}, {"chunk": 32, "max_chunks": 256},
+                {"chunk": 64, "max_chunks": 1}):
+        _batch_parity(g, 128, 512, [(0, 0), (0, 1), (3, 7)], tuning=tun)
+
+
+def test_quirks_off_and_no_negatives():
+    g = api.Graph.sbm(1000, 4, 0.02, 0.002, seed=9)
+    _batch_parity(g, 64, 300, [(0, 0), (2, 3)], quirks=0)
+    _batch_parity(g, 64, 300, [(0, 0), (2, 3)], ns=0)
+    _batch_parity(g, 64, 300, [(1, 3)], ns=10)
+
+
+def test_edges_into_last_vertex_skipped():
+    n = 64
+    u = np.arange(n - 1)
+    v = np.full(n - 1, n - 1)
+    g = api.Graph.from_edges(n, u, v)  # star around vertex n-1
+    _batch_parity(g, 32, 16, [(0, 0), (0, 3)], quirks=3)
+    _batch_parity(g, 32, 16, [(0, 0), (0, 3)], quirks=0)
+
+
+def test_isolated_and_empty_batches():
+    n = 300
+    g = api.Graph.from_edges(n, np.array([0, 5]), np.array([1, 6]))  # nearly all rows empty
+    _batch_parity(g, 64, 64, [(0, 0), (0, 4), (2, 2)])
+
+
+def test_tuning_does_not_change_results():
+    g = api.Graph.rmat(11, 16, seed=2)
+    X = api.init_uniform(g.n, 128, seed=1)
+    lists = api.reference_negatives(g.n, 20, 5)[0]
+    outs = []
+    for tun in ({"unroll": 1, "waves_per_block": 1, "items_per_wave": 1},
+                {"unroll": 4, "waves_per_block": 4, "items_per_wave": 2},
+                {"unroll": 8, "waves_per_block": 8, "items_per_wave": 4},
+                {"unroll": 2, "waves_per_block": 2, "items_per_wave": 16}):
+        ctx = _ctx(g, 128, X, 256, lists, tuning=tun)
+        ctx.train(3, 256, 5, 0.02, use_graph=False)
+        outs.append(ctx.download_x())
+        ctx.close()
+    for o in outs[1:]:
+        np.testing.assert_array_equal(o, outs[0])
+
+
+def test_graph_replay_equals_stream_launches_and_steps():
+    g = api.Graph.sbm(1500, 6, 0.02, 0.001, seed=4)
+    X = api.init_uniform(g.n, 64, seed=8)
+    B, T = 200, 4
+    lists = api.reference_negatives(g.n, T + 7, 5)[0]
+    res = []
+    for mode in ("graph", "stream", "steps"):
+        ctx = _ctx(g, 64, X, B, lists)
+        if mode == "steps":
+            for i in range(T):
+                for j in range(8):
+                    ctx.step(i, j)
+        else:
+            ctx.train(T, B, 5, 0.02, use_graph=(mode == "graph"))
+        res.append(ctx.download_x())
+        ctx.close()
+    np.testing.assert_array_equal(res[0], res[1])
+    np.testing.assert_array_equal(res[0], res[2])
+
+
+def test_training_tracks_oracle():
+    g = api.Graph.sbm(1000, 10, 18 / 99, 2 / 900, seed=0)
+    X = api.init_uniform(g.n, 128, seed=0)
+    B, T = 256, 10
+    lists = api.reference_negatives(g.n, T + 3, 5)[0]
+    ctx = _ctx(g, 128, X, B, lists)
+    ctx.train(T, B, 5, 0.02)
+    Xg = ctx.download_x()
+    ctx.close()
+    Xr = X.copy()
+    orc.train(g, Xr, B, T, lists)
+    rel = np.abs(Xg - Xr).max() / np.abs(Xr).max()
+    assert rel < 1e-4, rel
+
+
+def test_loss_matches_fp64_oracle():
+    g = api.Graph.rmat(12, 8, seed=1)
+    X = api.init_uniform(g.n, 128, seed=3)
+    ctx = _ctx(g, 128, X, 1024, None)
+    a, r = ctx.loss(eval_seed=7, eval_ns=5)
+    ctx.close()
+    ar, rr = orc.loss(g, X.astype(np.float64), seed=7, eval_ns=5)
+    assert a == pytest.approx(ar, rel=1e-6)
+    assert r == pytest.approx(rr, rel=1e-6)
+
+
+def test_nan_in_x_reported_by_loss():
+    g = api.Graph.sbm(200, 2, 0.05, 0.01, seed=1)
+    X = api.init_uniform(g.n, 32, seed=1)
+    X[17, 3] = np.nan
+    ctx = _ctx(g, 32, X, 64, None)
+    with pytest.raises(api.FeError) as e:
+        ctx.loss(0, 5)
+    assert e.value.status == 8
+    ctx.close()
+
+
+def test_bad_arguments_return_errors():
+    g = api.Graph.sbm(100, 2, 0.05, 0.01, seed=1)
+    ctx = api.ForceEmbed(100, 32)
+    with pytest.raises(api.FeError):
+        ctx.step(0, 0)  # nothing uploaded
+    ctx.upload_csr(g)
+    ctx.upload_x(api.init_uniform(100, 32))
+    with pytest.raises(api.FeError):
+        ctx.set_negatives(np.array([[100]], dtype=np.int32))
+    with pytest.raises(api.FeError):
+        ctx.configure(0)
+    ctx.configure(32)
+    ctx.set_negatives(np.zeros((2, 3), dtype=np.int32))
+    with pytest.raises(api.FeError):
+        ctx.step(5, 1)  # list 6 does not exist
+    with pytest.raises(api.FeError):
+        ctx.step(0, 4)  # batch out of range
+    ctx.close()
+    with pytest.raises(api.FeError):
+        api.ForceEmbed(100, 24)  # d/8 not a power of two
+
+
+# ---- full-size workload (R-MAT scale 20, d = 128, B = 16384) ----------------------
+
+@pytest.fixture(scope="module")
+def rmat20():
+    return api.Graph.rmat(20, 16, 0.57, 0.19, 0.19, seed=0)
+
+
+def test_rmat20_batches_match_oracle(rmat20):
+    g = rmat20
+    # batch 0 holds the hub rows (max degree ~6.5e4), batch 63 the tail
+    _batch_parity(g, 128, 16384, [(0, 0), (0, 1), (0, 63), (1, 17)])
+
+
+def test_rmat20_epoch_properties(rmat20):
+    g = rmat20
+    d, B = 128, 16384
+    X = api.init_uniform(g.n, d, seed=0)
+    lists = api.reference_negatives(g.n, 2 + 63, 5)[0]
+    ctx = _ctx(g, d, X, B, lists)
+    e, n0, n1 = ctx.epoch_units(5)
+    assert e == int((g.col_idx < g.n - 1).sum())
+    assert n0 == n1 == g.n * 5
+    ctx.train(2, B, 5, 0.02, use_graph=True)
+    X1 = ctx.download_x()
+    ctx.upload_x(X)
+    ctx.train(2, B, 5, 0.02, use_graph=True)
+    X2 = ctx.download_x()
+    ctx.close()
+    np.testing.assert_array_equal(X1, X2)  # bitwise reproducible
+    assert np.isfinite(X1).all()
+    # every row moves by at most 2 epochs x (deg + 5) x lr x bound per component
+    deg = np.diff(g.row_ptr)
+    step_max = 2 * (deg + 5) * 0.02 * 5.0
+    assert np.all(np.abs(X1 - X).max(axis=1) <= step_max + 1e-6)

@@ -1,0 +1,141 @@
+"""Host logic of the library (no GPU): C ABI exports, generators, CSR
+construction, MatrixMarket ingest, the embedding writer, argument checks."""
+import ctypes as C
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from distembed_amd import _lib, api
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def _header_functions():
+    names = []
+    for h in sorted((ROOT / "include").glob("*.h")):
+        text = h.read_text()
+        names += re.findall(r"^\s*(?:fe_status|void|const char \*|int64_t)\s+\**(fe_\w+)\s*\(",
+                            text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol(fe_lib):
+    names = _header_functions()
+    assert len(names) >= 25
+    missing = [n for n in names if not hasattr(fe_lib, n)]
+    assert not missing, missing
+    # and the Python binding declares a signature for each of them
+    assert not [n for n in names if n not in _lib.SIGNATURES]
+
+
+def test_create_without_device_reports_no_device(fe_lib):
+    if _lib.device_count() > 0:
+        pytest.skip("a GPU is present")
+    ctx = C.c_void_p()
+    st = fe_lib.fe_create(C.byref(ctx), 0, 100, 128, 0, 3)
+    assert st == _lib.FE_ERR_NO_DEVICE
+    assert b"no HIP device" in fe_lib.fe_last_error(None)
+
+
+def _check_csr(g, symmetric=True, self_loops=False):
+    rp, ci = g.row_ptr, g.col_idx
+    assert rp[0] == 0 and rp[-1] == g.nnz and np.all(np.diff(rp) >= 0)
+    assert ci.min(initial=0) >= 0 and ci.max(initial=0) < g.n
+    rows = np.repeat(np.arange(g.n), np.diff(rp))
+    for r in range(0, g.n, max(1, g.n // 200)):
+        seg = ci[rp[r]:rp[r + 1]]
+        assert np.all(np.diff(seg) > 0)  # sorted, no duplicates
+    if not self_loops:
+        assert not np.any(rows == ci)
+    if symmetric:
+        fwd = np.sort(rows.astype(np.int64) * g.n + ci)
+        bwd = np.sort(ci.astype(np.int64) * g.n + rows)
+        assert np.array_equal(fwd, bwd)
+
+
+def test_rmat_small_invariants_and_determinism():
+    g1 = api.Graph.rmat(14, 16, seed=3)
+    g2 = api.Graph.rmat(14, 16, seed=3)
+    g3 = api.Graph.rmat(14, 16, seed=4)
+    _check_csr(g1)
+    assert np.array_equal(g1.row_ptr, g2.row_ptr) and np.array_equal(g1.col_idx, g2.col_idx)
+    assert not np.array_equal(g1.col_idx[:1000], g3.col_idx[:1000])
+    deg = np.diff(g1.row_ptr)
+    assert deg[0] == deg.max()  # a=0.57 puts the heaviest hub at vertex 0 (natural order)
+
+
+def test_rmat20_size():
+    g = api.Graph.rmat(20, 16, 0.57, 0.19, 0.19, seed=0)
+    assert g.n == 1 << 20
+    # 16.8 M generated edges -> about 31.4 M directed entries after
+    # symmetrising, merging duplicates and dropping self-loops
+    assert 3.0e7 < g.nnz < 3.3e7
+    deg = np.diff(g.row_ptr)
+    assert deg.max() > 40_000 and (deg == 0).sum() > 300_000
+
+
+def test_sbm_block_structure():
+    n, blocks = 10_000, 10
+    g = api.Graph.sbm(n, blocks, 18 / 999, 2 / 9000, seed=0)
+    _check_csr(g)
+    rows = np.repeat(np.arange(n), np.diff(g.row_ptr))
+    same = (rows // 1000) == (g.col_idx // 1000)
+    intra = same.sum() / n
+    inter = (~same).sum() / n
+    assert 17.0 < intra < 19.0 and 1.7 < inter < 2.3, (intra, inter)
+
+
+def test_csr_from_edges_symmetrises_and_merges():
+    g = api.Graph.from_edges(5, [0, 1, 1, 3, 2, 4], [1, 0, 1, 2, 3, 4])
+    assert list(g.row_ptr) == [0, 1, 2, 3, 4, 4]
+    assert list(g.col_idx) == [1, 0, 3, 2]
+    with pytest.raises(api.FeError):
+        api.Graph.from_edges(3, [0], [3])
+
+
+def test_mtx_ingest(tmp_path):
+    p = tmp_path / "sym.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate pattern symmetric\n% comment\n"
+                 "4 4 4\n2 1\n3 1\n3 3\n4 2\n")
+    g = api.Graph.read_mtx(p)
+    assert g.n == 4
+    assert list(g.row_ptr) == [0, 2, 4, 6, 7]
+    assert list(g.col_idx) == [1, 2, 0, 3, 0, 2, 1]  # self-loop 2-2 kept
+    q = tmp_path / "gen.mtx"
+    q.write_text("%%MatrixMarket matrix coordinate real general\n3 3 4\n1 2 0.5\n1 2 7\n"
+                 "3 1 1.0\n2 3 2\n")
+    h = api.Graph.read_mtx(q)
+    assert list(h.row_ptr) == [0, 1, 2, 3] and list(h.col_idx) == [1, 2, 0]
+    bad = tmp_path / "bad.mtx"
+    bad.write_text("%%MatrixMarket matrix coordinate pattern general\n3 4 1\n1 2\n")
+    with pytest.raises(api.FeError):
+        api.Graph.read_mtx(bad)
+
+
+def test_embedding_writer_format(tmp_path):
+    x = np.array([[0.5, -1.25], [1e-6, 3.0]], dtype=np.float32)
+    p = tmp_path / "emb.txt"
+    api.write_embedding(p, x)
+    assert p.read_text() == "1 0.50000 -1.25000\n2 0.00000 3.00000\n"
+
+
+def test_init_uniform_range_and_determinism():
+    a = api.init_uniform(1000, 64, -0.5, 0.5, seed=0)
+    b = api.init_uniform(1000, 64, -0.5, 0.5, seed=0)
+    c = api.init_uniform(1000, 64, -0.5, 0.5, seed=1)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    assert a.min() >= -0.5 and a.max() < 0.5
+    assert abs(a.mean()) < 0.01 and abs(a.std() - (1 / 12) ** 0.5) < 0.01
+
+
+def test_host_argument_checks(fe_lib):
+    out = np.zeros(10, dtype=np.int32)
+    p = out.ctypes.data_as(C.POINTER(C.c_int32))
+    assert fe_lib.fe_reference_negatives(0, 1, 1, 1, p, None) == _lib.FE_ERR_INVALID_ARG
+    assert fe_lib.fe_reference_negatives(10, 1, 1, 7, p, None) == _lib.FE_ERR_INVALID_ARG
+    g = _lib.fe_graph()
+    assert fe_lib.fe_gen_rmat(0, 16, 0.57, 0.19, 0.19, 0, C.byref(g)) == _lib.FE_ERR_INVALID_ARG
+    assert fe_lib.fe_gen_sbm(10, 20, 0.1, 0.1, 0, C.byref(g)) == _lib.FE_ERR_INVALID_ARG
+    assert fe_lib.fe_read_mtx(b"/nonexistent.mtx", C.byref(g)) == _lib.FE_ERR_INVALID_ARG
