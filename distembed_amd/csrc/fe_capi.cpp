@@ -172,9 +172,14 @@ extern "C" fe_status fe_get_tuning(const fe_ctx *c, fe_tuning *t) {
 static fe_status harvest_events(fe_ctx *c) {
   for (size_t i = 0; i + 3 < c->events_used; i += 4) {
     float a = 0, b = 0, d = 0;
-    hipEventElapsedTime(&a, c->events[i], c->events[i + 1]);
-    hipEventElapsedTime(&b, c->events[i + 1], c->events[i + 2]);
-    hipEventElapsedTime(&d, c->events[i + 2], c->events[i + 3]);
+    hipError_t e1 = hipEventElapsedTime(&a, c->events[i], c->events[i + 1]);
+    hipError_t e2 = hipEventElapsedTime(&b, c->events[i + 1], c->events[i + 2]);
+    hipError_t e3 = hipEventElapsedTime(&d, c->events[i + 2], c->events[i + 3]);
+    if (e1 != hipSuccess || e2 != hipSuccess || e3 != hipSuccess) {
+      (void)hipGetLastError(); // do not leave the failure for the next launch check
+      c->events_used = 0;
+      return fail(c, FE_ERR_HIP, "kernel timing: event elapsed time unavailable");
+    }
     c->t_force += a;
     c->t_fixup += b;
     c->t_apply += d;
@@ -304,7 +309,7 @@ extern "C" fe_status fe_train(fe_ctx *c, int iterations, int batch_size, int ns,
         e = hipStreamSynchronize(c->stream);
         if (e != hipSuccess) return fail(c, FE_ERR_HIP, std::string("sync: ") + hipGetErrorString(e));
         const size_t used = c->events_used;
-        harvest_events(c);
+        if ((s = harvest_events(c)) != FE_OK) return s;
         c->events_used = used;
       }
     }
@@ -348,9 +353,12 @@ extern "C" fe_status fe_loss(fe_ctx *c, uint64_t eval_seed, int eval_ns, double 
   return from_job(c, job_loss(j, eval_seed, eval_ns, attr, rep, c->stream));
 }
 
-extern "C" fe_status fe_epoch_units(const fe_ctx *c, int ns, int64_t *edges, int64_t *negs0,
+extern "C" fe_status fe_epoch_units(fe_ctx *c, int ns, int64_t *edges, int64_t *negs0,
                                     int64_t *negs_later) {
-  if (!c || c->job.plan_dirty || c->job.B <= 0) return FE_ERR_STATE;
+  CTX_GUARD(c);
+  if (ns < 0) return fail(c, FE_ERR_INVALID_ARG, "fe_epoch_units: ns < 0");
+  fe_status s = from_job(c, job_prepare(c->job, c->stream));
+  if (s != FE_OK) return s;
   const Job &j = c->job;
   int64_t n0 = 0, n1 = 0;
   for (int b = 0; b < j.plan.batches; ++b) {

@@ -361,23 +361,33 @@ fe_status job_prepare(Job &j, hipStream_t st) {
   return FE_OK;
 }
 
+// Records `ev` on `st`; during stream capture the record becomes an
+// external event node, so every graph replay stamps the event again.
+static hipError_t record_event(hipEvent_t ev, hipStream_t st) {
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(st, &cs);
+  if (e != hipSuccess) return e;
+  if (cs == hipStreamCaptureStatusActive) return hipEventRecordWithFlags(ev, st, hipEventRecordExternal);
+  return hipEventRecord(ev, st);
+}
+
 fe_status job_launch_batch(Job &j, int b, bool delta_mode, hipStream_t st, hipEvent_t *ev) {
   const StepLaunch &L = j.launches[b];
   const StepDesc *sd = (delta_mode ? j.steps_delta : j.steps) + b;
   const int *pf = j.prefix + 6 * b;
   const size_t lds = (size_t)j.ns * j.d * sizeof(float);
-  if (ev) FE_HIP(j, hipEventRecord(ev[0], st), "event");
+  if (ev) FE_HIP(j, record_event(ev[0], st), "event");
   FE_HIP(j, launch_force(sd, pf, 1, L.force_blocks, j.tun.waves_per_block, j.tun.items_per_wave,
                          j.tun.unroll, lds, st),
          "force kernel launch");
-  if (ev) FE_HIP(j, hipEventRecord(ev[1], st), "event");
+  if (ev) FE_HIP(j, record_event(ev[1], st), "event");
   if (L.fixup_blocks > 0)
     FE_HIP(j, launch_fixup(sd, pf + 2, 1, L.fixup_blocks, j.tun.waves_per_block, 1, j.tun.unroll,
                            lds, st),
            "fixup kernel launch");
-  if (ev) FE_HIP(j, hipEventRecord(ev[2], st), "event");
+  if (ev) FE_HIP(j, record_event(ev[2], st), "event");
   if (!delta_mode) FE_HIP(j, launch_apply(sd, pf + 4, 1, L.apply_blocks, st), "apply kernel launch");
-  if (ev) FE_HIP(j, hipEventRecord(ev[3], st), "event");
+  if (ev) FE_HIP(j, record_event(ev[3], st), "event");
   return FE_OK;
 }
 

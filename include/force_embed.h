@@ -142,7 +142,7 @@ fe_status fe_loss(fe_ctx *ctx, uint64_t eval_seed, int eval_ns,
 
 /* Work-unit counts of one epoch at the configured batch size: processed CSR
  * entries, (row, negative) pairs in the first epoch and in later epochs.   */
-fe_status fe_epoch_units(const fe_ctx *ctx, int ns, int64_t *edges,
+fe_status fe_epoch_units(fe_ctx *ctx, int ns, int64_t *edges,
                          int64_t *negs_epoch0, int64_t *negs_later);
 /* Average duration (ms) of the force kernel over the launches timed since
  * the last reset, measured with HIP events on the context's stream.

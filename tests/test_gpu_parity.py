@@ -1,12 +1,16 @@
 """GPU parity of the HIP force step against the oracle (through the C ABI).
 
-Tolerance (FP32 path): per batch, on identical X and identical negative
-lists, the updated rows X' = X + delta match the FP32 oracle with
-|X'_gpu - X'_ref| <= 1e-5 * |X'_ref| + 1e-5 * rms(X'_ref), and the batch
-delta matches normwise, ||delta_gpu - delta_ref|| <= 1e-5 ||delta_ref||.
-The GPU reduces |v|^2 with a DPP tree and sums a row's edges in a fixed
-interleaved order; the reference sums serially, so results agree to
-rounding, not bitwise. Every GPU result is bitwise reproducible.
+Tolerance (FP32 path), per batch, on identical X and identical negative lists:
+  * normwise: ||delta_gpu - delta_32|| <= 1e-5 ||delta_32||;
+  * elementwise on the updated rows X' = X + delta:
+      |X'_gpu - X'_32| <= 1e-5 (|X'_32| + rms(X'_32)) + 2 |X'_32 - X'_64|,
+    where X'_32 / X'_64 come from the FP32 / FP64 oracle. The last term
+    admits the FP32 reference's own rounding error: it sums each component
+    serially in float (64 701 terms for the heaviest R-MAT-20 row), while the
+    GPU sums short interleaved runs and combines them in a fixed tree, so on
+    hub rows it sits closer to the FP64 result than the FP32 reference does.
+The GPU reduces |v|^2 with a DPP tree, so results agree to rounding, not
+bitwise. Every GPU result is bitwise reproducible.
 """
 import numpy as np
 import pytest
@@ -35,10 +39,11 @@ def _rows(n, B, j):
     return min((j + 1) * B, n) - j * B
 
 
-def _check_rows(new_gpu, new_ref, delta_gpu, delta_ref, what=""):
-    scale = float(np.sqrt(np.mean(new_ref.astype(np.float64) ** 2)))
+def _check_rows(new_gpu, new_ref, delta_gpu, delta_ref, new_ref64, what=""):
+    new_ref = new_ref.astype(np.float64)
+    scale = float(np.sqrt(np.mean(new_ref ** 2)))
     err = np.abs(new_gpu.astype(np.float64) - new_ref)
-    bound = RTOL * np.abs(new_ref) + RTOL * scale
+    bound = RTOL * (np.abs(new_ref) + scale) + 2.0 * np.abs(new_ref - new_ref64)
     assert np.all(err <= bound), f"{what}: max excess {(err - bound).max():.3e}"
     num = np.linalg.norm((delta_gpu - delta_ref).astype(np.float64))
     den = np.linalg.norm(delta_ref.astype(np.float64))
@@ -57,9 +62,12 @@ def _batch_parity(g, d, B, steps, ns=5, quirks=3, tuning=None, seed=0, lr=0.02):
             dg = ctx.step_delta(epoch, j, rows)
             dr = orc.step(g, X.copy(), B, epoch, j, lists, lr=lr, quirks=quirks, apply=False,
                           want_delta=True)
+            X64 = X.astype(np.float64)
+            dr64 = orc.step(g, X64.copy(), B, epoch, j, lists, lr=lr, quirks=quirks,
+                            apply=False, want_delta=True, dtype=np.float64)
             r0 = j * B
             _check_rows(X[r0:r0 + rows] + dg, X[r0:r0 + rows] + dr, dg, dr,
-                        f"d={d} B={B} epoch={epoch} batch={j}")
+                        X64[r0:r0 + rows] + dr64, f"d={d} B={B} epoch={epoch} batch={j}")
             # the applied step writes exactly X + delta into the batch rows
             ctx.step(epoch, j)
             Xg = ctx.download_x()
