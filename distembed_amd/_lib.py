@@ -103,7 +103,8 @@ SIGNATURES = {
     "fe_engine_create": (C.c_int, [C.POINTER(_P), C.c_int, C.c_int64]),
     "fe_engine_destroy": (None, [_P]),
     "fe_engine_last_error": (C.c_char_p, [_P]),
-    "fe_engine_job_bytes": (C.c_int64, [C.POINTER(fe_job_desc)]),
+    "fe_engine_job_bytes": (C.c_int64, [_P, C.POINTER(fe_job_desc)]),
+    "fe_engine_graph_timing": (C.c_int, [_P, _DP, _DP, _I64P]),
     "fe_engine_submit": (C.c_int, [_P, C.POINTER(fe_job_desc), C.POINTER(C.c_int)]),
     "fe_engine_run": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
     "fe_engine_run_graph": (C.c_int, [_P]),

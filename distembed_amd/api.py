@@ -217,3 +217,103 @@ class ForceEmbed:
                                               C.byref(n)))
         return {"force_ms": f.value, "fixup_ms": x.value, "apply_ms": a.value,
                 "launches": n.value}
+
+
+class Engine:
+    """Multi-job engine (fe_engine): batching and serving forms."""
+
+    def __init__(self, hbm_budget_bytes: int, device: int = 0):
+        self._lib = _lib_or_raise()
+        self._e = C.c_void_p()
+        st = self._lib.fe_engine_create(C.byref(self._e), device, int(hbm_budget_bytes))
+        if st != _lib.FE_OK:
+            raise FeError(st, self._lib.fe_engine_last_error(None).decode())
+        self._keep = {}
+
+    def _err(self) -> str:
+        return self._lib.fe_engine_last_error(self._e).decode()
+
+    def _check(self, st: int) -> None:
+        _lib.check(st, self._err)
+
+    def close(self) -> None:
+        if self._e:
+            self._lib.fe_engine_destroy(self._e)
+            self._e = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @staticmethod
+    def _desc(g: Graph, x0: np.ndarray, batch_size: int, iterations: int, ns: int = 5,
+              lr: float = 0.02, gather: int = FP32, quirks: int = QUIRKS_REFERENCE,
+              neg_lists=None, max_bound: float = 5.0):
+        rp = np.ascontiguousarray(g.row_ptr, dtype=np.int64)
+        ci = np.ascontiguousarray(g.col_idx, dtype=np.int32)
+        x0 = np.ascontiguousarray(x0, dtype=np.float32)
+        keep = [rp, ci, x0]
+        d = _lib.fe_job_desc()
+        d.n, d.d, d.gather_dtype, d.quirks = g.n, x0.shape[1], gather, quirks
+        d.batch_size, d.iterations, d.ns, d.lr, d.max_bound = batch_size, iterations, ns, lr, max_bound
+        d.row_ptr, d.col_idx, d.nnz = _ptr(rp, C.c_int64), _ptr(ci, C.c_int32), ci.shape[0]
+        d.x0 = _ptr(x0, C.c_float)
+        if neg_lists is not None:
+            nl = np.ascontiguousarray(neg_lists, dtype=np.int32)
+            keep.append(nl)
+            d.neg_lists = _ptr(nl, C.c_int32)
+        return d, keep
+
+    def job_bytes(self, g: Graph, x0: np.ndarray, batch_size: int, iterations: int, **kw) -> int:
+        d, _keep = self._desc(g, x0, batch_size, iterations, **kw)
+        return int(self._lib.fe_engine_job_bytes(self._e, C.byref(d)))
+
+    def submit(self, g: Graph, x0: np.ndarray, batch_size: int, iterations: int, **kw) -> int:
+        d, keep = self._desc(g, x0, batch_size, iterations, **kw)
+        jid = C.c_int(-1)
+        self._check(self._lib.fe_engine_submit(self._e, C.byref(d), C.byref(jid)))
+        self._keep[jid.value] = (g.n, x0.shape[1])
+        del keep
+        return jid.value
+
+    def run(self, rounds: int = 0) -> int:
+        running = C.c_int(0)
+        self._check(self._lib.fe_engine_run(self._e, rounds, C.byref(running)))
+        return running.value
+
+    def run_graph(self) -> None:
+        self._check(self._lib.fe_engine_run_graph(self._e))
+
+    def graph_timing(self):
+        p, x, r = C.c_double(), C.c_double(), C.c_int64()
+        self._check(self._lib.fe_engine_graph_timing(self._e, C.byref(p), C.byref(x), C.byref(r)))
+        return {"prepare_ms": p.value, "exec_ms": x.value, "rounds": r.value}
+
+    def done(self, job_id: int) -> bool:
+        d = C.c_int(0)
+        self._check(self._lib.fe_engine_job_done(self._e, job_id, C.byref(d)))
+        return bool(d.value)
+
+    def collect(self, job_id: int) -> np.ndarray:
+        n, dim = self._keep[job_id]
+        x = np.empty((n, dim), dtype=np.float32)
+        self._check(self._lib.fe_engine_collect(self._e, job_id, _ptr(x, C.c_float)))
+        return x
+
+    def set_tuning(self, chunk=0, max_chunks=0, unroll=0, waves_per_block=0,
+                   items_per_wave=0) -> None:
+        t = fe_tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave)
+        self._check(self._lib.fe_engine_set_tuning(self._e, C.byref(t)))
+
+    def stats(self):
+        b, r, u = C.c_int64(), C.c_int64(), C.c_int64()
+        self._check(self._lib.fe_engine_stats(self._e, C.byref(b), C.byref(r), C.byref(u)))
+        return {"bytes_in_use": b.value, "rounds_run": r.value, "units_done": u.value}

@@ -26,7 +26,9 @@ DRIVER = PKG / "bin" / "dist_embed"
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    "-munsafe-fp-atomics",
+    # no implicit contraction: every fused multiply-add in the kernels is an
+    # explicit fmaf, so rounding does not depend on the unroll instantiation
+    "-ffp-contract=off",
 ]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-D__HIP_PLATFORM_AMD__",
               "-Wall", "-Wno-unused-result"]

@@ -1,9 +1,16 @@
 // Device-side types shared by the HIP kernels (fe_kernels.hip) and the host
-// code that fills them (fe_capi.cpp). Plain C++ types only.
+// code that fills them (fe_job.cpp). Plain C++ types only.
 #pragma once
 #include <stdint.h>
 
 namespace fe {
+
+// Work item: kItemInts ints {row, e0, cnt, pslot, sid, p0, k, 0}.
+//   row, edges [e0, e0+cnt) of that row;
+//   pslot < 0: the item is the whole row and writes the new row;
+//   pslot >= 0: the item is chunk pslot-p0 of heavy row `row`, split id sid,
+//   whose k partials p0..p0+k-1 the last chunk to finish adds in index order.
+constexpr int kItemInts = 8;
 
 // One minibatch step of one job (graph). Everything here is invariant across
 // epochs: the epoch-dependent parts (negative list index epoch+batch,
@@ -11,18 +18,18 @@ namespace fe {
 // derived in the kernels from the job's device epoch counter.
 struct alignas(16) StepDesc {
   const float *X;        // FP32 master copy, n x D row-major
-  float *Xw;             // same buffer, written by the apply kernel
+  float *Xw;             // same buffer, written by the copy phase / apply
   const uint16_t *Xs;    // BF16 shadow (gather source when bf16 != 0)
-  uint16_t *Xsw;         // same buffer, refreshed by the apply kernel
+  uint16_t *Xsw;         // same buffer, refreshed with the master rows
   const int32_t *col;    // CSR column ids (all of the job's graph)
-  const int32_t *items;  // this batch's work items, 4 ints each: {row, e0, cnt, pslot}
-  const int32_t *splits; // this batch's split rows, 4 ints each: {row, p0, k, 0}
-  float *S;              // rows x D: new rows (or deltas in delta mode)
+  const int32_t *items;  // this batch's work items
+  float *S;              // rows x D: new rows of this batch (or deltas)
   float *P;              // partial sums of heavy-row chunks
+  int *counters;         // per split row of the batch: chunks finished
   const int32_t *negs;   // negative lists, list l at negs + l*ns
-  int *epoch;            // device epoch counter of the job
+  const int *epoch;      // device epoch counter of the job
+  const float *Sprev;    // new rows of the previous batch, not yet in X
   int n_items;
-  int n_splits;
   int row_begin;         // batch rows [row_begin, row_begin + rows)
   int rows;
   int rep0;              // rows that get repulsion in epoch 0
@@ -34,12 +41,12 @@ struct alignas(16) StepDesc {
   int G;                 // lanes per row = D / 8
   int bf16;              // gathers read the BF16 shadow
   int c_limit;           // attractive neighbours must satisfy c < c_limit
-  int delta_mode;        // S receives the delta; apply is not run
-  int last_batch;        // apply advances the epoch counter
+  int delta_mode;        // S receives the delta instead of the new row
+  int prev_begin;        // rows of the previous batch held in Sprev
+  int prev_rows;         // 0: no pending previous batch
   float lr;
   float bound;           // MAX_BOUND of cpp/algo/algo.hpp:65-72
   int pad0;
-  int pad1;
 };
 
 // Loss over one job's graph (FP64 reduction).

@@ -1,0 +1,635 @@
+// Multi-job engine: the batching and serving forms of the force step.
+//
+// Jobs are independent graphs. Their device state lives in one arena carved
+// from a single hipMalloc of the HBM budget (first fit, coalescing free
+// list), so admission is a pure accounting question and nothing else in the
+// process competes for that memory. Every engine round advances each
+// running job by one minibatch: the round's step descriptors (one per job)
+// go to the device in one copy and all jobs' work items share one force
+// launch and one apply launch. A job's work items, chunking and summation
+// order are fixed when it is submitted, and no job reads or writes another
+// job's memory, so its result does not depend on the mix it ran in.
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "fe_job.hpp"
+
+using namespace fe;
+
+namespace {
+
+constexpr size_t kAlign = 256;
+
+// First-fit arena over one device allocation.
+class Arena : public DeviceAllocator {
+public:
+  fe_status init(size_t bytes, std::string &err) {
+    bytes = (bytes + kAlign - 1) / kAlign * kAlign;
+    hipError_t e = hipMalloc(&base_, bytes);
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      err = std::string("engine arena: hipMalloc of ") + std::to_string(bytes) +
+            " bytes failed: " + hipGetErrorString(e);
+      return FE_ERR_OOM;
+    }
+    size_ = bytes;
+    free_[0] = bytes;
+    return FE_OK;
+  }
+  ~Arena() override {
+    if (base_) hipFree(base_);
+  }
+  hipError_t alloc(void **p, size_t bytes) override {
+    bytes = std::max<size_t>((bytes + kAlign - 1) / kAlign * kAlign, kAlign);
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+      if (it->second >= bytes) {
+        const size_t off = it->first, len = it->second;
+        free_.erase(it);
+        if (len > bytes) free_[off + bytes] = len - bytes;
+        used_[off] = bytes;
+        in_use_ += bytes;
+        *p = static_cast<char *>(base_) + off;
+        return hipSuccess;
+      }
+    }
+    return hipErrorOutOfMemory;
+  }
+  void release(void *p) override {
+    if (!p) return;
+    const size_t off = static_cast<char *>(p) - static_cast<char *>(base_);
+    auto u = used_.find(off);
+    if (u == used_.end()) return;
+    size_t len = u->second;
+    used_.erase(u);
+    in_use_ -= len;
+    size_t start = off;
+    auto next = free_.lower_bound(off);
+    if (next != free_.end() && next->first == off + len) {
+      len += next->second;
+      next = free_.erase(next);
+    }
+    if (next != free_.begin()) {
+      auto prev = std::prev(next);
+      if (prev->first + prev->second == start) {
+        start = prev->first;
+        len += prev->second;
+        free_.erase(prev);
+      }
+    }
+    free_[start] = len;
+  }
+  size_t in_use() const { return in_use_; }
+  size_t size() const { return size_; }
+  size_t largest_free() const {
+    size_t m = 0;
+    for (auto &f : free_) m = std::max(m, f.second);
+    return m;
+  }
+
+private:
+  void *base_ = nullptr;
+  size_t size_ = 0;
+  size_t in_use_ = 0;
+  std::map<size_t, size_t> free_; // offset -> length
+  std::map<size_t, size_t> used_;
+};
+
+struct EngineJob {
+  Job job;
+  int id = 0;
+  int iterations = 0;
+  int epoch = 0;        // next step (epoch, batch)
+  int batch = 0;
+  bool launched_all = false;
+  bool collected = false;
+  hipEvent_t uploaded = nullptr; // uploads finished (on the upload stream)
+  hipEvent_t done = nullptr;     // the job's last round finished
+  bool waited_upload = false;
+  int64_t units = 0;
+};
+
+} // namespace
+
+struct fe_engine {
+  int device = 0;
+  hipStream_t stream = nullptr;    // rounds
+  hipStream_t upload = nullptr;    // job uploads and result downloads
+  std::unique_ptr<Arena> arena;
+  std::vector<std::unique_ptr<EngineJob>> jobs;
+  Tuning tun;
+  // round staging: pinned host ring of 2 and one device buffer
+  struct Stage {
+    char *host = nullptr;
+    hipEvent_t copied = nullptr;
+  } stage[2];
+  int stage_next = 0;
+  size_t stage_cap = 0; // steps
+  char *dev_round = nullptr;
+  int64_t rounds_run = 0;
+  int64_t units_done = 0;
+  double graph_prepare_ms = 0; // last fe_engine_run_graph: schedule, capture, instantiate
+  double graph_exec_ms = 0;    // ... and the replay itself (HIP events)
+  int64_t graph_rounds = 0;
+  std::string err;
+};
+
+static thread_local std::string g_engine_create_error;
+
+static fe_status efail(fe_engine *e, fe_status s, const std::string &m) {
+  e->err = m;
+  return s;
+}
+
+static fe_status ehip(fe_engine *e, hipError_t h, const char *what) {
+  if (h == hipSuccess) return FE_OK;
+  return efail(e, h == hipErrorOutOfMemory ? FE_ERR_OOM : FE_ERR_HIP,
+               std::string(what) + ": " + hipGetErrorString(h));
+}
+
+// Round layout in the staging buffers for K steps:
+//   StepDesc[K] | int epochs[K] | int force_prefix[K+1] | int apply_prefix[K+1]
+static size_t round_bytes(size_t K) {
+  size_t b = K * sizeof(StepDesc);
+  b += K * sizeof(int) + 2 * (K + 1) * sizeof(int);
+  return (b + 255) / 256 * 256;
+}
+
+static fe_status ensure_stage(fe_engine *e, size_t K) {
+  if (K <= e->stage_cap) return FE_OK;
+  size_t cap = std::max<size_t>(64, e->stage_cap);
+  while (cap < K) cap *= 2;
+  hipStreamSynchronize(e->stream);
+  for (auto &s : e->stage) {
+    if (s.host) hipHostFree(s.host);
+    s.host = nullptr;
+    fe_status st = ehip(e, hipHostMalloc((void **)&s.host, round_bytes(cap)), "pinned staging");
+    if (st != FE_OK) return st;
+  }
+  if (e->dev_round) hipFree(e->dev_round);
+  e->dev_round = nullptr;
+  fe_status st = ehip(e, hipMalloc((void **)&e->dev_round, round_bytes(cap)), "round buffer");
+  if (st != FE_OK) return st;
+  e->stage_cap = cap;
+  return FE_OK;
+}
+
+static fe_status validate_desc(const fe_job_desc *d, std::string &err) {
+  if (!d) {
+    err = "job description is NULL";
+    return FE_ERR_INVALID_ARG;
+  }
+  if (!d->row_ptr || (!d->col_idx && d->nnz > 0) || !d->x0 || d->batch_size <= 0 ||
+      d->iterations < 0 || d->ns < 0 || d->n <= 0) {
+    err = "job description: missing array or non-positive size";
+    return FE_ERR_INVALID_ARG;
+  }
+  return FE_OK;
+}
+
+// Host-side plan of a job description (for sizing before any allocation).
+static fe_status plan_job(const fe_job_desc *d, const Tuning &tun, Job &tmp, std::string &err) {
+  fe_status s = validate_desc(d, err);
+  if (s != FE_OK) return s;
+  tmp.n = d->n;
+  tmp.d = d->d;
+  tmp.dtype = d->gather_dtype;
+  tmp.nnz = d->nnz;
+  tmp.ns = d->ns;
+  tmp.tun = tun;
+  const int64_t c_limit = (d->quirks & FE_QUIRK_SKIP_LAST_COL) ? d->n - 1 : d->n;
+  s = build_plan(d->n, d->row_ptr, nullptr, d->batch_size, tun.chunk, tun.max_chunks, c_limit,
+                 &tmp.plan);
+  if (s != FE_OK) {
+    err = host_error();
+    return s;
+  }
+  tmp.n_lists = (int)(d->iterations + tmp.plan.batches - 1);
+  return FE_OK;
+}
+
+extern "C" fe_status fe_engine_create(fe_engine **out, int device, int64_t budget) {
+  if (!out || budget <= 0) {
+    g_engine_create_error = "fe_engine_create: invalid argument";
+    return FE_ERR_INVALID_ARG;
+  }
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
+    (void)hipGetLastError();
+    g_engine_create_error = "fe_engine_create: no HIP device available";
+    return FE_ERR_NO_DEVICE;
+  }
+  if (device < 0 || device >= count) {
+    g_engine_create_error = "fe_engine_create: device index out of range";
+    return FE_ERR_NO_DEVICE;
+  }
+  auto *e = new (std::nothrow) fe_engine();
+  if (!e) return FE_ERR_OOM;
+  e->device = device;
+  fe_status s = ehip(e, hipSetDevice(device), "hipSetDevice");
+  if (s == FE_OK) s = ehip(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "stream");
+  if (s == FE_OK) s = ehip(e, hipStreamCreateWithFlags(&e->upload, hipStreamNonBlocking), "stream");
+  for (auto &st : e->stage)
+    if (s == FE_OK) s = ehip(e, hipEventCreateWithFlags(&st.copied, hipEventDisableTiming), "event");
+  if (s == FE_OK) {
+    e->arena.reset(new Arena());
+    s = e->arena->init((size_t)budget, e->err);
+  }
+  if (s == FE_OK) s = ensure_stage(e, 64);
+  if (s != FE_OK) {
+    g_engine_create_error = "fe_engine_create: " + e->err;
+    fe_engine_destroy(e);
+    return s;
+  }
+  *out = e;
+  return FE_OK;
+}
+
+extern "C" void fe_engine_destroy(fe_engine *e) {
+  if (!e) return;
+  hipSetDevice(e->device);
+  if (e->stream) hipStreamSynchronize(e->stream);
+  if (e->upload) hipStreamSynchronize(e->upload);
+  for (auto &jp : e->jobs) {
+    job_free(jp->job);
+    if (jp->uploaded) hipEventDestroy(jp->uploaded);
+    if (jp->done) hipEventDestroy(jp->done);
+  }
+  e->jobs.clear();
+  for (auto &st : e->stage) {
+    if (st.host) hipHostFree(st.host);
+    if (st.copied) hipEventDestroy(st.copied);
+  }
+  if (e->dev_round) hipFree(e->dev_round);
+  e->arena.reset();
+  if (e->stream) hipStreamDestroy(e->stream);
+  if (e->upload) hipStreamDestroy(e->upload);
+  delete e;
+}
+
+extern "C" const char *fe_engine_last_error(const fe_engine *e) {
+  return e ? e->err.c_str() : g_engine_create_error.c_str();
+}
+
+extern "C" int64_t fe_engine_job_bytes(const fe_engine *e, const fe_job_desc *d) {
+  Job tmp;
+  std::string err;
+  const Tuning tun = e ? e->tun : Tuning();
+  if (plan_job(d, tun, tmp, err) != FE_OK) return -1;
+  // arena rounding: every buffer is a multiple of 256 bytes
+  return job_bytes_estimate(tmp) + 16 * (int64_t)kAlign;
+}
+
+extern "C" fe_status fe_engine_set_tuning(fe_engine *e, const fe_tuning *t) {
+  if (!e || !t) return FE_ERR_INVALID_ARG;
+  Tuning nt = e->tun;
+  if (t->chunk) nt.chunk = t->chunk;
+  if (t->max_chunks) nt.max_chunks = t->max_chunks;
+  if (t->unroll) nt.unroll = t->unroll;
+  if (t->waves_per_block) nt.waves_per_block = t->waves_per_block;
+  if (t->items_per_wave) nt.items_per_wave = t->items_per_wave;
+  Job probe;
+  if (job_set_tuning(probe, nt) != FE_OK) return efail(e, FE_ERR_INVALID_ARG, probe.err);
+  // chunking of already submitted jobs is fixed; launch shape applies to all
+  e->tun = nt;
+  return FE_OK;
+}
+
+extern "C" fe_status fe_engine_submit(fe_engine *e, const fe_job_desc *d, int *job_id) {
+  if (!e || !job_id) return FE_ERR_INVALID_ARG;
+  fe_status s = ehip(e, hipSetDevice(e->device), "hipSetDevice");
+  if (s != FE_OK) return s;
+  const int64_t need = fe_engine_job_bytes(e, d);
+  if (need < 0) {
+    std::string err;
+    validate_desc(d, err);
+    return efail(e, FE_ERR_INVALID_ARG, "fe_engine_submit: " + (err.empty() ? std::string(host_error()) : err));
+  }
+  if ((size_t)need > e->arena->size() - e->arena->in_use())
+    return efail(e, FE_ERR_OVER_BUDGET, "fe_engine_submit: job needs " + std::to_string(need) +
+                                            " bytes, " +
+                                            std::to_string(e->arena->size() - e->arena->in_use()) +
+                                            " free in the budget");
+  auto ej = std::make_unique<EngineJob>();
+  Job &j = ej->job;
+  j.allocator = e->arena.get();
+  j.tun = e->tun;
+  s = job_init(j, d->n, d->d, d->gather_dtype, d->quirks);
+  if (s == FE_OK) s = job_upload_csr(j, d->row_ptr, d->col_idx, d->nnz, e->upload);
+  if (s == FE_OK) s = job_upload_x(j, d->x0, e->upload);
+  if (s == FE_OK) s = job_configure(j, d->batch_size, d->lr, d->max_bound);
+  std::vector<int32_t> lists;
+  if (s == FE_OK) {
+    const int batches = (int)((d->n + d->batch_size - 1) / d->batch_size);
+    const int n_lists = d->iterations + batches - 1;
+    if (d->ns > 0 && n_lists > 0) {
+      if (d->neg_lists) {
+        s = job_set_negatives(j, d->neg_lists, n_lists, d->ns, e->upload);
+      } else {
+        lists.resize((size_t)n_lists * d->ns);
+        s = fe_reference_negatives(d->n, n_lists, d->ns, FE_NEG_REDRAW, lists.data(), nullptr);
+        if (s != FE_OK) j.err = host_error();
+        else s = job_set_negatives(j, lists.data(), n_lists, d->ns, e->upload);
+      }
+    }
+  }
+  if (s == FE_OK) s = job_prepare(j, e->upload);
+  if (s == FE_OK) s = ehip(e, hipEventCreateWithFlags(&ej->uploaded, hipEventDisableTiming), "event");
+  if (s == FE_OK) s = ehip(e, hipEventCreateWithFlags(&ej->done, hipEventDisableTiming), "event");
+  if (s == FE_OK) s = ehip(e, hipEventRecord(ej->uploaded, e->upload), "event");
+  if (s != FE_OK) {
+    if (e->err.empty() || s != FE_ERR_HIP) e->err = "fe_engine_submit: " + j.err;
+    if (s == FE_ERR_OOM) s = FE_ERR_OVER_BUDGET;
+    hipStreamSynchronize(e->upload);
+    job_free(j);
+    if (ej->uploaded) hipEventDestroy(ej->uploaded);
+    if (ej->done) hipEventDestroy(ej->done);
+    return s;
+  }
+  ej->iterations = d->iterations;
+  ej->launched_all = d->iterations == 0;
+  ej->id = (int)e->jobs.size();
+  if (ej->launched_all) hipEventRecord(ej->done, e->upload);
+  *job_id = ej->id;
+  e->jobs.push_back(std::move(ej));
+  return FE_OK;
+}
+
+// Step descriptors of the next round of every running job; advances them.
+static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *host, int &K,
+                             size_t &lds, std::vector<EngineJob *> &finished) {
+  K = (int)run.size();
+  StepDesc *sd = reinterpret_cast<StepDesc *>(host);
+  int *ep = reinterpret_cast<int *>(host + K * sizeof(StepDesc));
+  int *pf = ep + K;
+  int *pa = pf + K + 1;
+  int *dep = reinterpret_cast<int *>(e->dev_round + K * sizeof(StepDesc));
+  const int ipb = e->tun.waves_per_block * e->tun.items_per_wave;
+  pf[0] = pa[0] = 0;
+  lds = 0;
+  for (int k = 0; k < K; ++k) {
+    EngineJob *ej = run[k];
+    Job &j = ej->job;
+    const int b = ej->batch;
+    sd[k] = j.host_desc[(size_t)j.plan.batches * DESC_SINGLE + b];
+    sd[k].epoch = dep + k;
+    ep[k] = ej->epoch;
+    pf[k + 1] = pf[k] + std::max(1, (sd[k].n_items + ipb - 1) / ipb);
+    pa[k + 1] = pa[k] + j.apply_blocks[b];
+    lds = std::max(lds, (size_t)j.ns * j.d * sizeof(float));
+    int r0, r1;
+    rep_rows(j.plan, j.quirks, b, &r0, &r1);
+    ej->units += j.plan.edges_eff[b] + (int64_t)(ej->epoch == 0 ? r0 : r1) * j.ns;
+    e->units_done += j.plan.edges_eff[b] + (int64_t)(ej->epoch == 0 ? r0 : r1) * j.ns;
+    if (++ej->batch == j.plan.batches) {
+      ej->batch = 0;
+      if (++ej->epoch == ej->iterations) {
+        ej->launched_all = true;
+        finished.push_back(ej);
+      }
+    }
+  }
+  return FE_OK;
+}
+
+extern "C" fe_status fe_engine_run(fe_engine *e, int rounds, int *running) {
+  if (!e) return FE_ERR_INVALID_ARG;
+  fe_status s = ehip(e, hipSetDevice(e->device), "hipSetDevice");
+  if (s != FE_OK) return s;
+  int done_rounds = 0;
+  while (rounds <= 0 || done_rounds < rounds) {
+    std::vector<EngineJob *> run;
+    for (auto &jp : e->jobs)
+      if (!jp->launched_all && !jp->collected) run.push_back(jp.get());
+    if (run.empty()) break;
+    if ((s = ensure_stage(e, run.size())) != FE_OK) return s;
+    // jobs whose uploads are still in flight join once they have landed
+    for (EngineJob *ej : run) {
+      if (!ej->waited_upload) {
+        if ((s = ehip(e, hipStreamWaitEvent(e->stream, ej->uploaded, 0), "wait upload")) != FE_OK)
+          return s;
+        ej->waited_upload = true;
+      }
+    }
+    auto &stg = e->stage[e->stage_next];
+    e->stage_next ^= 1;
+    if ((s = ehip(e, hipEventSynchronize(stg.copied), "staging reuse")) != FE_OK) return s;
+    int K = 0;
+    size_t lds = 0;
+    std::vector<EngineJob *> finished;
+    build_round(e, run, stg.host, K, lds, finished);
+    const int *hpf = reinterpret_cast<const int *>(stg.host + K * sizeof(StepDesc)) + K;
+    const int *hpa = hpf + K + 1;
+    const int fblocks = hpf[K], ablocks = hpa[K];
+    if ((s = ehip(e, hipMemcpyAsync(e->dev_round, stg.host, round_bytes(K), hipMemcpyHostToDevice,
+                                    e->stream),
+                  "round copy")) != FE_OK)
+      return s;
+    if ((s = ehip(e, hipEventRecord(stg.copied, e->stream), "event")) != FE_OK) return s;
+    const StepDesc *dsd = reinterpret_cast<const StepDesc *>(e->dev_round);
+    const int *dpf = reinterpret_cast<const int *>(e->dev_round + K * sizeof(StepDesc)) + K;
+    const int *dpa = dpf + K + 1;
+    if ((s = ehip(e, launch_force(dsd, dpf, K, fblocks, e->tun.waves_per_block,
+                                  e->tun.items_per_wave, e->tun.unroll, lds, e->stream),
+                  "force launch")) != FE_OK)
+      return s;
+    if ((s = ehip(e, launch_apply(dsd, dpa, K, ablocks, e->stream), "apply launch")) != FE_OK)
+      return s;
+    for (EngineJob *ej : finished)
+      if ((s = ehip(e, hipEventRecord(ej->done, e->stream), "event")) != FE_OK) return s;
+    ++e->rounds_run;
+    ++done_rounds;
+  }
+  if (running) {
+    int n = 0;
+    for (auto &jp : e->jobs)
+      if (!jp->launched_all && !jp->collected) ++n;
+    *running = n;
+  }
+  return FE_OK;
+}
+
+extern "C" fe_status fe_engine_job_done(fe_engine *e, int job_id, int *done) {
+  if (!e || !done || job_id < 0 || job_id >= (int)e->jobs.size())
+    return e ? efail(e, FE_ERR_INVALID_ARG, "fe_engine_job_done: bad job id") : FE_ERR_INVALID_ARG;
+  EngineJob *ej = e->jobs[job_id].get();
+  if (ej->collected) {
+    *done = 1;
+    return FE_OK;
+  }
+  if (!ej->launched_all) {
+    *done = 0;
+    return FE_OK;
+  }
+  hipError_t q = hipEventQuery(ej->done);
+  if (q == hipErrorNotReady) {
+    (void)hipGetLastError();
+    *done = 0;
+    return FE_OK;
+  }
+  if (q != hipSuccess) return ehip(e, q, "fe_engine_job_done");
+  *done = 1;
+  return FE_OK;
+}
+
+extern "C" fe_status fe_engine_collect(fe_engine *e, int job_id, float *x_out) {
+  if (!e || !x_out || job_id < 0 || job_id >= (int)e->jobs.size())
+    return e ? efail(e, FE_ERR_INVALID_ARG, "fe_engine_collect: bad argument") : FE_ERR_INVALID_ARG;
+  fe_status s = ehip(e, hipSetDevice(e->device), "hipSetDevice");
+  if (s != FE_OK) return s;
+  EngineJob *ej = e->jobs[job_id].get();
+  if (ej->collected) return efail(e, FE_ERR_STATE, "fe_engine_collect: already collected");
+  if (!ej->launched_all) return efail(e, FE_ERR_STATE, "fe_engine_collect: job still running");
+  // the download waits for the job's last round only, on the upload stream
+  if ((s = ehip(e, hipStreamWaitEvent(e->upload, ej->done, 0), "wait done")) != FE_OK) return s;
+  Job &j = ej->job;
+  s = job_download_x(j, x_out, e->upload);
+  if (s != FE_OK) return efail(e, s, j.err);
+  job_free(j);
+  hipEventDestroy(ej->uploaded);
+  hipEventDestroy(ej->done);
+  ej->uploaded = ej->done = nullptr;
+  ej->collected = true;
+  return FE_OK;
+}
+
+extern "C" fe_status fe_engine_stats(const fe_engine *e, int64_t *bytes_in_use,
+                                     int64_t *rounds_run, int64_t *units_done) {
+  if (!e) return FE_ERR_INVALID_ARG;
+  if (bytes_in_use) *bytes_in_use = (int64_t)e->arena->in_use();
+  if (rounds_run) *rounds_run = e->rounds_run;
+  if (units_done) *units_done = e->units_done;
+  return FE_OK;
+}
+
+// Batching form: every submitted job runs to completion through rounds whose
+// descriptors are all built up front and whose launches are captured in HIP
+// graphs (segments of kGraphRounds rounds), each replayed once.
+extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
+  if (!e) return FE_ERR_INVALID_ARG;
+  fe_status s = ehip(e, hipSetDevice(e->device), "hipSetDevice");
+  if (s != FE_OK) return s;
+  std::vector<EngineJob *> all;
+  for (auto &jp : e->jobs)
+    if (!jp->launched_all && !jp->collected) all.push_back(jp.get());
+  if (all.empty()) return FE_OK;
+  const auto t0 = std::chrono::steady_clock::now();
+  if ((s = ehip(e, hipStreamSynchronize(e->upload), "upload sync")) != FE_OK) return s;
+  // schedule: round r holds every job that still has steps at r
+  struct RoundInfo {
+    size_t offset; // bytes into the schedule buffer
+    int K;
+    int fblocks, ablocks;
+    size_t lds;
+  };
+  std::vector<RoundInfo> rounds;
+  std::vector<char> host;
+  std::vector<std::pair<int, EngineJob *>> finish_at; // round index -> job
+  size_t total = 0;
+  while (true) {
+    std::vector<EngineJob *> run;
+    for (EngineJob *ej : all)
+      if (!ej->launched_all) run.push_back(ej);
+    if (run.empty()) break;
+    const size_t rb = round_bytes(run.size());
+    host.resize(total + rb);
+    int K = 0;
+    size_t lds = 0;
+    std::vector<EngineJob *> finished;
+    // descriptors point at this round's own epoch slots in the device schedule
+    char *h = host.data() + total;
+    build_round(e, run, h, K, lds, finished);
+    const int *hpf = reinterpret_cast<const int *>(h + K * sizeof(StepDesc)) + K;
+    rounds.push_back({total, K, hpf[K], hpf[2 * K + 1], lds});
+    for (EngineJob *ej : finished) finish_at.push_back({(int)rounds.size() - 1, ej});
+    total += rb;
+  }
+  char *dsched = nullptr;
+  if ((s = ehip(e, hipMalloc((void **)&dsched, std::max<size_t>(total, 256)), "schedule buffer")) != FE_OK)
+    return s;
+  // rebase the per-round epoch pointers (built against dev_round) onto the schedule
+  for (const RoundInfo &ri : rounds) {
+    StepDesc *sd = reinterpret_cast<StepDesc *>(host.data() + ri.offset);
+    int *dep = reinterpret_cast<int *>(dsched + ri.offset + ri.K * sizeof(StepDesc));
+    for (int k = 0; k < ri.K; ++k) sd[k].epoch = dep + k;
+  }
+  s = ehip(e, hipMemcpy(dsched, host.data(), total, hipMemcpyHostToDevice), "schedule copy");
+  // capture and instantiate every segment first, then replay them back to back
+  constexpr int kGraphRounds = 512;
+  std::vector<hipGraph_t> graphs;
+  std::vector<hipGraphExec_t> execs;
+  std::vector<size_t> seg_end;
+  for (size_t r0 = 0; s == FE_OK && r0 < rounds.size(); r0 += kGraphRounds) {
+    const size_t r1 = std::min(rounds.size(), r0 + kGraphRounds);
+    if ((s = ehip(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal), "capture")) != FE_OK)
+      break;
+    fe_status ls = FE_OK;
+    for (size_t r = r0; r < r1 && ls == FE_OK; ++r) {
+      const RoundInfo &ri = rounds[r];
+      const StepDesc *dsd = reinterpret_cast<const StepDesc *>(dsched + ri.offset);
+      const int *dpf = reinterpret_cast<const int *>(dsched + ri.offset + ri.K * sizeof(StepDesc)) + ri.K;
+      const int *dpa = dpf + ri.K + 1;
+      ls = ehip(e, launch_force(dsd, dpf, ri.K, ri.fblocks, e->tun.waves_per_block,
+                                e->tun.items_per_wave, e->tun.unroll, ri.lds, e->stream),
+                "force launch");
+      if (ls == FE_OK) ls = ehip(e, launch_apply(dsd, dpa, ri.K, ri.ablocks, e->stream), "apply launch");
+    }
+    hipGraph_t g = nullptr;
+    hipError_t ce = hipStreamEndCapture(e->stream, &g);
+    if (ls != FE_OK) {
+      s = ls;
+      if (g) hipGraphDestroy(g);
+      break;
+    }
+    if ((s = ehip(e, ce, "end capture")) != FE_OK) break;
+    graphs.push_back(g);
+    hipGraphExec_t ge = nullptr;
+    if ((s = ehip(e, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0), "graph instantiate")) != FE_OK)
+      break;
+    execs.push_back(ge);
+    seg_end.push_back(r1);
+  }
+  const auto t1 = std::chrono::steady_clock::now();
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  if (s == FE_OK) s = ehip(e, hipEventCreate(&ev0), "event");
+  if (s == FE_OK) s = ehip(e, hipEventCreate(&ev1), "event");
+  if (s == FE_OK) s = ehip(e, hipEventRecord(ev0, e->stream), "event");
+  size_t fi = 0;
+  for (size_t gi = 0; s == FE_OK && gi < execs.size(); ++gi) {
+    s = ehip(e, hipGraphLaunch(execs[gi], e->stream), "graph launch");
+    // job completion events after the segment holding their last round
+    while (s == FE_OK && fi < finish_at.size() && (size_t)finish_at[fi].first < seg_end[gi]) {
+      s = ehip(e, hipEventRecord(finish_at[fi].second->done, e->stream), "event");
+      ++fi;
+    }
+  }
+  if (s == FE_OK) s = ehip(e, hipEventRecord(ev1, e->stream), "event");
+  if (s == FE_OK) s = ehip(e, hipStreamSynchronize(e->stream), "graph sync");
+  float ms = 0.f;
+  if (s == FE_OK) s = ehip(e, hipEventElapsedTime(&ms, ev0, ev1), "event time");
+  e->graph_exec_ms = ms;
+  e->graph_prepare_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+  e->graph_rounds = (int64_t)rounds.size();
+  e->rounds_run += (int64_t)rounds.size();
+  hipStreamSynchronize(e->stream);
+  if (ev0) hipEventDestroy(ev0);
+  if (ev1) hipEventDestroy(ev1);
+  for (auto ge : execs) hipGraphExecDestroy(ge);
+  for (auto g : graphs) hipGraphDestroy(g);
+  hipFree(dsched);
+  return s;
+}
+
+extern "C" fe_status fe_engine_graph_timing(const fe_engine *e, double *prepare_ms,
+                                            double *exec_ms, int64_t *rounds) {
+  if (!e) return FE_ERR_INVALID_ARG;
+  if (prepare_ms) *prepare_ms = e->graph_prepare_ms;
+  if (exec_ms) *exec_ms = e->graph_exec_ms;
+  if (rounds) *rounds = e->graph_rounds;
+  return FE_OK;
+}

@@ -16,12 +16,11 @@ fe_status build_csr(int64_t n, const int64_t *u, const int64_t *v, int64_t m, bo
                     bool drop_self, fe_graph *out);
 
 // Work plan of one graph at one batch size: the rows of batch j
-// (cpp/algo/algo.hpp:394-397) cut into work items.
-//   item  {row, e0, cnt, pslot}: edges [e0, e0+cnt) of `row`; pslot < 0
-//         means the item is the whole row and writes the new row, otherwise
-//         it writes partial sum `pslot` of a heavy row.
-//   split {row, p0, k, 0}: heavy row whose partials p0..p0+k-1 the fixup
-//         kernel adds in index order.
+// (cpp/algo/algo.hpp:394-397) cut into work items of kItemInts ints
+//   {row, e0, cnt, pslot, sid, p0, k, 0}: edges [e0, e0+cnt) of `row`.
+//   pslot < 0: the item is the whole row and writes the new row.
+//   pslot >= 0: chunk pslot-p0 of heavy row `row` (split sid of its batch);
+//   the last of its k chunks to finish sums partials p0..p0+k-1 in order.
 // A row of degree <= chunk is one item; a heavier row is cut into
 // k = ceil(deg / max(chunk, ceil(deg / max_chunks))) near-equal chunks.
 struct Plan {
@@ -32,10 +31,11 @@ struct Plan {
   int max_chunks = 0;
   std::vector<int32_t> items;
   std::vector<int64_t> item_off;  // batches + 1, in items
-  std::vector<int32_t> splits;
+  std::vector<int32_t> splits;    // {row, p0, k, 0} per heavy row
   std::vector<int64_t> split_off; // batches + 1, in splits
   std::vector<int32_t> pslots;    // partial slots used by batch j
   int max_pslots = 0;
+  int max_splits = 0;
   std::vector<int64_t> edges_eff; // CSR entries with c < c_limit in batch j
   int64_t edges_eff_total = 0;
 };

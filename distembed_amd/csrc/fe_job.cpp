@@ -21,21 +21,31 @@ fe_status job_check(Job &j, hipError_t e, const char *what) {
     if (_s != FE_OK) return _s;                                                                \
   } while (0)
 
+static hipError_t dev_alloc(Job &j, void **p, size_t bytes) {
+  return j.allocator ? j.allocator->alloc(p, bytes) : hipMalloc(p, bytes);
+}
+
+static void dev_free(Job &j, void *p) {
+  if (!p) return;
+  if (j.allocator) j.allocator->release(p);
+  else hipFree(p);
+}
+
 template <typename T>
 static fe_status ensure_cap(Job &j, T *&ptr, size_t &cap, size_t count, const char *what) {
   if (count <= cap && ptr) return FE_OK;
   if (ptr) {
-    hipFree(ptr);
+    dev_free(j, ptr);
     j.device_bytes -= (int64_t)(cap * sizeof(T));
     ptr = nullptr;
     cap = 0;
   }
   const size_t want = std::max<size_t>(count, 1);
   void *p = nullptr;
-  hipError_t e = hipMalloc(&p, want * sizeof(T));
+  hipError_t e = dev_alloc(j, &p, want * sizeof(T));
   if (e != hipSuccess) {
     (void)hipGetLastError();
-    j.err = std::string(what) + ": hipMalloc of " + std::to_string(want * sizeof(T)) +
+    j.err = std::string(what) + ": allocation of " + std::to_string(want * sizeof(T)) +
             " bytes failed: " + hipGetErrorString(e);
     return FE_ERR_OOM;
   }
@@ -48,6 +58,17 @@ static fe_status ensure_cap(Job &j, T *&ptr, size_t &cap, size_t count, const ch
 int rows_of_batch(const Job &j, int b) {
   const int64_t r0 = (int64_t)b * j.plan.B;
   return (int)(std::min<int64_t>(r0 + j.plan.B, j.n) - r0);
+}
+
+hipError_t record_event(hipEvent_t ev, hipStream_t st) {
+  // during stream capture the record becomes an external event node, so
+  // every graph replay stamps the event again
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  hipError_t e = hipStreamIsCapturing(st, &cs);
+  if (e != hipSuccess) return e;
+  if (cs == hipStreamCaptureStatusActive)
+    return hipEventRecordWithFlags(ev, st, hipEventRecordExternal);
+  return hipEventRecord(ev, st);
 }
 
 fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks) {
@@ -73,42 +94,27 @@ fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks) {
   j.dtype = dtype;
   j.quirks = quirks;
   j.c_limit = (quirks & FE_QUIRK_SKIP_LAST_COL) ? n - 1 : n;
-  size_t cap = 0;
-  fe_status s = ensure_cap(j, j.X, cap, (size_t)n * d, "X");
+  fe_status s = ensure_cap(j, j.X, j.cap_X, (size_t)n * d, "X");
   if (s != FE_OK) return s;
   if (dtype == FE_GATHER_BF16) {
-    size_t cap2 = 0;
-    s = ensure_cap(j, j.Xs, cap2, (size_t)n * d, "BF16 shadow");
+    s = ensure_cap(j, j.Xs, j.cap_Xs, (size_t)n * d, "BF16 shadow");
     if (s != FE_OK) return s;
   }
-  size_t cap3 = 0;
-  s = ensure_cap(j, j.epoch, cap3, 1, "epoch counter");
-  if (s != FE_OK) return s;
-  return FE_OK;
+  return ensure_cap(j, j.epoch, j.cap_epoch, 1, "epoch counter");
 }
 
 void job_free(Job &j) {
-  hipFree(j.X);
-  hipFree(j.Xs);
-  hipFree(j.col);
-  hipFree(j.items);
-  hipFree(j.splits);
-  hipFree(j.S);
-  hipFree(j.P);
-  hipFree(j.negs);
-  hipFree(j.epoch);
-  hipFree(j.steps);
-  hipFree(j.steps_delta);
-  hipFree(j.prefix);
+  void *bufs[] = {j.X, j.Xs, j.col, j.items, j.S, j.P, j.counters, j.negs, j.epoch, j.desc,
+                  j.prefix};
+  for (void *p : bufs) dev_free(j, p);
   j.X = nullptr;
   j.Xs = nullptr;
-  j.col = nullptr;
-  j.items = j.splits = nullptr;
+  j.col = j.items = j.negs = nullptr;
   j.S = j.P = nullptr;
-  j.negs = nullptr;
-  j.epoch = nullptr;
-  j.steps = j.steps_delta = nullptr;
-  j.prefix = nullptr;
+  j.counters = j.epoch = j.prefix = nullptr;
+  j.desc = nullptr;
+  j.cap_items = j.cap_S = j.cap_P = j.cap_ctr = j.cap_negs = j.cap_desc = 0;
+  j.cap_prefix = j.cap_col = j.cap_X = j.cap_Xs = j.cap_epoch = 0;
   j.device_bytes = 0;
 }
 
@@ -237,22 +243,32 @@ fe_status job_set_tuning(Job &j, const Tuning &t) {
   return FE_OK;
 }
 
-static StepDesc make_desc(const Job &j, int b, bool delta_mode) {
+bool job_can_fuse(const Job &j) { return j.plan.batches > 1; }
+
+// Staging slot of batch b when batches hand rows over: slots alternate, and
+// with an odd batch count the final batch takes a third slot so that batch 0
+// of the next epoch never writes the slot it reads.
+static int out_slot(const Job &j, int b) {
+  const int nb = j.plan.batches;
+  if ((nb & 1) && b == nb - 1) return 2;
+  return b & 1;
+}
+
+static StepDesc make_desc(const Job &j, int b, DescKind kind) {
   StepDesc sd;
   std::memset(&sd, 0, sizeof sd);
+  const size_t slot = (size_t)j.plan.B * j.d;
   sd.X = j.X;
   sd.Xw = j.X;
   sd.Xs = j.Xs;
   sd.Xsw = j.Xs;
   sd.col = j.col;
-  sd.items = j.items + 4 * j.plan.item_off[b];
-  sd.splits = j.splits + 4 * j.plan.split_off[b];
-  sd.S = j.S;
+  sd.items = j.items + (size_t)kItemInts * j.plan.item_off[b];
   sd.P = j.P;
+  sd.counters = j.counters;
   sd.negs = j.negs;
   sd.epoch = j.epoch;
   sd.n_items = (int)(j.plan.item_off[b + 1] - j.plan.item_off[b]);
-  sd.n_splits = (int)(j.plan.split_off[b + 1] - j.plan.split_off[b]);
   sd.row_begin = (int)((int64_t)b * j.plan.B);
   sd.rows = rows_of_batch(j, b);
   rep_rows(j.plan, j.quirks, b, &sd.rep0, &sd.rep1);
@@ -263,11 +279,37 @@ static StepDesc make_desc(const Job &j, int b, bool delta_mode) {
   sd.G = j.G;
   sd.bf16 = j.dtype == FE_GATHER_BF16 ? 1 : 0;
   sd.c_limit = (int)j.c_limit;
-  sd.delta_mode = delta_mode ? 1 : 0;
-  sd.last_batch = (b == j.plan.batches - 1) ? 1 : 0;
   sd.lr = j.lr;
   sd.bound = j.bound;
+  sd.delta_mode = kind == DESC_DELTA ? 1 : 0;
+  if (kind == DESC_FUSED || kind == DESC_FIRST) {
+    sd.S = j.S + slot * out_slot(j, b);
+    if (kind == DESC_FUSED) {
+      const int pb = b > 0 ? b - 1 : j.plan.batches - 1;
+      sd.Sprev = j.S + slot * out_slot(j, pb);
+      sd.prev_begin = (int)((int64_t)pb * j.plan.B);
+      sd.prev_rows = rows_of_batch(j, pb);
+    }
+  } else {
+    sd.S = j.S;
+  }
   return sd;
+}
+
+const StepDesc *job_desc(const Job &j, int b, DescKind kind) {
+  const int nb = j.plan.batches;
+  switch (kind) {
+  case DESC_FUSED: return j.desc + b;
+  case DESC_SINGLE: return j.desc + nb + b;
+  case DESC_DELTA: return j.desc + 2 * nb + b;
+  default: return j.desc + 3 * nb;
+  }
+}
+
+fe_status job_reset_counters(Job &j, hipStream_t st) {
+  if (!j.counters) return FE_OK;
+  FE_HIP(j, hipMemsetAsync(j.counters, 0, sizeof(int) * j.cap_ctr, st), "counter reset");
+  return FE_OK;
 }
 
 fe_status job_prepare(Job &j, hipStream_t st) {
@@ -301,22 +343,20 @@ fe_status job_prepare(Job &j, hipStream_t st) {
       p.edges_eff_total += eff;
     }
     j.plan = std::move(p);
-    const size_t n_items = j.plan.items.size() / 4, n_splits = j.plan.splits.size() / 4;
-    if ((s = ensure_cap(j, j.items, j.cap_items, 4 * n_items, "work items")) != FE_OK) return s;
-    if ((s = ensure_cap(j, j.splits, j.cap_splits, 4 * std::max<size_t>(n_splits, 1),
-                        "split rows")) != FE_OK)
+    if ((s = ensure_cap(j, j.items, j.cap_items, j.plan.items.size(), "work items")) != FE_OK)
       return s;
-    if ((s = ensure_cap(j, j.S, j.cap_S, (size_t)j.B * j.d, "staging rows")) != FE_OK) return s;
+    if ((s = ensure_cap(j, j.S, j.cap_S, (size_t)3 * j.B * j.d, "staging rows")) != FE_OK)
+      return s;
     if ((s = ensure_cap(j, j.P, j.cap_P, (size_t)std::max(j.plan.max_pslots, 1) * j.d,
                         "partials")) != FE_OK)
+      return s;
+    if ((s = ensure_cap(j, j.counters, j.cap_ctr, (size_t)std::max(j.plan.max_splits, 1),
+                        "split counters")) != FE_OK)
       return s;
     FE_HIP(j, hipMemcpyAsync(j.items, j.plan.items.data(), sizeof(int32_t) * j.plan.items.size(),
                              hipMemcpyHostToDevice, st),
            "work items copy");
-    if (n_splits)
-      FE_HIP(j, hipMemcpyAsync(j.splits, j.plan.splits.data(),
-                               sizeof(int32_t) * j.plan.splits.size(), hipMemcpyHostToDevice, st),
-             "split rows copy");
+    if ((s = job_reset_counters(j, st)) != FE_OK) return s;
     FE_HIP(j, hipStreamSynchronize(st), "plan upload sync");
     j.plan_dirty = false;
     j.steps_dirty = true;
@@ -324,70 +364,52 @@ fe_status job_prepare(Job &j, hipStream_t st) {
   if (j.steps_dirty) {
     const int nb = j.plan.batches;
     fe_status s;
-    if ((s = ensure_cap(j, j.steps, j.cap_steps, (size_t)nb, "step descriptors")) != FE_OK)
+    if ((s = ensure_cap(j, j.desc, j.cap_desc, (size_t)3 * nb + 1, "step descriptors")) != FE_OK)
       return s;
-    if ((s = ensure_cap(j, j.steps_delta, j.cap_steps_d, (size_t)nb, "step descriptors")) != FE_OK)
+    if ((s = ensure_cap(j, j.prefix, j.cap_prefix, (size_t)nb * 4, "block prefixes")) != FE_OK)
       return s;
-    if ((s = ensure_cap(j, j.prefix, j.cap_prefix, (size_t)nb * 6, "block prefixes")) != FE_OK)
-      return s;
-    std::vector<StepDesc> hs(nb), hd(nb);
-    std::vector<int> pf((size_t)nb * 6, 0);
-    j.launches.assign(nb, StepLaunch());
+    std::vector<StepDesc> hd((size_t)3 * nb + 1);
+    std::vector<int> pf((size_t)nb * 4, 0);
+    j.force_blocks.assign(nb, 0);
+    j.apply_blocks.assign(nb, 0);
     const int ipb = j.tun.waves_per_block * j.tun.items_per_wave;
     for (int b = 0; b < nb; ++b) {
-      hs[b] = make_desc(j, b, false);
-      hd[b] = make_desc(j, b, true);
-      StepLaunch &L = j.launches[b];
-      L.force_blocks = (hs[b].n_items + ipb - 1) / ipb;
-      L.fixup_blocks = (hs[b].n_splits + j.tun.waves_per_block - 1) / j.tun.waves_per_block;
-      const int64_t q4 = (int64_t)hs[b].rows * j.d / 4;
-      L.apply_blocks = (int)std::min<int64_t>((q4 + 255) / 256, 4096);
-      pf[6 * b + 1] = L.force_blocks;
-      pf[6 * b + 3] = L.fixup_blocks;
-      pf[6 * b + 5] = L.apply_blocks;
+      hd[b] = make_desc(j, b, DESC_FUSED);
+      hd[nb + b] = make_desc(j, b, DESC_SINGLE);
+      hd[2 * nb + b] = make_desc(j, b, DESC_DELTA);
+      j.force_blocks[b] = std::max(1, (hd[b].n_items + ipb - 1) / ipb);
+      const int64_t q4 = (int64_t)hd[b].rows * j.d / 4;
+      j.apply_blocks[b] = (int)std::max<int64_t>(1, std::min<int64_t>((q4 + 255) / 256, 4096));
+      pf[4 * b + 1] = j.force_blocks[b];
+      pf[4 * b + 3] = j.apply_blocks[b];
     }
-    FE_HIP(j, hipMemcpyAsync(j.steps, hs.data(), sizeof(StepDesc) * nb, hipMemcpyHostToDevice, st),
-           "descriptor copy");
-    FE_HIP(j, hipMemcpyAsync(j.steps_delta, hd.data(), sizeof(StepDesc) * nb,
+    hd[3 * nb] = make_desc(j, 0, DESC_FIRST);
+    FE_HIP(j, hipMemcpyAsync(j.desc, hd.data(), sizeof(StepDesc) * hd.size(),
                              hipMemcpyHostToDevice, st),
            "descriptor copy");
     FE_HIP(j, hipMemcpyAsync(j.prefix, pf.data(), sizeof(int) * pf.size(), hipMemcpyHostToDevice,
                              st),
            "prefix copy");
     FE_HIP(j, hipStreamSynchronize(st), "descriptor sync");
-    j.host_steps.swap(hs);
+    j.host_desc.swap(hd);
     j.steps_dirty = false;
   }
   return FE_OK;
 }
 
-// Records `ev` on `st`; during stream capture the record becomes an
-// external event node, so every graph replay stamps the event again.
-static hipError_t record_event(hipEvent_t ev, hipStream_t st) {
-  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
-  hipError_t e = hipStreamIsCapturing(st, &cs);
-  if (e != hipSuccess) return e;
-  if (cs == hipStreamCaptureStatusActive) return hipEventRecordWithFlags(ev, st, hipEventRecordExternal);
-  return hipEventRecord(ev, st);
-}
-
-fe_status job_launch_batch(Job &j, int b, bool delta_mode, hipStream_t st, hipEvent_t *ev) {
-  const StepLaunch &L = j.launches[b];
-  const StepDesc *sd = (delta_mode ? j.steps_delta : j.steps) + b;
-  const int *pf = j.prefix + 6 * b;
+fe_status job_launch_force(Job &j, int b, DescKind kind, hipStream_t st, hipEvent_t *ev) {
   const size_t lds = (size_t)j.ns * j.d * sizeof(float);
   if (ev) FE_HIP(j, record_event(ev[0], st), "event");
-  FE_HIP(j, launch_force(sd, pf, 1, L.force_blocks, j.tun.waves_per_block, j.tun.items_per_wave,
-                         j.tun.unroll, lds, st),
+  FE_HIP(j, launch_force(job_desc(j, b, kind), j.prefix + 4 * b, 1, j.force_blocks[b],
+                         j.tun.waves_per_block, j.tun.items_per_wave, j.tun.unroll, lds, st),
          "force kernel launch");
   if (ev) FE_HIP(j, record_event(ev[1], st), "event");
-  if (L.fixup_blocks > 0)
-    FE_HIP(j, launch_fixup(sd, pf + 2, 1, L.fixup_blocks, j.tun.waves_per_block, 1, j.tun.unroll,
-                           lds, st),
-           "fixup kernel launch");
-  if (ev) FE_HIP(j, record_event(ev[2], st), "event");
-  if (!delta_mode) FE_HIP(j, launch_apply(sd, pf + 4, 1, L.apply_blocks, st), "apply kernel launch");
-  if (ev) FE_HIP(j, record_event(ev[3], st), "event");
+  return FE_OK;
+}
+
+fe_status job_launch_apply(Job &j, int b, DescKind kind, hipStream_t st) {
+  FE_HIP(j, launch_apply(job_desc(j, b, kind), j.prefix + 4 * b + 2, 1, j.apply_blocks[b], st),
+         "apply kernel launch");
   return FE_OK;
 }
 
@@ -402,7 +424,7 @@ fe_status job_loss(Job &j, uint64_t seed, int eval_ns, double *attr, double *rep
   ld.X = j.X;
   ld.col = j.col;
   ld.items = j.items;
-  ld.n_items = (int)(j.plan.items.size() / 4);
+  ld.n_items = (int)(j.plan.items.size() / kItemInts);
   ld.D = j.d;
   ld.G = j.G;
   ld.c_limit = (int)j.c_limit;
@@ -450,10 +472,11 @@ int64_t job_bytes_estimate(const Job &j) {
   const int64_t nd = j.n * (int64_t)j.d;
   int64_t b = nd * 4 + (j.dtype == FE_GATHER_BF16 ? nd * 2 : 0);
   b += std::max<int64_t>(j.nnz, 1) * 4;
-  b += (int64_t)j.plan.items.size() * 4 + (int64_t)std::max<size_t>(j.plan.splits.size(), 4) * 4;
-  b += (int64_t)j.plan.B * j.d * 4 + (int64_t)std::max(j.plan.max_pslots, 1) * j.d * 4;
+  b += (int64_t)j.plan.items.size() * 4;
+  b += 3 * (int64_t)j.plan.B * j.d * 4 + (int64_t)std::max(j.plan.max_pslots, 1) * j.d * 4;
+  b += (int64_t)std::max(j.plan.max_splits, 1) * 4;
   b += (int64_t)j.n_lists * j.ns * 4;
-  b += (int64_t)j.plan.batches * (2 * (int64_t)sizeof(StepDesc) + 24) + 4;
+  b += (int64_t)(3 * j.plan.batches + 1) * (int64_t)sizeof(StepDesc) + 16LL * j.plan.batches + 4;
   return b;
 }
 

@@ -15,10 +15,9 @@ namespace fe {
 // Launch wrappers (fe_kernels.hip).
 hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
                         int wpb, int ipw, int unroll, size_t lds_bytes, hipStream_t st);
-hipError_t launch_fixup(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
-                        int wpb, int ipw, int unroll, size_t lds_bytes, hipStream_t st);
 hipError_t launch_apply(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
                         hipStream_t st);
+hipError_t launch_tick(int *const *ctrs, int n, hipStream_t st);
 hipError_t launch_set_int(int *p, int v, hipStream_t st);
 hipError_t launch_to_bf16(const float *x, uint16_t *xs, int64_t count, hipStream_t st);
 hipError_t launch_loss_edges(const LossDesc &ld, double *block_out, int n_blocks, int ipw,
@@ -35,14 +34,23 @@ struct Tuning {
   int items_per_wave = 1;
 };
 
-// Block counts of the three kernels for one step.
-struct StepLaunch {
-  int force_blocks = 0;
-  int fixup_blocks = 0;
-  int apply_blocks = 0;
+// Descriptor variants of one batch.
+//   FUSED: the previous batch's rows are pending in its staging slot; this
+//          launch reads them from there and copies them into X.
+//   FIRST: batch 0 of a run, nothing pending.
+//   SINGLE: nothing pending; an apply launch follows (fe_step).
+//   DELTA: nothing pending; S receives the delta, no apply (fe_step_delta).
+enum DescKind { DESC_FUSED = 0, DESC_SINGLE = 1, DESC_DELTA = 2, DESC_FIRST = 3 };
+
+// Device memory source of a job: hipMalloc by default, or an engine arena.
+struct DeviceAllocator {
+  virtual ~DeviceAllocator() = default;
+  virtual hipError_t alloc(void **p, size_t bytes) = 0;
+  virtual void release(void *p) = 0;
 };
 
 struct Job {
+  DeviceAllocator *allocator = nullptr; // null: hipMalloc / hipFree
   // shape
   int64_t n = 0;
   int d = 0;
@@ -63,23 +71,23 @@ struct Job {
   int n_lists = 0;
   Tuning tun;
   Plan plan;
-  std::vector<StepLaunch> launches; // per batch
+  std::vector<int> force_blocks; // per batch
+  std::vector<int> apply_blocks; // per batch
   // device buffers
   float *X = nullptr;
   uint16_t *Xs = nullptr;
   int32_t *col = nullptr;
   int32_t *items = nullptr;
-  int32_t *splits = nullptr;
-  float *S = nullptr;
+  float *S = nullptr;        // 3 staging slots of B x d
   float *P = nullptr;
+  int *counters = nullptr;
   int32_t *negs = nullptr;
   int *epoch = nullptr;
-  StepDesc *steps = nullptr;      // per batch (normal mode)
-  StepDesc *steps_delta = nullptr; // per batch (delta mode)
-  int *prefix = nullptr;          // per batch: 3 kernels x {0, blocks}
-  std::vector<StepDesc> host_steps;
-  size_t cap_items = 0, cap_splits = 0, cap_S = 0, cap_P = 0, cap_negs = 0, cap_steps = 0;
-  size_t cap_steps_d = 0, cap_prefix = 0, cap_col = 0;
+  StepDesc *desc = nullptr;  // 3 * batches + 1: FUSED[b], SINGLE[b], DELTA[b], FIRST
+  int *prefix = nullptr;     // per batch: {0, force blocks}, {0, apply blocks}
+  std::vector<StepDesc> host_desc;
+  size_t cap_items = 0, cap_S = 0, cap_P = 0, cap_ctr = 0, cap_negs = 0, cap_desc = 0;
+  size_t cap_prefix = 0, cap_col = 0, cap_X = 0, cap_Xs = 0, cap_epoch = 0;
   bool plan_dirty = true;  // plan must be rebuilt (batch size, chunking, CSR)
   bool steps_dirty = true; // descriptors must be rebuilt
   int64_t device_bytes = 0;
@@ -98,14 +106,20 @@ fe_status job_configure(Job &j, int B, float lr, float bound);
 fe_status job_set_tuning(Job &j, const Tuning &t);
 // Builds the plan / descriptors if dirty and uploads them.
 fe_status job_prepare(Job &j, hipStream_t st);
-// Launches batch `b` (force, fixup, apply) on `st`; the epoch counter must
-// already hold the epoch. delta_mode: skip apply, S holds the delta.
-fe_status job_launch_batch(Job &j, int b, bool delta_mode, hipStream_t st,
-                           hipEvent_t *ev /* 4 events or null */);
+// Zeroes the split counters (outside any graph capture).
+fe_status job_reset_counters(Job &j, hipStream_t st);
+const StepDesc *job_desc(const Job &j, int b, DescKind kind);
+// Force launch of batch `b` with descriptor `kind`; ev = {start, end} or null.
+fe_status job_launch_force(Job &j, int b, DescKind kind, hipStream_t st, hipEvent_t *ev);
+// Apply launch (S -> X) of batch `b` with descriptor `kind`.
+fe_status job_launch_apply(Job &j, int b, DescKind kind, hipStream_t st);
+// Whether consecutive batches can hand rows over through staging slots.
+bool job_can_fuse(const Job &j);
 fe_status job_loss(Job &j, uint64_t seed, int eval_ns, double *attr, double *rep, hipStream_t st);
-// Device bytes the job holds once prepared (plan must be built).
+// Device bytes a prepared job holds.
 int64_t job_bytes_estimate(const Job &j);
 fe_status job_check(Job &j, hipError_t e, const char *what);
 int rows_of_batch(const Job &j, int b);
+hipError_t record_event(hipEvent_t ev, hipStream_t st);
 
 } // namespace fe

@@ -18,9 +18,13 @@
 // heavy row's edges); its groups take the item's edges round-robin, U rows in
 // flight per group, and the group accumulators are combined at the end with
 // a fixed xor tree. Heavy rows are split into chunks whose partial deltas
-// are summed in index order by the fixup kernel, so every result is
-// deterministic. The batch's negative rows are staged once per block in LDS.
-// No MFMA: the step is a sparse gather, not a dense contraction.
+// are summed in index order by whichever chunk finishes last (agent-scope
+// release/acquire around a per-row counter), so every result is
+// deterministic and independent of the launch shape. The batch's negative
+// rows are staged once per block in LDS. Batch j's launch also copies batch
+// j-1's new rows from staging into X, reading them from staging meanwhile,
+// so one launch per minibatch suffices. No MFMA: the step is a sparse
+// gather, not a dense contraction.
 #include <hip/hip_runtime.h>
 #include "fe_device.hpp"
 
@@ -43,27 +47,73 @@ __device__ __forceinline__ float group_sum(float v, int G) {
   return v;
 }
 
+// Device-memory accesses go through address-space-1 pointers so they are
+// emitted as global_* instructions (counted on vmcnt only) rather than flat_*
+// ones, whose waits also drain LDS and scalar traffic.
+typedef float f4v __attribute__((ext_vector_type(4)));
+typedef unsigned int u4v __attribute__((ext_vector_type(4)));
+typedef __attribute__((address_space(1))) const f4v gcf4;
+typedef __attribute__((address_space(1))) f4v gf4;
+typedef __attribute__((address_space(1))) const u4v gcu4;
+typedef __attribute__((address_space(1))) const int gcint;
+
+__device__ __forceinline__ int gld_int(const int32_t *p) { return *(gcint *)p; }
+
+__device__ __forceinline__ u4v gld_u4(const int32_t *p) { return *(gcu4 *)p; }
+
 __device__ __forceinline__ void load8_f32(const float *p, float (&o)[8]) {
-  const float4 *q = reinterpret_cast<const float4 *>(p);
-  float4 a = q[0];
-  float4 b = q[1];
+  const f4v a = ((gcf4 *)p)[0];
+  const f4v b = ((gcf4 *)p)[1];
   o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
   o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
 }
 
 __device__ __forceinline__ void load8_bf16(const uint16_t *p, float (&o)[8]) {
-  uint4 w = *reinterpret_cast<const uint4 *>(p);
+  const u4v w = *(gcu4 *)p;
   o[0] = __uint_as_float(w.x << 16); o[1] = __uint_as_float(w.x & 0xFFFF0000u);
   o[2] = __uint_as_float(w.y << 16); o[3] = __uint_as_float(w.y & 0xFFFF0000u);
   o[4] = __uint_as_float(w.z << 16); o[5] = __uint_as_float(w.z & 0xFFFF0000u);
   o[6] = __uint_as_float(w.w << 16); o[7] = __uint_as_float(w.w & 0xFFFF0000u);
 }
 
+// Global memory only (never an LDS pointer: see store8_lds).
 __device__ __forceinline__ void store8_f32(float *p, const float (&o)[8]) {
-  float4 *q = reinterpret_cast<float4 *>(p);
-  q[0] = make_float4(o[0], o[1], o[2], o[3]);
-  q[1] = make_float4(o[4], o[5], o[6], o[7]);
+  f4v a = {o[0], o[1], o[2], o[3]};
+  f4v b = {o[4], o[5], o[6], o[7]};
+  ((gf4 *)p)[0] = a;
+  ((gf4 *)p)[1] = b;
 }
+
+typedef __attribute__((address_space(3))) f4v lf4;
+
+__device__ __forceinline__ void store8_lds(float *p, const float (&o)[8]) {
+  f4v a = {o[0], o[1], o[2], o[3]};
+  f4v b = {o[4], o[5], o[6], o[7]};
+  ((lf4 *)p)[0] = a;
+  ((lf4 *)p)[1] = b;
+}
+
+// Partials of heavy rows cross CUs (and XCDs, whose L2s are not coherent with
+// each other) inside one launch. They are written and read with agent-scope
+// relaxed atomics, which the hardware performs at the device-coherent level
+// (sc1), so no L2 writeback or invalidation is needed around them.
+typedef __attribute__((address_space(1))) float gfloat;
+typedef __attribute__((address_space(1))) int gint;
+
+__device__ __forceinline__ void store8_coherent(float *p, const float (&o)[8]) {
+  gfloat *g = (gfloat *)p;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) __hip_atomic_store(g + k, o[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void load8_coherent(const float *p, float (&o)[8]) {
+  gfloat *g = (gfloat *)p;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) o[k] = __hip_atomic_load(g + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// s_waitcnt vmcnt(0): this wave's stores have completed.
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // FP32 -> BF16, round to nearest even; NaN stays a quiet NaN.
 __device__ __forceinline__ uint32_t bf16_bits(float f) {
@@ -88,25 +138,36 @@ __device__ __forceinline__ int find_step(const int *__restrict__ prefix,
   return lo;
 }
 
-// Stage the `ns` negative rows of list `list` as FP32 in LDS.
-__device__ __forceinline__ void stage_negatives(const StepDesc &sd, int list,
-                                                float *lds) {
+// Row c as 8 FP32 components [8q, 8q+8). Rows of the previous batch whose
+// new values still sit in Sprev are read from there; with BF16 gathers they
+// are rounded exactly as the shadow refresh will round them.
+__device__ __forceinline__ void load_row(const StepDesc &sd, int c, int q, float (&o)[8]) {
   const int D = sd.D;
-  const int total = sd.ns * D;
-  const int32_t *nl = sd.negs + (size_t)list * sd.ns;
-  for (int idx = threadIdx.x * 4; idx < total; idx += blockDim.x * 4) {
-    int k = idx / D;
-    int comp = idx - k * D;
-    size_t n = (size_t)nl[k];
-    float4 v;
+  const unsigned off = (unsigned)(c - sd.prev_begin);
+  if (off < (unsigned)sd.prev_rows) {
+    load8_f32(sd.Sprev + (size_t)off * D + 8 * q, o);
     if (sd.bf16) {
-      uint2 w = *reinterpret_cast<const uint2 *>(sd.Xs + n * D + comp);
-      v = make_float4(__uint_as_float(w.x << 16), __uint_as_float(w.x & 0xFFFF0000u),
-                      __uint_as_float(w.y << 16), __uint_as_float(w.y & 0xFFFF0000u));
-    } else {
-      v = *reinterpret_cast<const float4 *>(sd.X + n * D + comp);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) o[k] = __uint_as_float(bf16_bits(o[k]) << 16);
     }
-    *reinterpret_cast<float4 *>(lds + idx) = v;
+  } else if (sd.bf16) {
+    load8_bf16(sd.Xs + (size_t)c * D + 8 * q, o);
+  } else {
+    load8_f32(sd.X + (size_t)c * D + 8 * q, o);
+  }
+}
+
+// Stage the `ns` negative rows of list `list` as FP32 in LDS.
+__device__ __forceinline__ void stage_negatives(const StepDesc &sd, int list, float *lds) {
+  const int G = sd.G;
+  const int total8 = sd.ns * G; // 8-component pieces
+  const int32_t *nl = sd.negs + (size_t)list * sd.ns;
+  for (int p = threadIdx.x; p < total8; p += blockDim.x) {
+    const int k = p / G;
+    const int q = p - k * G;
+    float v[8];
+    load_row(sd, gld_int(nl + k), q, v);
+    store8_lds(lds + (size_t)k * sd.D + 8 * q, v);
   }
 }
 
@@ -129,7 +190,7 @@ __device__ __forceinline__ void repulse(const StepDesc &sd, const float *lds,
 #pragma unroll
     for (int k2 = 0; k2 < 8; ++k2) {
       v[k2] = ok ? xr[k2] - xn[k2] : 0.f;
-      p += v[k2] * v[k2];
+      p = fmaf(v[k2], v[k2], p);
     }
     const float s = group_sum(p, G);
     const float d1 = 2.0f / ((s + 1e-6f) * (1.0f + s));
@@ -145,6 +206,35 @@ __device__ __forceinline__ void combine_groups(float (&acc)[8], int G) {
   }
 }
 
+// Sum partials p0..p0+k-1: group g takes g, g+NG, ... in order, then the
+// fixed xor tree of combine_groups.
+template <int U>
+__device__ __forceinline__ void sum_partials(const StepDesc &sd, int p0, int k, int NG, int g,
+                                             int q, float (&acc)[8]) {
+  const int D = sd.D;
+  const int rounds = (k + NG - 1) / NG;
+  for (int u0 = 0; u0 < rounds; u0 += U) {
+    float pv[U][8];
+    bool ok[U];
+#pragma unroll
+    for (int uu = 0; uu < U; ++uu) {
+      const int idx = (u0 + uu) * NG + g;
+      ok[uu] = idx < k;
+      if (ok[uu]) load8_coherent(sd.P + (size_t)(p0 + idx) * D + 8 * q, pv[uu]);
+    }
+#pragma unroll
+    for (int uu = 0; uu < U; ++uu) {
+      if (ok[uu]) {
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) acc[k2] += pv[uu][k2];
+      }
+    }
+  }
+}
+
+// One minibatch of one or more jobs: attractive gathers, repulsion against the
+// staged negatives, heavy rows reduced by their last chunk, new rows to S;
+// then this block's share of the previous batch's rows is copied Sprev -> X.
 template <int U>
 __global__ __launch_bounds__(512) void force_kernel(
     const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
@@ -174,21 +264,30 @@ __global__ __launch_bounds__(512) void force_kernel(
   for (int i = 0; i < ipw; ++i) {
     const int it = (lb * ipw + i) * wpb + wave;
     if (it >= sd.n_items) break;
-    const int r = sd.items[4 * it + 0];
-    const int e0 = sd.items[4 * it + 1];
-    const int cnt = sd.items[4 * it + 2];
-    const int pslot = sd.items[4 * it + 3];
+    const int32_t *item = sd.items + (size_t)kItemInts * it;
+    const u4v ia = gld_u4(item);
+    const int r = (int)ia.x;
+    const int e0 = (int)ia.y;
+    const int cnt = (int)ia.z;
+    const int pslot = (int)ia.w;
     const int t = r - sd.row_begin;
 
     float xr[8];
     load8_f32(sd.X + (size_t)r * D + 8 * q, xr);
+    // With BF16 gathers the distances use the row as the shadow holds it
+    // (rounded here from the master, no extra traffic), so a row meeting
+    // itself as a negative or through a self-loop feels exactly zero force,
+    // as in the reference; the update still goes to the FP32 master.
+    float xd[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) xd[k] = sd.bf16 ? __uint_as_float(bf16_bits(xr[k]) << 16) : xr[k];
     float acc[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) acc[k] = 0.f;
 
     for (int eb = 0; eb < cnt; eb += 64) {
       const int m = min(64, cnt - eb);
-      const int cl = lane < m ? sd.col[e0 + eb + lane] : 0;
+      const int cl = lane < m ? gld_int(sd.col + e0 + eb + lane) : 0;
       const int rounds = (m + NG - 1) / NG;
       for (int u0 = 0; u0 < rounds; u0 += U) {
         float xc[U][8];
@@ -199,8 +298,7 @@ __global__ __launch_bounds__(512) void force_kernel(
           const int c = __shfl(cl, k & 63);
           ok[uu] = (k < m) && (c < c_limit);
           if (ok[uu]) {
-            if (sd.bf16) load8_bf16(sd.Xs + (size_t)c * D + 8 * q, xc[uu]);
-            else load8_f32(sd.X + (size_t)c * D + 8 * q, xc[uu]);
+            load_row(sd, c, q, xc[uu]);
           } else {
 #pragma unroll
             for (int k2 = 0; k2 < 8; ++k2) xc[uu][k2] = 0.f;
@@ -212,8 +310,8 @@ __global__ __launch_bounds__(512) void force_kernel(
           float p = 0.f;
 #pragma unroll
           for (int k2 = 0; k2 < 8; ++k2) {
-            v[k2] = ok[uu] ? xr[k2] - xc[uu][k2] : 0.f;
-            p += v[k2] * v[k2];
+            v[k2] = ok[uu] ? xd[k2] - xc[uu][k2] : 0.f;
+            p = fmaf(v[k2], v[k2], p);
           }
           const float s = group_sum(p, G);
           const float d1 = -2.0f / (1.0f + s);
@@ -223,12 +321,35 @@ __global__ __launch_bounds__(512) void force_kernel(
         }
       }
     }
-    if (pslot < 0 && has_neg && t < rep) repulse(sd, lds_neg, xr, acc, G, NG, g, q);
-    combine_groups(acc, G);
-    if (g == 0) {
-      if (pslot >= 0) {
-        store8_f32(sd.P + (size_t)pslot * D + 8 * q, acc);
-      } else {
+
+    bool finish = pslot < 0;
+    if (!finish) {
+      const u4v ib = gld_u4(item + 4);
+      const int sid = (int)ib.x, p0 = (int)ib.y, kparts = (int)ib.z;
+      // heavy row: publish this chunk's partial (device-coherent stores, then
+      // wait for them to complete), count it, and let the chunk that counts
+      // last sum all partials of the row
+      combine_groups(acc, G);
+      if (g == 0) store8_coherent(sd.P + (size_t)pslot * D + 8 * q, acc);
+      wait_stores();
+      int old = 0;
+      if (lane == 0)
+        old = __hip_atomic_fetch_add((gint *)(sd.counters + sid), 1, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+      old = __shfl(old, 0);
+      if (old == kparts - 1) {
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) acc[k2] = 0.f;
+        sum_partials<U>(sd, p0, kparts, NG, g, q, acc);
+        if (lane == 0)
+          __hip_atomic_store((gint *)(sd.counters + sid), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        finish = true;
+      }
+    }
+    if (finish) {
+      if (has_neg && t < rep) repulse(sd, lds_neg, xd, acc, G, NG, g, q);
+      combine_groups(acc, G);
+      if (g == 0) {
         float o[8];
 #pragma unroll
         for (int k = 0; k < 8; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
@@ -236,77 +357,33 @@ __global__ __launch_bounds__(512) void force_kernel(
       }
     }
   }
-}
 
-// Heavy rows: sum the chunk partials in index order, add the repulsion and
-// write the new row.
-template <int U>
-__global__ __launch_bounds__(512) void fixup_kernel(
-    const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
-    int n_steps, int ipw) {
-  extern __shared__ float lds_neg[];
-  const int b = blockIdx.x;
-  const int si = find_step(blk_prefix, n_steps, b);
-  const StepDesc &sd = steps[si];
-  const int lb = b - blk_prefix[si];
-  const int epoch = *sd.epoch;
-  const int rep = epoch == 0 ? sd.rep0 : sd.rep1;
-  const int list = epoch + sd.batch;
-  const bool has_neg = sd.ns > 0 && list < sd.n_lists;
-  if (has_neg) stage_negatives(sd, list, lds_neg);
-  __syncthreads();
-
-  const int D = sd.D, G = sd.G;
-  const int NG = 64 / G;
-  const int lane = threadIdx.x & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int wpb = blockDim.x >> 6;
-  const int g = lane / G;
-  const int q = lane & (G - 1);
-
-  for (int i = 0; i < ipw; ++i) {
-    const int it = (lb * ipw + i) * wpb + wave;
-    if (it >= sd.n_splits) break;
-    const int r = sd.splits[4 * it + 0];
-    const int p0 = sd.splits[4 * it + 1];
-    const int k = sd.splits[4 * it + 2];
-    const int t = r - sd.row_begin;
-    float xr[8];
-    load8_f32(sd.X + (size_t)r * D + 8 * q, xr);
-    float acc[8];
-#pragma unroll
-    for (int k2 = 0; k2 < 8; ++k2) acc[k2] = 0.f;
-    const int rounds = (k + NG - 1) / NG;
-    for (int u0 = 0; u0 < rounds; u0 += U) {
-      float pv[U][8];
-      bool ok[U];
-#pragma unroll
-      for (int uu = 0; uu < U; ++uu) {
-        const int idx = (u0 + uu) * NG + g;
-        ok[uu] = idx < k;
-        if (ok[uu]) load8_f32(sd.P + (size_t)(p0 + idx) * D + 8 * q, pv[uu]);
+  // copy phase: the previous batch's new rows into X (and the shadow); no
+  // thread of this launch reads those rows of X (load_row redirects them)
+  if (sd.prev_rows > 0) {
+    const int nb = blk_prefix[si + 1] - blk_prefix[si];
+    const int64_t total4 = (int64_t)sd.prev_rows * D / 4;
+    const int64_t per = (total4 + nb - 1) / nb;
+    const int64_t lo = (int64_t)lb * per;
+    const int64_t hi = min(total4, lo + per);
+    const int64_t base4 = (int64_t)sd.prev_begin * D / 4;
+    const float4 *S4 = reinterpret_cast<const float4 *>(sd.Sprev);
+    float4 *X4 = reinterpret_cast<float4 *>(sd.Xw);
+    for (int64_t idx = lo + threadIdx.x; idx < hi; idx += blockDim.x) {
+      const float4 v = S4[idx];
+      X4[base4 + idx] = v;
+      if (sd.bf16) {
+        uint2 w;
+        w.x = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
+        w.y = bf16_bits(v.z) | (bf16_bits(v.w) << 16);
+        reinterpret_cast<uint2 *>(sd.Xsw)[base4 + idx] = w;
       }
-#pragma unroll
-      for (int uu = 0; uu < U; ++uu) {
-        if (ok[uu]) {
-#pragma unroll
-          for (int k2 = 0; k2 < 8; ++k2) acc[k2] += pv[uu][k2];
-        }
-      }
-    }
-    if (has_neg && t < rep) repulse(sd, lds_neg, xr, acc, G, NG, g, q);
-    combine_groups(acc, G);
-    if (g == 0) {
-      float o[8];
-#pragma unroll
-      for (int k2 = 0; k2 < 8; ++k2) o[k2] = sd.delta_mode ? acc[k2] : xr[k2] + acc[k2];
-      store8_f32(sd.S + (size_t)t * D + 8 * q, o);
     }
   }
 }
 
 // X[batch] = S (cpp/algo/algo.hpp:651-666 with the add already folded into
-// S), BF16 shadow refresh of the same rows, epoch counter advance.
+// S) and the BF16 shadow refresh of the same rows.
 __global__ __launch_bounds__(256) void apply_kernel(
     const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
     int n_steps) {
@@ -330,11 +407,15 @@ __global__ __launch_bounds__(256) void apply_kernel(
       reinterpret_cast<uint2 *>(sd.Xsw)[base4 + idx] = w;
     }
   }
-  if (lb == 0 && threadIdx.x == 0 && sd.last_batch) *sd.epoch = *sd.epoch + 1;
 }
 
 __global__ void set_int_kernel(int *p, int v) {
   if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
+}
+
+// Advances epoch counters (one per job) at the start of an epoch.
+__global__ void tick_kernel(int *const *ctrs, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) *ctrs[i] = *ctrs[i] + 1;
 }
 
 __global__ void to_bf16_kernel(const float *__restrict__ x, uint16_t *__restrict__ xs,
@@ -390,9 +471,9 @@ __global__ __launch_bounds__(256) void loss_edges_kernel(LossDesc ld, double *bl
   for (int i = 0; i < ipw; ++i) {
     const int it = (blockIdx.x * ipw + i) * wpb + wave;
     if (it >= ld.n_items) break;
-    const int r = ld.items[4 * it + 0];
-    const int e0 = ld.items[4 * it + 1];
-    const int cnt = ld.items[4 * it + 2];
+    const int r = ld.items[kItemInts * it + 0];
+    const int e0 = ld.items[kItemInts * it + 1];
+    const int cnt = ld.items[kItemInts * it + 2];
     float xr[8];
     load8_f32(ld.X + (size_t)r * D + 8 * q, xr);
     for (int eb = 0; eb < cnt; eb += 64) {
@@ -409,7 +490,7 @@ __global__ __launch_bounds__(256) void loss_edges_kernel(LossDesc ld, double *bl
 #pragma unroll
         for (int k2 = 0; k2 < 8; ++k2) {
           const float v = xr[k2] - xc[k2];
-          p += v * v;
+          p = fmaf(v, v, p);
         }
         const float s = group_sum(p, G);
         if (ok && q == 0) tot += log1p((double)s);
@@ -442,7 +523,7 @@ __global__ __launch_bounds__(256) void loss_negs_kernel(LossDesc ld, double *blo
 #pragma unroll
     for (int k2 = 0; k2 < 8; ++k2) {
       const float v = xr[k2] - xm[k2];
-      s2 += v * v;
+      s2 = fmaf(v, v, s2);
     }
     const float s = group_sum(s2, G);
     if (ok && q == 0) {
@@ -484,16 +565,8 @@ hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_step
   return hipGetLastError();
 }
 
-hipError_t launch_fixup(const StepDesc *steps, const int *blk_prefix, int n_steps,
-                        int n_blocks, int wpb, int ipw, int unroll, size_t lds_bytes,
-                        hipStream_t st) {
-  dim3 grid(n_blocks), block(64 * wpb);
-  switch (unroll) {
-  case 1: hipLaunchKernelGGL(fixup_kernel<1>, grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw); break;
-  case 2: hipLaunchKernelGGL(fixup_kernel<2>, grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw); break;
-  case 8: hipLaunchKernelGGL(fixup_kernel<8>, grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw); break;
-  default: hipLaunchKernelGGL(fixup_kernel<4>, grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw); break;
-  }
+hipError_t launch_tick(int *const *ctrs, int n, hipStream_t st) {
+  hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(256), 0, st, ctrs, n);
   return hipGetLastError();
 }
 

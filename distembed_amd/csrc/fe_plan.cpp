@@ -1,6 +1,7 @@
 // Work-plan construction (host). See Plan in fe_host.hpp.
 #include <algorithm>
 
+#include "fe_device.hpp"
 #include "fe_host.hpp"
 
 namespace fe {
@@ -27,10 +28,11 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
   p.split_off.assign(p.batches + 1, 0);
   p.pslots.assign(p.batches, 0);
   p.edges_eff.assign(p.batches, 0);
+  p.items.reserve((size_t)kItemInts * (size_t)(n + (row_ptr[n] / chunk) + 16));
   for (int j = 0; j < p.batches; ++j) {
     const int64_t r0 = (int64_t)j * B;
     const int64_t r1 = std::min<int64_t>(r0 + B, n);
-    int32_t pcount = 0;
+    int32_t pcount = 0, scount = 0;
     int64_t eff = 0;
     for (int64_t r = r0; r < r1; ++r) {
       const int64_t e0 = row_ptr[r];
@@ -39,23 +41,25 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
         for (int64_t e = e0; e < e0 + deg; ++e) eff += (col[e] < c_limit) ? 1 : 0;
       }
       if (deg <= chunk) {
-        p.items.insert(p.items.end(), {(int32_t)r, (int32_t)e0, (int32_t)deg, -1});
+        p.items.insert(p.items.end(), {(int32_t)r, (int32_t)e0, (int32_t)deg, -1, 0, 0, 0, 0});
         continue;
       }
       const int64_t per = std::max<int64_t>(chunk, (deg + max_chunks - 1) / max_chunks);
       const int64_t k = (deg + per - 1) / per;
       for (int64_t i = 0; i < k; ++i) {
         const int64_t lo = deg * i / k, hi = deg * (i + 1) / k;
-        p.items.insert(p.items.end(),
-                       {(int32_t)r, (int32_t)(e0 + lo), (int32_t)(hi - lo), (int32_t)(pcount + i)});
+        p.items.insert(p.items.end(), {(int32_t)r, (int32_t)(e0 + lo), (int32_t)(hi - lo),
+                                       (int32_t)(pcount + i), scount, pcount, (int32_t)k, 0});
       }
       p.splits.insert(p.splits.end(), {(int32_t)r, pcount, (int32_t)k, 0});
       pcount += (int32_t)k;
+      ++scount;
     }
-    p.item_off[j + 1] = (int64_t)p.items.size() / 4;
+    p.item_off[j + 1] = (int64_t)p.items.size() / kItemInts;
     p.split_off[j + 1] = (int64_t)p.splits.size() / 4;
     p.pslots[j] = pcount;
     p.max_pslots = std::max(p.max_pslots, (int)pcount);
+    p.max_splits = std::max(p.max_splits, (int)scount);
     p.edges_eff[j] = eff;
     p.edges_eff_total += eff;
   }
@@ -65,7 +69,7 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
 
 void rep_rows(const Plan &p, uint32_t quirks, int j, int *rep0, int *rep1) {
   // considering_batch_size starts at B and becomes `last` at the final batch
-  // of every epoch; without the reset quirk it is B for the other batches.
+  // of every epoch; without the sticky quirk it is B for the other batches.
   const bool final_batch = (j == p.batches - 1);
   *rep0 = final_batch ? p.last : p.B;
   if (final_batch) *rep1 = p.last;

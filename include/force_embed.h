@@ -200,6 +200,63 @@ fe_status fe_read_mtx(const char *path, fe_graph *out);
 fe_status fe_write_embedding(const char *path, const float *x, int64_t n,
                              int d);
 
+/* ---- multi-job engine: batching and serving forms ------------------------
+ * No counterpart in the reference, which embeds one graph per process run
+ * (cpp/dist_embed.cpp:330-347); each job below is one such run.
+ * An engine owns one device, two streams (rounds; uploads and downloads) and
+ * an HBM budget held as one arena. Jobs (independent graphs with their own
+ * CSR, X, d, gather dtype, batch size, epochs and negative lists) are
+ * admitted while their device state fits the free budget. Every engine round
+ * advances each running job by one minibatch, all jobs' work items sharing
+ * one force launch and one apply launch. A job's chunking is fixed when it is
+ * submitted and no job touches another job's memory, so its result does not
+ * depend on which other jobs share its rounds.                              */
+typedef struct fe_engine fe_engine;
+typedef struct fe_job_desc {
+  int64_t n;
+  int d;
+  int gather_dtype;
+  uint32_t quirks;
+  int batch_size;
+  int iterations;
+  int ns;
+  float lr;
+  float max_bound;
+  const int64_t *row_ptr;   /* borrowed during fe_engine_submit only         */
+  const int32_t *col_idx;
+  int64_t nnz;
+  const float *x0;          /* n x d                                          */
+  const int32_t *neg_lists; /* (iterations + batches - 1) x ns, or NULL for
+                               the reference lists (FE_NEG_REDRAW)            */
+} fe_job_desc;
+
+fe_status fe_engine_create(fe_engine **out, int device, int64_t hbm_budget_bytes);
+void fe_engine_destroy(fe_engine *e);
+const char *fe_engine_last_error(const fe_engine *e);
+/* Bytes of the budget the job would hold while admitted (-1: invalid). */
+int64_t fe_engine_job_bytes(const fe_engine *e, const fe_job_desc *desc);
+/* Copies the job to the device (upload stream) if it fits the free budget,
+ * else FE_ERR_OVER_BUDGET with nothing kept. *job_id names it afterwards. */
+fe_status fe_engine_submit(fe_engine *e, const fe_job_desc *desc, int *job_id);
+/* Enqueues `rounds` rounds (all remaining when rounds <= 0); *running gets
+ * the number of jobs with minibatches still to enqueue.                    */
+fe_status fe_engine_run(fe_engine *e, int rounds, int *running);
+/* Batching form: runs every submitted job to completion from a schedule
+ * built up front, its rounds captured in HIP graphs and replayed back to
+ * back. Timing of the last call: fe_engine_graph_timing.                   */
+fe_status fe_engine_run_graph(fe_engine *e);
+fe_status fe_engine_graph_timing(const fe_engine *e, double *prepare_ms,
+                                 double *exec_ms, int64_t *rounds);
+/* *done = 1 once the job's last minibatch has finished on the device.     */
+fe_status fe_engine_job_done(fe_engine *e, int job_id, int *done);
+/* Copies a finished job's X (n x d) to host and returns its memory to the
+ * budget. Waits only for that job's last round.                           */
+fe_status fe_engine_collect(fe_engine *e, int job_id, float *x_out);
+/* Launch shape for later rounds and chunking for later submissions.        */
+fe_status fe_engine_set_tuning(fe_engine *e, const fe_tuning *t);
+fe_status fe_engine_stats(const fe_engine *e, int64_t *bytes_in_use,
+                          int64_t *rounds_run, int64_t *units_done);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,114 @@
+"""Multi-job engine (batching and serving forms) on the GPU.
+
+Each job's result must be bitwise the same whether it runs alone, mixed
+with other jobs in shared rounds, through the captured-graph schedule, or
+as a single-graph context (fe_train): the work items and summation order of
+a job are fixed at submission and no job touches another's memory.
+"""
+import numpy as np
+import pytest
+
+from distembed_amd import api
+
+pytestmark = pytest.mark.gpu
+
+GiB = 1 << 30
+
+
+def _jobs():
+    out = []
+    specs = [  # (graph, d, gather, batch, epochs)
+        (api.Graph.rmat(12, 16, seed=1), 128, api.FP32, 512, 4),
+        (api.Graph.sbm(3000, 12, 0.01, 0.0005, seed=2), 64, api.BF16, 1024, 6),
+        (api.Graph.sbm(1200, 4, 0.02, 0.001, seed=3), 32, api.FP32, 256, 3),
+        (api.Graph.rmat(11, 8, seed=4), 64, api.FP32, 333, 5),
+    ]
+    for k, (g, d, gather, B, T) in enumerate(specs):
+        x0 = api.init_uniform(g.n, d, seed=10 + k)
+        out.append(dict(g=g, x0=x0, batch_size=B, iterations=T, gather=gather))
+    return out
+
+
+def _submit(e, spec):
+    return e.submit(spec["g"], spec["x0"], spec["batch_size"], spec["iterations"],
+                    gather=spec["gather"], ns=5, lr=0.02)
+
+
+def _run_all(e, ids):
+    while e.run(16) > 0:
+        pass
+    return [e.collect(i) for i in ids]
+
+
+def test_alone_equals_mixed_and_graph_schedule():
+    specs = _jobs()
+    alone = []
+    for s in specs:
+        with api.Engine(2 * GiB) as e:
+            alone.append(_run_all(e, [_submit(e, s)])[0])
+    with api.Engine(2 * GiB) as e:
+        ids = [_submit(e, s) for s in specs]
+        mixed = _run_all(e, ids)
+    with api.Engine(2 * GiB) as e:
+        ids = [_submit(e, s) for s in specs]
+        e.run_graph()
+        assert all(e.done(i) for i in ids)
+        graphed = [e.collect(i) for i in ids]
+    for a, m, g in zip(alone, mixed, graphed):
+        assert np.isfinite(a).all()
+        np.testing.assert_array_equal(a, m)
+        np.testing.assert_array_equal(a, g)
+
+
+def test_engine_job_equals_single_graph_context():
+    s = _jobs()[0]
+    with api.Engine(2 * GiB) as e:
+        xe = _run_all(e, [_submit(e, s)])[0]
+    g = s["g"]
+    with api.ForceEmbed(g.n, s["x0"].shape[1], gather=s["gather"]) as ctx:
+        ctx.upload_csr(g)
+        ctx.upload_x(s["x0"])
+        ctx.train(s["iterations"], s["batch_size"], 5, 0.02)
+        xc = ctx.download_x()
+    np.testing.assert_array_equal(xe, xc)
+
+
+def test_admission_by_budget_and_reuse():
+    g = api.Graph.rmat(14, 16, seed=7)
+    x0 = api.init_uniform(g.n, 128, seed=1)
+    with api.Engine(1 << 20) as probe:
+        need = probe.job_bytes(g, x0, 2048, 2)
+    assert need > 8 * g.n * 128 // 2
+    with api.Engine(need * 3 // 2) as e:  # room for one copy, not two
+        a = e.submit(g, x0, 2048, 2)
+        with pytest.raises(api.FeError) as err:
+            e.submit(g, x0, 2048, 2)  # the second copy does not fit
+        assert err.value.status == 4
+        used = e.stats()["bytes_in_use"]
+        assert used >= need * 0.5
+        while e.run(8) > 0:
+            pass
+        xa = e.collect(a)
+        assert e.stats()["bytes_in_use"] == 0
+        b = e.submit(g, x0, 2048, 2)  # memory returned to the budget
+        while e.run(8) > 0:
+            pass
+        np.testing.assert_array_equal(e.collect(b), xa)
+        big = api.Graph.rmat(17, 16, seed=7)
+        with pytest.raises(api.FeError) as err:
+            e.submit(big, api.init_uniform(big.n, 128), 4096, 1)
+        assert err.value.status == 4
+
+
+def test_jobs_admitted_while_others_run():
+    specs = _jobs()
+    with api.Engine(2 * GiB) as e:
+        first = _submit(e, specs[0])
+        e.run(3)  # job 0 is mid-way when job 1 arrives
+        second = _submit(e, specs[1])
+        while e.run(4) > 0:
+            pass
+        x0, x1 = e.collect(first), e.collect(second)
+    for s, x in ((specs[0], x0), (specs[1], x1)):
+        with api.Engine(2 * GiB) as e:
+            np.testing.assert_array_equal(_run_all(e, [_submit(e, s)])[0], x)

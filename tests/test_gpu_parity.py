@@ -252,3 +252,100 @@ def test_rmat20_epoch_properties(rmat20):
     deg = np.diff(g.row_ptr)
     step_max = 2 * (deg + 5) * 0.02 * 5.0
     assert np.all(np.abs(X1 - X).max(axis=1) <= step_max + 1e-6)
+
+
+# ---- BF16 gather shadow -----------------------------------------------------------
+
+def _train_x(g, d, X, B, T, lists, gather, use_graph=True, steps=False):
+    ctx = _ctx(g, d, X, B, lists, gather=gather)
+    try:
+        if steps:
+            batches = -(-g.n // B)
+            for i in range(T):
+                for j in range(batches):
+                    ctx.step(i, j)
+        else:
+            ctx.train(T, B, lists.shape[1], 0.02, use_graph=use_graph)
+        return ctx.download_x(), ctx.loss(0, 5)
+    finally:
+        ctx.close()
+
+
+def test_bf16_chain_equals_single_steps():
+    # the fused chain reads the previous batch's rows from staging rounded to
+    # BF16, exactly as the shadow refresh of a separate apply would
+    g = api.Graph.sbm(1500, 6, 0.02, 0.001, seed=4)
+    X = api.init_uniform(g.n, 128, seed=8)
+    lists = api.reference_negatives(g.n, 12, 5)[0]
+    a, _ = _train_x(g, 128, X, 200, 4, lists, api.BF16)
+    b, _ = _train_x(g, 128, X, 200, 4, lists, api.BF16, use_graph=False)
+    c, _ = _train_x(g, 128, X, 200, 4, lists, api.BF16, steps=True)
+    np.testing.assert_array_equal(a, b)
+    np.testing.assert_array_equal(a, c)
+
+
+def test_bf16_row_meeting_itself_feels_no_force():
+    # a row drawn as its own negative: zero repulsion in the reference (v = 0);
+    # with BF16 gathers the distance must use the row as the shadow holds it
+    n, d = 64, 128
+    g = api.Graph.from_edges(n, np.array([3]), np.array([3]))  # only a self-loop (dropped)
+    X = api.init_uniform(n, d, seed=5)
+    lists = np.array([[7, 7, 7]], dtype=np.int32)
+    ctx = _ctx(g, d, X, 16, lists, gather=api.BF16)
+    delta = ctx.step_delta(0, 0, 16)
+    ctx.close()
+    assert np.all(delta[7] == 0)
+    assert np.all(np.abs(delta).max(axis=1)[[0, 1, 2, 4, 5, 6, 8]] > 0)
+
+
+def test_bf16_gradient_error_vs_fp64():
+    g = api.Graph.rmat(14, 16, seed=3)
+    d, B = 128, 2048
+    X = api.init_uniform(g.n, d, seed=2)
+    lists = api.reference_negatives(g.n, 12, 5)[0]
+    ctx = _ctx(g, d, X, B, lists, gather=api.BF16)
+    try:
+        for (epoch, j) in ((0, 0), (0, 3), (2, 7)):
+            rows = _rows(g.n, B, j)
+            dg = ctx.step_delta(epoch, j, rows).astype(np.float64)
+            d64 = orc.step(g, X.astype(np.float64), B, epoch, j, lists, apply=False,
+                           want_delta=True, dtype=np.float64)
+            err = np.linalg.norm(dg - d64) / np.linalg.norm(d64)
+            # BF16 keeps 8 significant bits: 2^-9 relative per gathered value
+            assert err < 1e-2, (epoch, j, err)
+    finally:
+        ctx.close()
+
+
+def test_bf16_final_loss_within_one_percent_of_fp32():
+    # SBM (the reference's CPU workload shape): well conditioned, FP32 and FP64
+    # runs agree to ~1e-8 after 20 epochs, so the BF16 gap is the BF16 effect
+    g = api.Graph.sbm(10_000, 10, 18 / 999, 2 / 9000, seed=0)
+    d, B, T = 128, 256, 20
+    X = api.init_uniform(g.n, d, seed=0)
+    lists = api.reference_negatives(g.n, T + 40, 5)[0]
+    _, (a32, r32) = _train_x(g, d, X, B, T, lists, api.FP32)
+    _, (a16, r16) = _train_x(g, d, X, B, T, lists, api.BF16)
+    l32, l16 = a32 + r32, a16 + r16
+    assert abs(l16 - l32) <= 0.01 * abs(l32), (l16, l32)
+
+
+def test_bf16_loss_on_rmat_within_fp32_rounding_spread():
+    # R-MAT hubs (degree in the thousands at lr 0.02) overshoot every step, so
+    # trajectories depend on rounding: the FP32 run itself ends ~2% away from
+    # the FP64 one. BF16 must stay within that spread of the FP64 run.
+    g = api.Graph.rmat(14, 16, seed=1)
+    d, B, T = 128, 2048, 20
+    X = api.init_uniform(g.n, d, seed=0)
+    lists = api.reference_negatives(g.n, T + 8, 5)[0]
+    _, (a32, r32) = _train_x(g, d, X, B, T, lists, api.FP32)
+    _, (a16, r16) = _train_x(g, d, X, B, T, lists, api.BF16)
+    x64 = X.astype(np.float64)
+    orc.train(g, x64, B, T, lists)
+    l64 = sum(orc.loss(g, x64, seed=0, eval_ns=5))
+    x32o = X.copy()
+    orc.train(g, x32o, B, T, lists)  # FP32 with serial sums: another rounding variant
+    l32o = sum(orc.loss(g, x32o.astype(np.float64), seed=0, eval_ns=5))
+    l32, l16 = a32 + r32, a16 + r16
+    spread = max(abs(l32 - l64), abs(l32o - l64))
+    assert abs(l16 - l64) <= 1.5 * spread + 0.005 * l64, (l16, l32, l32o, l64)
