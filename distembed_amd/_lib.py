@@ -50,7 +50,7 @@ class fe_graph(C.Structure):
 
 class fe_tuning(C.Structure):
     _fields_ = [("chunk", C.c_int), ("max_chunks", C.c_int), ("unroll", C.c_int),
-                ("waves_per_block", C.c_int), ("items_per_wave", C.c_int)]
+                ("waves_per_block", C.c_int), ("items_per_wave", C.c_int), ("light", C.c_int)]
 
 
 class fe_job_desc(C.Structure):
@@ -88,6 +88,8 @@ SIGNATURES = {
     "fe_epoch_units": (C.c_int, [_P, C.c_int, _I64P, _I64P, _I64P]),
     "fe_set_timing": (C.c_int, [_P, C.c_int]),
     "fe_kernel_times": (C.c_int, [_P, _DP, _DP, _DP, _I64P]),
+    "fe_autotune": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(fe_tuning), _DP, C.c_int,
+                              C.POINTER(C.c_int), C.POINTER(fe_tuning), _DP]),
     "fe_reference_negatives": (C.c_int, [C.c_int64, C.c_int, C.c_int, C.c_int, _I32P, _I64P]),
     "fe_eval_negative": (C.c_int64, [C.c_uint64, C.c_int64, C.c_int, C.c_int64]),
     "fe_gen_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,

@@ -173,8 +173,8 @@ class ForceEmbed:
         self._check(self._lib.fe_configure(self._ctx, batch_size, lr, max_bound))
 
     def set_tuning(self, chunk=0, max_chunks=0, unroll=0, waves_per_block=0,
-                   items_per_wave=0) -> None:
-        t = fe_tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave)
+                   items_per_wave=0, light=0) -> None:
+        t = fe_tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light)
         self._check(self._lib.fe_set_tuning(self._ctx, C.byref(t)))
 
     def get_tuning(self) -> dict:
@@ -207,6 +207,21 @@ class ForceEmbed:
         e, n0, n1 = C.c_int64(), C.c_int64(), C.c_int64()
         self._check(self._lib.fe_epoch_units(self._ctx, ns, C.byref(e), C.byref(n0), C.byref(n1)))
         return e.value, n0.value, n1.value
+
+    def autotune(self, epochs: int = 2, search_chunking: bool = False):
+        """Times candidate tilings (fe_autotune), keeps the fastest; returns
+        (best tuning dict, best ms/epoch, [(tuning dict, ms/epoch), ...])."""
+        cap = 512
+        res = (fe_tuning * cap)()
+        ms = (C.c_double * cap)()
+        n = C.c_int(0)
+        best = fe_tuning()
+        best_ms = C.c_double(0)
+        self._check(self._lib.fe_autotune(self._ctx, epochs, 1 if search_chunking else 0, res, ms,
+                                          cap, C.byref(n), C.byref(best), C.byref(best_ms)))
+        as_dict = lambda t: {k: getattr(t, k) for k, _ in fe_tuning._fields_}
+        table = [(as_dict(res[i]), ms[i]) for i in range(min(n.value, cap))]
+        return as_dict(best), best_ms.value, table
 
     def set_timing(self, enable: bool) -> None:
         self._check(self._lib.fe_set_timing(self._ctx, 1 if enable else 0))
@@ -309,8 +324,8 @@ class Engine:
         return x
 
     def set_tuning(self, chunk=0, max_chunks=0, unroll=0, waves_per_block=0,
-                   items_per_wave=0) -> None:
-        t = fe_tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave)
+                   items_per_wave=0, light=0) -> None:
+        t = fe_tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light)
         self._check(self._lib.fe_engine_set_tuning(self._e, C.byref(t)))
 
     def stats(self):

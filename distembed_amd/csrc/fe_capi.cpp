@@ -163,6 +163,7 @@ extern "C" fe_status fe_set_tuning(fe_ctx *c, const fe_tuning *t) {
   if (t->unroll) nt.unroll = t->unroll;
   if (t->waves_per_block) nt.waves_per_block = t->waves_per_block;
   if (t->items_per_wave) nt.items_per_wave = t->items_per_wave;
+  if (t->light) nt.light = t->light < 0 ? 0 : t->light;
   drop_graph(c);
   return from_job(c, job_set_tuning(c->job, nt));
 }
@@ -174,6 +175,7 @@ extern "C" fe_status fe_get_tuning(const fe_ctx *c, fe_tuning *t) {
   t->unroll = c->job.tun.unroll;
   t->waves_per_block = c->job.tun.waves_per_block;
   t->items_per_wave = c->job.tun.items_per_wave;
+  t->light = c->job.tun.light;
   return FE_OK;
 }
 
@@ -363,6 +365,117 @@ extern "C" fe_status fe_train(fe_ctx *c, int iterations, int batch_size, int ns,
   }
   if ((s = hip_ok(c, hipStreamSynchronize(c->stream), "fe_train sync")) != FE_OK) return s;
   if (c->timing && !use_graph) return harvest_events(c);
+  return FE_OK;
+}
+
+// ---- autotuner ------------------------------------------------------------
+
+// Times `epochs` epochs of the captured chain for every candidate tiling and
+// keeps the fastest. X (and the shadow) are restored after every trial, so
+// the call leaves the embedding as it found it. Unroll, waves per block and
+// items per wave never change results; chunk and max_chunks change the
+// summation order of heavy rows and are only searched when allowed.
+extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_tuning *results,
+                                 double *results_ms, int max_results, int *n_results,
+                                 fe_tuning *best, double *best_ms) {
+  CTX_GUARD(c);
+  Job &j = c->job;
+  if (epochs < 1) return fail(c, FE_ERR_INVALID_ARG, "fe_autotune: epochs must be >= 1");
+  fe_status s = from_job(c, job_prepare(j, c->stream));
+  if (s != FE_OK) return s;
+  if (j.ns > 0 && j.n_lists < epochs + j.plan.batches - 1)
+    return fail(c, FE_ERR_STATE, "fe_autotune: set negative lists covering the trial epochs");
+  const Tuning orig = j.tun;
+  // staged search: launch shape, then (optionally) the cutting of rows at
+  // the best shape, then the launch shape again at the best cutting
+  auto shapes = [](const Tuning &base) {
+    std::vector<Tuning> v;
+    for (int u : {1, 2, 4})
+      for (int w : {2, 4, 8})
+        for (int ipw : {1, 2, 4}) {
+          Tuning t = base;
+          t.unroll = u;
+          t.waves_per_block = w;
+          t.items_per_wave = ipw;
+          v.push_back(t);
+        }
+    return v;
+  };
+  auto cuttings = [](const Tuning &base) {
+    std::vector<Tuning> v;
+    for (int ch : {32, 64})
+      for (int mc : {64, 256, 1024})
+        for (int li : {0, 8, 16}) {
+          Tuning t = base;
+          t.chunk = ch;
+          t.max_chunks = mc;
+          t.light = li;
+          v.push_back(t);
+        }
+    return v;
+  };
+  const int stages = search_chunking ? 3 : 1;
+  std::vector<Tuning> cands = shapes(orig);
+  size_t stage_end = cands.size();
+  int stage = 1;
+  const size_t xbytes = sizeof(float) * (size_t)j.n * j.d;
+  void *backup = nullptr;
+  if ((s = hip_ok(c, hipMalloc(&backup, xbytes), "fe_autotune backup")) != FE_OK) return s;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  s = hip_ok(c, hipMemcpyAsync(backup, j.X, xbytes, hipMemcpyDeviceToDevice, c->stream), "backup");
+  const bool was_timing = c->timing;
+  c->timing = false;
+  double best_t = 1e300;
+  Tuning best_tun = orig;
+  int n = 0;
+  for (size_t i = 0; s == FE_OK && i < cands.size(); ++i) {
+    drop_graph(c);
+    if ((s = from_job(c, job_set_tuning(j, cands[i]))) != FE_OK) break;
+    if ((s = from_job(c, job_prepare(j, c->stream))) != FE_OK) break;
+    // one untimed epoch captures the graphs and warms the caches
+    if ((s = fe_train(c, 1, j.B, j.ns, j.lr, 1)) != FE_OK) break;
+    if ((s = hip_ok(c, hipEventRecord(e0, c->stream), "event")) != FE_OK) break;
+    if ((s = fe_train(c, epochs, j.B, j.ns, j.lr, 1)) != FE_OK) break;
+    if ((s = hip_ok(c, hipEventRecord(e1, c->stream), "event")) != FE_OK) break;
+    if ((s = hip_ok(c, hipEventSynchronize(e1), "event sync")) != FE_OK) break;
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    const double per_epoch = ms / epochs;
+    if (results && n < max_results) {
+      results[n] = fe_tuning{cands[i].chunk, cands[i].max_chunks, cands[i].unroll,
+                             cands[i].waves_per_block, cands[i].items_per_wave, cands[i].light};
+      if (results_ms) results_ms[n] = per_epoch;
+    }
+    ++n;
+    if (per_epoch < best_t) {
+      best_t = per_epoch;
+      best_tun = cands[i];
+    }
+    s = hip_ok(c, hipMemcpyAsync(j.X, backup, xbytes, hipMemcpyDeviceToDevice, c->stream), "restore");
+    if (s == FE_OK && j.dtype == FE_GATHER_BF16)
+      s = from_job(c, job_check(j, launch_to_bf16(j.X, j.Xs, (int64_t)j.n * j.d, c->stream), "shadow"));
+    if (i + 1 == stage_end && stage < stages) {
+      const std::vector<Tuning> next = (stage == 1) ? cuttings(best_tun) : shapes(best_tun);
+      cands.insert(cands.end(), next.begin(), next.end());
+      stage_end = cands.size();
+      ++stage;
+    }
+  }
+  c->timing = was_timing;
+  drop_graph(c);
+  fe_status s2 = from_job(c, job_set_tuning(j, s == FE_OK ? best_tun : orig));
+  if (s == FE_OK) s = s2;
+  hipStreamSynchronize(c->stream);
+  hipFree(backup);
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (s != FE_OK) return s;
+  if (n_results) *n_results = n;
+  if (best) *best = fe_tuning{best_tun.chunk, best_tun.max_chunks, best_tun.unroll,
+                              best_tun.waves_per_block, best_tun.items_per_wave, best_tun.light};
+  if (best_ms) *best_ms = best_t;
   return FE_OK;
 }
 

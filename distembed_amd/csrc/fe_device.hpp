@@ -10,7 +10,12 @@ namespace fe {
 //   pslot < 0: the item is the whole row and writes the new row;
 //   pslot >= 0: the item is chunk pslot-p0 of heavy row `row`, split id sid,
 //   whose k partials p0..p0+k-1 the last chunk to finish adds in index order.
+// A batch's items are its block items (one per block, cut into kSub
+// sub-ranges), then its wave items (one row per wave), then its light rows
+// {row, e0, deg, -1, pack_max_deg, 0, 0, 0} (one row per lane group, sorted
+// by degree, pack_max_deg = largest degree of the row's pack).
 constexpr int kItemInts = 8;
+constexpr int kSub = 4;
 
 // One minibatch step of one job (graph). Everything here is invariant across
 // epochs: the epoch-dependent parts (negative list index epoch+batch,
@@ -29,7 +34,9 @@ struct alignas(16) StepDesc {
   const int32_t *negs;   // negative lists, list l at negs + l*ns
   const int *epoch;      // device epoch counter of the job
   const float *Sprev;    // new rows of the previous batch, not yet in X
-  int n_items;
+  int n_items;           // block items + wave items + light rows
+  int n_block_items;
+  int n_light;           // light rows (one per lane group), last in items
   int row_begin;         // batch rows [row_begin, row_begin + rows)
   int rows;
   int rep0;              // rows that get repulsion in epoch 0
@@ -46,7 +53,6 @@ struct alignas(16) StepDesc {
   int prev_rows;         // 0: no pending previous batch
   float lr;
   float bound;           // MAX_BOUND of cpp/algo/algo.hpp:65-72
-  int pad0;
 };
 
 // Loss over one job's graph (FP64 reduction).

@@ -202,8 +202,12 @@ static fe_status plan_job(const fe_job_desc *d, const Tuning &tun, Job &tmp, std
   tmp.ns = d->ns;
   tmp.tun = tun;
   const int64_t c_limit = (d->quirks & FE_QUIRK_SKIP_LAST_COL) ? d->n - 1 : d->n;
-  s = build_plan(d->n, d->row_ptr, nullptr, d->batch_size, tun.chunk, tun.max_chunks, c_limit,
-                 &tmp.plan);
+  if (d->d < 8 || d->d > 512 || d->d % 8 != 0 || ((d->d / 8) & (d->d / 8 - 1)) != 0) {
+    err = "job description: d must be 8 x a power of two in [8, 512]";
+    return FE_ERR_INVALID_ARG;
+  }
+  s = build_plan(d->n, d->row_ptr, nullptr, d->batch_size, tun.chunk, tun.max_chunks, tun.light,
+                 64 / (d->d / 8), c_limit, &tmp.plan);
   if (s != FE_OK) {
     err = host_error();
     return s;
@@ -293,6 +297,7 @@ extern "C" fe_status fe_engine_set_tuning(fe_engine *e, const fe_tuning *t) {
   if (t->unroll) nt.unroll = t->unroll;
   if (t->waves_per_block) nt.waves_per_block = t->waves_per_block;
   if (t->items_per_wave) nt.items_per_wave = t->items_per_wave;
+  if (t->light) nt.light = t->light < 0 ? 0 : t->light;
   Job probe;
   if (job_set_tuning(probe, nt) != FE_OK) return efail(e, FE_ERR_INVALID_ARG, probe.err);
   // chunking of already submitted jobs is fixed; launch shape applies to all
@@ -361,6 +366,13 @@ extern "C" fe_status fe_engine_submit(fe_engine *e, const fe_job_desc *d, int *j
 }
 
 // Step descriptors of the next round of every running job; advances them.
+// Kernel variant of a round: 0 all FP32 gathers, 1 all BF16, 2 mixed.
+static int round_mode(const std::vector<EngineJob *> &run) {
+  bool f = false, h = false;
+  for (const EngineJob *ej : run) (ej->job.dtype == FE_GATHER_BF16 ? h : f) = true;
+  return (f && h) ? 2 : (h ? 1 : 0);
+}
+
 static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *host, int &K,
                              size_t &lds, std::vector<EngineJob *> &finished) {
   K = (int)run.size();
@@ -379,9 +391,9 @@ static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *
     sd[k] = j.host_desc[(size_t)j.plan.batches * DESC_SINGLE + b];
     sd[k].epoch = dep + k;
     ep[k] = ej->epoch;
-    pf[k + 1] = pf[k] + std::max(1, (sd[k].n_items + ipb - 1) / ipb);
+    pf[k + 1] = pf[k] + force_blocks_for(sd[k], ipb);
     pa[k + 1] = pa[k] + j.apply_blocks[b];
-    lds = std::max(lds, (size_t)j.ns * j.d * sizeof(float));
+    lds = std::max(lds, force_lds_bytes(j.ns, j.d));
     int r0, r1;
     rep_rows(j.plan, j.quirks, b, &r0, &r1);
     ej->units += j.plan.edges_eff[b] + (int64_t)(ej->epoch == 0 ? r0 : r1) * j.ns;
@@ -435,7 +447,8 @@ extern "C" fe_status fe_engine_run(fe_engine *e, int rounds, int *running) {
     const int *dpf = reinterpret_cast<const int *>(e->dev_round + K * sizeof(StepDesc)) + K;
     const int *dpa = dpf + K + 1;
     if ((s = ehip(e, launch_force(dsd, dpf, K, fblocks, e->tun.waves_per_block,
-                                  e->tun.items_per_wave, e->tun.unroll, lds, e->stream),
+                                  e->tun.items_per_wave, e->tun.unroll, lds, round_mode(run),
+                                  e->stream),
                   "force launch")) != FE_OK)
       return s;
     if ((s = ehip(e, launch_apply(dsd, dpa, K, ablocks, e->stream), "apply launch")) != FE_OK)
@@ -526,6 +539,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
     int K;
     int fblocks, ablocks;
     size_t lds;
+    int mode;
   };
   std::vector<RoundInfo> rounds;
   std::vector<char> host;
@@ -545,7 +559,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
     char *h = host.data() + total;
     build_round(e, run, h, K, lds, finished);
     const int *hpf = reinterpret_cast<const int *>(h + K * sizeof(StepDesc)) + K;
-    rounds.push_back({total, K, hpf[K], hpf[2 * K + 1], lds});
+    rounds.push_back({total, K, hpf[K], hpf[2 * K + 1], lds, round_mode(run)});
     for (EngineJob *ej : finished) finish_at.push_back({(int)rounds.size() - 1, ej});
     total += rb;
   }
@@ -575,7 +589,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
       const int *dpf = reinterpret_cast<const int *>(dsched + ri.offset + ri.K * sizeof(StepDesc)) + ri.K;
       const int *dpa = dpf + ri.K + 1;
       ls = ehip(e, launch_force(dsd, dpf, ri.K, ri.fblocks, e->tun.waves_per_block,
-                                e->tun.items_per_wave, e->tun.unroll, ri.lds, e->stream),
+                                e->tun.items_per_wave, e->tun.unroll, ri.lds, ri.mode, e->stream),
                 "force launch");
       if (ls == FE_OK) ls = ehip(e, launch_apply(dsd, dpa, ri.K, ri.ablocks, e->stream), "apply launch");
     }

@@ -21,8 +21,10 @@ fe_status build_csr(int64_t n, const int64_t *u, const int64_t *v, int64_t m, bo
 //   pslot < 0: the item is the whole row and writes the new row.
 //   pslot >= 0: chunk pslot-p0 of heavy row `row` (split sid of its batch);
 //   the last of its k chunks to finish sums partials p0..p0+k-1 in order.
-// A row of degree <= chunk is one item; a heavier row is cut into
-// k = ceil(deg / max(chunk, ceil(deg / max_chunks))) near-equal chunks.
+// A row of degree <= chunk is a wave item. A row of degree <= kSub*chunk is
+// one block item. A heavier row is cut into
+// k = ceil(deg / max(kSub*chunk, ceil(deg / max_chunks))) near-equal block
+// items. Each batch lists its block items first, then its wave items.
 struct Plan {
   int B = 0;
   int batches = 0;
@@ -31,6 +33,10 @@ struct Plan {
   int max_chunks = 0;
   std::vector<int32_t> items;
   std::vector<int64_t> item_off;  // batches + 1, in items
+  std::vector<int32_t> block_items; // per batch: leading block items
+  std::vector<int32_t> light_rows;  // per batch: trailing light rows
+  int light = 0;                    // rows of degree <= light go one per lane group
+  int pack = 1;                     // lane groups per wave (64 / (d / 8))
   std::vector<int32_t> splits;    // {row, p0, k, 0} per heavy row
   std::vector<int64_t> split_off; // batches + 1, in splits
   std::vector<int32_t> pslots;    // partial slots used by batch j
@@ -41,7 +47,7 @@ struct Plan {
 };
 
 fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int B, int chunk,
-                     int max_chunks, int64_t c_limit, Plan *out);
+                     int max_chunks, int light, int pack, int64_t c_limit, Plan *out);
 
 // Repulsive row counts of batch j in epoch 0 and in later epochs
 // (cpp/algo/algo.hpp:135,167-169,594).

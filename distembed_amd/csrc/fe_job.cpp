@@ -55,6 +55,14 @@ static fe_status ensure_cap(Job &j, T *&ptr, size_t &cap, size_t count, const ch
   return FE_OK;
 }
 
+int force_blocks_for(const StepDesc &sd, int items_per_block) {
+  const int pack = 64 / sd.G;
+  const int units = (sd.n_items - sd.n_block_items - sd.n_light) + (sd.n_light + pack - 1) / pack;
+  return std::max(1, sd.n_block_items + (units + items_per_block - 1) / items_per_block);
+}
+
+size_t force_lds_bytes(int ns, int d) { return (size_t)(ns + kSub) * d * sizeof(float); }
+
 int rows_of_batch(const Job &j, int b) {
   const int64_t r0 = (int64_t)b * j.plan.B;
   return (int)(std::min<int64_t>(r0 + j.plan.B, j.n) - r0);
@@ -233,11 +241,12 @@ fe_status job_set_tuning(Job &j, const Tuning &t) {
   auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
   if (t.chunk < 1 || t.chunk > 64 || t.max_chunks < 1 || !pow2(t.unroll) || t.unroll > 8 ||
       !pow2(t.waves_per_block) || t.waves_per_block > 8 || t.items_per_wave < 1 ||
-      t.items_per_wave > 64) {
+      t.items_per_wave > 64 || t.light < 0 || t.light > 64) {
     j.err = "tuning out of range";
     return FE_ERR_INVALID_ARG;
   }
-  if (t.chunk != j.tun.chunk || t.max_chunks != j.tun.max_chunks) j.plan_dirty = true;
+  if (t.chunk != j.tun.chunk || t.max_chunks != j.tun.max_chunks || t.light != j.tun.light)
+    j.plan_dirty = true;
   j.tun = t;
   j.steps_dirty = true;
   return FE_OK;
@@ -269,6 +278,8 @@ static StepDesc make_desc(const Job &j, int b, DescKind kind) {
   sd.negs = j.negs;
   sd.epoch = j.epoch;
   sd.n_items = (int)(j.plan.item_off[b + 1] - j.plan.item_off[b]);
+  sd.n_block_items = j.plan.block_items[b];
+  sd.n_light = j.plan.light_rows[b];
   sd.row_begin = (int)((int64_t)b * j.plan.B);
   sd.rows = rows_of_batch(j, b);
   rep_rows(j.plan, j.quirks, b, &sd.rep0, &sd.rep1);
@@ -328,7 +339,7 @@ fe_status job_prepare(Job &j, hipStream_t st) {
   if (j.plan_dirty) {
     Plan p;
     fe_status s = build_plan(j.n, j.row_ptr.data(), nullptr, j.B, j.tun.chunk,
-                             j.tun.max_chunks, j.c_limit, &p);
+                             j.tun.max_chunks, j.tun.light, 64 / j.G, j.c_limit, &p);
     if (s != FE_OK) {
       j.err = host_error();
       return s;
@@ -377,7 +388,7 @@ fe_status job_prepare(Job &j, hipStream_t st) {
       hd[b] = make_desc(j, b, DESC_FUSED);
       hd[nb + b] = make_desc(j, b, DESC_SINGLE);
       hd[2 * nb + b] = make_desc(j, b, DESC_DELTA);
-      j.force_blocks[b] = std::max(1, (hd[b].n_items + ipb - 1) / ipb);
+      j.force_blocks[b] = force_blocks_for(hd[b], ipb);
       const int64_t q4 = (int64_t)hd[b].rows * j.d / 4;
       j.apply_blocks[b] = (int)std::max<int64_t>(1, std::min<int64_t>((q4 + 255) / 256, 4096));
       pf[4 * b + 1] = j.force_blocks[b];
@@ -398,10 +409,11 @@ fe_status job_prepare(Job &j, hipStream_t st) {
 }
 
 fe_status job_launch_force(Job &j, int b, DescKind kind, hipStream_t st, hipEvent_t *ev) {
-  const size_t lds = (size_t)j.ns * j.d * sizeof(float);
+  const size_t lds = force_lds_bytes(j.ns, j.d);
   if (ev) FE_HIP(j, record_event(ev[0], st), "event");
   FE_HIP(j, launch_force(job_desc(j, b, kind), j.prefix + 4 * b, 1, j.force_blocks[b],
-                         j.tun.waves_per_block, j.tun.items_per_wave, j.tun.unroll, lds, st),
+                         j.tun.waves_per_block, j.tun.items_per_wave, j.tun.unroll, lds,
+                         j.dtype == FE_GATHER_BF16 ? 1 : 0, st),
          "force kernel launch");
   if (ev) FE_HIP(j, record_event(ev[1], st), "event");
   return FE_OK;

@@ -13,8 +13,9 @@
 namespace fe {
 
 // Launch wrappers (fe_kernels.hip).
+// mode: 0 every step gathers FP32, 1 every step gathers BF16, 2 mixed.
 hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
-                        int wpb, int ipw, int unroll, size_t lds_bytes, hipStream_t st);
+                        int wpb, int ipw, int unroll, size_t lds_bytes, int mode, hipStream_t st);
 hipError_t launch_apply(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
                         hipStream_t st);
 hipError_t launch_tick(int *const *ctrs, int n, hipStream_t st);
@@ -26,12 +27,14 @@ hipError_t launch_loss_negs(const LossDesc &ld, double *block_out, int n_blocks,
                             int pairs_per_group, hipStream_t st);
 hipError_t launch_reduce(const double *in, int n, double *out, hipStream_t st);
 
+// Defaults: the autotuner's choice for R-MAT scale 20 at d = 128 on MI355X.
 struct Tuning {
-  int chunk = 32;
-  int max_chunks = 256;
-  int unroll = 4;
+  int chunk = 64;
+  int max_chunks = 1024;
+  int unroll = 2;
   int waves_per_block = 4;
   int items_per_wave = 1;
+  int light = 16;
 };
 
 // Descriptor variants of one batch.
@@ -120,6 +123,11 @@ fe_status job_loss(Job &j, uint64_t seed, int eval_ns, double *attr, double *rep
 int64_t job_bytes_estimate(const Job &j);
 fe_status job_check(Job &j, hipError_t e, const char *what);
 int rows_of_batch(const Job &j, int b);
+// Blocks of a force launch for one step: one per block item, then the wave
+// items `items_per_block` (= waves per block x items per wave) at a time.
+int force_blocks_for(const StepDesc &sd, int items_per_block);
+// Dynamic LDS of a force launch: staged negatives plus kSub sub-range sums.
+size_t force_lds_bytes(int ns, int d);
 hipError_t record_event(hipEvent_t ev, hipStream_t st);
 
 } // namespace fe

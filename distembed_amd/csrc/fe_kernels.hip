@@ -141,16 +141,23 @@ __device__ __forceinline__ int find_step(const int *__restrict__ prefix,
 // Row c as 8 FP32 components [8q, 8q+8). Rows of the previous batch whose
 // new values still sit in Sprev are read from there; with BF16 gathers they
 // are rounded exactly as the shadow refresh will round them.
+// MODE 0: FP32 gathers, 1: BF16 gathers, 2: per step at run time (a launch
+// mixing jobs of both kinds).
+template <int MODE> __device__ __forceinline__ bool use_bf16(const StepDesc &sd) {
+  return MODE == 2 ? sd.bf16 != 0 : MODE == 1;
+}
+
+template <int MODE>
 __device__ __forceinline__ void load_row(const StepDesc &sd, int c, int q, float (&o)[8]) {
   const int D = sd.D;
   const unsigned off = (unsigned)(c - sd.prev_begin);
   if (off < (unsigned)sd.prev_rows) {
     load8_f32(sd.Sprev + (size_t)off * D + 8 * q, o);
-    if (sd.bf16) {
+    if (use_bf16<MODE>(sd)) {
 #pragma unroll
       for (int k = 0; k < 8; ++k) o[k] = __uint_as_float(bf16_bits(o[k]) << 16);
     }
-  } else if (sd.bf16) {
+  } else if (use_bf16<MODE>(sd)) {
     load8_bf16(sd.Xs + (size_t)c * D + 8 * q, o);
   } else {
     load8_f32(sd.X + (size_t)c * D + 8 * q, o);
@@ -158,6 +165,7 @@ __device__ __forceinline__ void load_row(const StepDesc &sd, int c, int q, float
 }
 
 // Stage the `ns` negative rows of list `list` as FP32 in LDS.
+template <int MODE>
 __device__ __forceinline__ void stage_negatives(const StepDesc &sd, int list, float *lds) {
   const int G = sd.G;
   const int total8 = sd.ns * G; // 8-component pieces
@@ -166,7 +174,7 @@ __device__ __forceinline__ void stage_negatives(const StepDesc &sd, int list, fl
     const int k = p / G;
     const int q = p - k * G;
     float v[8];
-    load_row(sd, gld_int(nl + k), q, v);
+    load_row<MODE>(sd, gld_int(nl + k), q, v);
     store8_lds(lds + (size_t)k * sd.D + 8 * q, v);
   }
 }
@@ -232,23 +240,186 @@ __device__ __forceinline__ void sum_partials(const StepDesc &sd, int p0, int k, 
   }
 }
 
-// One minibatch of one or more jobs: attractive gathers, repulsion against the
-// staged negatives, heavy rows reduced by their last chunk, new rows to S;
-// then this block's share of the previous batch's rows is copied Sprev -> X.
-template <int U>
+// Attractive forces of edges [e0, e0+cnt) of the row on the per-group
+// accumulators: group g takes edges g, g+NG, ... in order, U rows in flight.
+template <int U, int MODE>
+__device__ __forceinline__ void gather_edges(const StepDesc &sd, const float (&xd)[8], int e0,
+                                             int cnt, int NG, int g, int q, int lane,
+                                             float (&acc)[8]) {
+  const int G = sd.G;
+  const float lr = sd.lr, bound = sd.bound;
+  const int c_limit = sd.c_limit;
+  for (int eb = 0; eb < cnt; eb += 64) {
+    const int m = min(64, cnt - eb);
+    const int cl = lane < m ? gld_int(sd.col + e0 + eb + lane) : 0;
+    const int rounds = (m + NG - 1) / NG;
+    for (int u0 = 0; u0 < rounds; u0 += U) {
+      float xc[U][8];
+      bool ok[U];
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) {
+        const int k = (u0 + uu) * NG + g;
+        const int c = __shfl(cl, k & 63);
+        ok[uu] = (k < m) && (c < c_limit);
+        if (ok[uu]) {
+          load_row<MODE>(sd, c, q, xc[uu]);
+        } else {
+#pragma unroll
+          for (int k2 = 0; k2 < 8; ++k2) xc[uu][k2] = 0.f;
+        }
+      }
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) {
+        float v[8];
+        float p = 0.f;
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) {
+          v[k2] = ok[uu] ? xd[k2] - xc[uu][k2] : 0.f;
+          p = fmaf(v[k2], v[k2], p);
+        }
+        const float s = group_sum(p, G);
+        const float d1 = -2.0f / (1.0f + s);
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) acc[k2] = fmaf(lr, clampf(v[k2] * d1, bound), acc[k2]);
+      }
+    }
+  }
+}
+
+// Row load for the distances: the master row, rounded as the shadow holds it
+// when gathers read BF16 (no extra traffic), so a row meeting itself as a
+// negative or through a self-loop feels exactly zero force, as in the
+// reference; the update still goes to the FP32 master.
+template <int MODE>
+__device__ __forceinline__ void load_own(const StepDesc &sd, int r, int q, float (&xr)[8],
+                                         float (&xd)[8]) {
+  load8_f32(sd.X + (size_t)r * sd.D + 8 * q, xr);
+#pragma unroll
+  for (int k = 0; k < 8; ++k)
+    xd[k] = use_bf16<MODE>(sd) ? __uint_as_float(bf16_bits(xr[k]) << 16) : xr[k];
+}
+
+// Repulsion (first `rep` rows), group combine, new row (or delta) to S.
+__device__ __forceinline__ void finish_row(const StepDesc &sd, const float *lds_neg, bool repel,
+                                           int t, const float (&xr)[8], const float (&xd)[8],
+                                           float (&acc)[8], int G, int NG, int g, int q) {
+  if (repel) repulse(sd, lds_neg, xd, acc, G, NG, g, q);
+  combine_groups(acc, G);
+  if (g == 0) {
+    float o[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
+    store8_f32(sd.S + (size_t)t * sd.D + 8 * q, o);
+  }
+}
+
+// Light rows, one per lane group: group g takes entry pack*NG + g of the
+// batch's light rows (sorted by degree), gathers its edges in CSR order (U
+// in flight), repels against every staged negative and writes its row. The
+// loops run to the pack's largest degree, so control flow stays uniform.
+template <int U, int MODE>
+__device__ __forceinline__ void light_pack(const StepDesc &sd, const int32_t *lrows, int n_light,
+                                           int pack, const float *lds_neg, bool has_neg, int rep,
+                                           int NG, int g, int q, int lane) {
+  const int G = sd.G;
+  const int idx = pack * NG + g;
+  const bool valid = idx < n_light;
+  const u4v ia = gld_u4(lrows + (size_t)kItemInts * (valid ? idx : pack * NG));
+  const int r = (int)ia.x, e0 = (int)ia.y, deg = valid ? (int)ia.z : 0;
+  const int kmax = __builtin_amdgcn_readfirstlane((int)gld_u4(lrows + (size_t)kItemInts * pack * NG + 4).x);
+  const float lr = sd.lr, bound = sd.bound;
+  const int c_limit = sd.c_limit;
+  float xr[8], xd[8], acc[8];
+  load_own<MODE>(sd, r, q, xr, xd);
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+  for (int kb = 0; kb < kmax; kb += G) {
+    const int cl = (kb + q < deg) ? gld_int(sd.col + e0 + kb + q) : 0;
+    const int kend = min(kmax, kb + G);
+    for (int k0 = kb; k0 < kend; k0 += U) {
+      float xc[U][8];
+      bool ok[U];
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) {
+        const int k = k0 + uu;
+        const int c = __shfl(cl, g * G + ((k - kb) & (G - 1)));
+        ok[uu] = (k < kend) && (k < deg) && (c < c_limit);
+        if (ok[uu]) {
+          load_row<MODE>(sd, c, q, xc[uu]);
+        } else {
+#pragma unroll
+          for (int k2 = 0; k2 < 8; ++k2) xc[uu][k2] = 0.f;
+        }
+      }
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) {
+        float v[8];
+        float p = 0.f;
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) {
+          v[k2] = ok[uu] ? xd[k2] - xc[uu][k2] : 0.f;
+          p = fmaf(v[k2], v[k2], p);
+        }
+        const float s = group_sum(p, G);
+        const float d1 = -2.0f / (1.0f + s);
+#pragma unroll
+        for (int k2 = 0; k2 < 8; ++k2) acc[k2] = fmaf(lr, clampf(v[k2] * d1, bound), acc[k2]);
+      }
+    }
+  }
+  const int t = r - sd.row_begin;
+  if (has_neg) {
+    const bool repel = valid && t < rep;
+    for (int n = 0; n < sd.ns; ++n) {
+      const float *pn = lds_neg + (size_t)n * sd.D + 8 * q;
+      const float4 a = *reinterpret_cast<const float4 *>(pn);
+      const float4 b = *reinterpret_cast<const float4 *>(pn + 4);
+      const float xn[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
+      float v[8];
+      float p = 0.f;
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) {
+        v[k2] = repel ? xd[k2] - xn[k2] : 0.f;
+        p = fmaf(v[k2], v[k2], p);
+      }
+      const float s = group_sum(p, G);
+      const float d1 = 2.0f / ((s + 1e-6f) * (1.0f + s));
+#pragma unroll
+      for (int k2 = 0; k2 < 8; ++k2) acc[k2] = fmaf(lr, clampf(v[k2] * d1, bound), acc[k2]);
+    }
+  }
+  if (valid) {
+    float o[8];
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
+    store8_f32(sd.S + (size_t)t * sd.D + 8 * q, o);
+  }
+}
+
+// One minibatch of one or more jobs. Blocks [0, n_block_items) of a step
+// each take one block item: a row of degree <= kSub*chunk, or one chunk of a
+// heavier row, cut into kSub fixed sub-ranges that the block's waves gather
+// and then add in sub-range order through LDS. A chunk publishes its
+// partial (device-coherent stores) and bumps the row's counter; the chunk
+// that counts last adds all partials of the row in index order. The
+// remaining blocks' waves walk the light rows with a grid stride, one row
+// per wave at a time. Every block then copies its share of the previous
+// batch's rows from staging into X.
+template <int U, int MODE>
 __global__ __launch_bounds__(512) void force_kernel(
     const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
     int n_steps, int ipw) {
-  extern __shared__ float lds_neg[];
+  extern __shared__ float lds[];
   const int b = blockIdx.x;
   const int si = find_step(blk_prefix, n_steps, b);
   const StepDesc &sd = steps[si];
   const int lb = b - blk_prefix[si];
+  const int nb = blk_prefix[si + 1] - blk_prefix[si];
   const int epoch = *sd.epoch;
   const int rep = epoch == 0 ? sd.rep0 : sd.rep1;
   const int list = epoch + sd.batch;
   const bool has_neg = sd.ns > 0 && list < sd.n_lists;
-  if (has_neg) stage_negatives(sd, list, lds_neg);
+  if (has_neg) stage_negatives<MODE>(sd, list, lds);
   __syncthreads();
 
   const int D = sd.D, G = sd.G;
@@ -258,110 +429,87 @@ __global__ __launch_bounds__(512) void force_kernel(
   const int wpb = blockDim.x >> 6;
   const int g = lane / G;
   const int q = lane & (G - 1);
-  const float lr = sd.lr, bound = sd.bound;
-  const int c_limit = sd.c_limit;
 
-  for (int i = 0; i < ipw; ++i) {
-    const int it = (lb * ipw + i) * wpb + wave;
-    if (it >= sd.n_items) break;
-    const int32_t *item = sd.items + (size_t)kItemInts * it;
+  if (lb < sd.n_block_items) {
+    const int32_t *item = sd.items + (size_t)kItemInts * lb;
     const u4v ia = gld_u4(item);
-    const int r = (int)ia.x;
-    const int e0 = (int)ia.y;
-    const int cnt = (int)ia.z;
-    const int pslot = (int)ia.w;
+    const int r = (int)ia.x, e0 = (int)ia.y, cnt = (int)ia.z, pslot = (int)ia.w;
     const int t = r - sd.row_begin;
-
-    float xr[8];
-    load8_f32(sd.X + (size_t)r * D + 8 * q, xr);
-    // With BF16 gathers the distances use the row as the shadow holds it
-    // (rounded here from the master, no extra traffic), so a row meeting
-    // itself as a negative or through a self-loop feels exactly zero force,
-    // as in the reference; the update still goes to the FP32 master.
-    float xd[8];
+    float xr[8], xd[8], acc[8];
+    load_own<MODE>(sd, r, q, xr, xd);
+    float *lds_sub = lds + (has_neg ? sd.ns * D : 0);
+    for (int s = wave; s < kSub; s += wpb) {
+      const int lo = (int)((int64_t)cnt * s / kSub), hi = (int)((int64_t)cnt * (s + 1) / kSub);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) xd[k] = sd.bf16 ? __uint_as_float(bf16_bits(xr[k]) << 16) : xr[k];
-    float acc[8];
-#pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] = 0.f;
-
-    for (int eb = 0; eb < cnt; eb += 64) {
-      const int m = min(64, cnt - eb);
-      const int cl = lane < m ? gld_int(sd.col + e0 + eb + lane) : 0;
-      const int rounds = (m + NG - 1) / NG;
-      for (int u0 = 0; u0 < rounds; u0 += U) {
-        float xc[U][8];
-        bool ok[U];
-#pragma unroll
-        for (int uu = 0; uu < U; ++uu) {
-          const int k = (u0 + uu) * NG + g;
-          const int c = __shfl(cl, k & 63);
-          ok[uu] = (k < m) && (c < c_limit);
-          if (ok[uu]) {
-            load_row(sd, c, q, xc[uu]);
-          } else {
-#pragma unroll
-            for (int k2 = 0; k2 < 8; ++k2) xc[uu][k2] = 0.f;
-          }
-        }
-#pragma unroll
-        for (int uu = 0; uu < U; ++uu) {
-          float v[8];
-          float p = 0.f;
-#pragma unroll
-          for (int k2 = 0; k2 < 8; ++k2) {
-            v[k2] = ok[uu] ? xd[k2] - xc[uu][k2] : 0.f;
-            p = fmaf(v[k2], v[k2], p);
-          }
-          const float s = group_sum(p, G);
-          const float d1 = -2.0f / (1.0f + s);
-#pragma unroll
-          for (int k2 = 0; k2 < 8; ++k2)
-            acc[k2] = fmaf(lr, clampf(v[k2] * d1, bound), acc[k2]);
-        }
-      }
-    }
-
-    bool finish = pslot < 0;
-    if (!finish) {
-      const u4v ib = gld_u4(item + 4);
-      const int sid = (int)ib.x, p0 = (int)ib.y, kparts = (int)ib.z;
-      // heavy row: publish this chunk's partial (device-coherent stores, then
-      // wait for them to complete), count it, and let the chunk that counts
-      // last sum all partials of the row
+      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+      gather_edges<U, MODE>(sd, xd, e0 + lo, hi - lo, NG, g, q, lane, acc);
       combine_groups(acc, G);
-      if (g == 0) store8_coherent(sd.P + (size_t)pslot * D + 8 * q, acc);
-      wait_stores();
-      int old = 0;
-      if (lane == 0)
-        old = __hip_atomic_fetch_add((gint *)(sd.counters + sid), 1, __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-      old = __shfl(old, 0);
-      if (old == kparts - 1) {
-#pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2) acc[k2] = 0.f;
-        sum_partials<U>(sd, p0, kparts, NG, g, q, acc);
-        if (lane == 0)
-          __hip_atomic_store((gint *)(sd.counters + sid), 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        finish = true;
-      }
+      if (g == 0) store8_lds(lds_sub + (size_t)s * D + 8 * q, acc);
     }
-    if (finish) {
-      if (has_neg && t < rep) repulse(sd, lds_neg, xd, acc, G, NG, g, q);
-      combine_groups(acc, G);
+    __syncthreads();
+    if (wave == 0) {
+      // group 0 holds the sum of the sub-ranges, in sub-range order
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
       if (g == 0) {
-        float o[8];
+        for (int s = 0; s < kSub; ++s) {
+          const float *p = lds_sub + (size_t)s * D + 8 * q;
 #pragma unroll
-        for (int k = 0; k < 8; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
-        store8_f32(sd.S + (size_t)t * D + 8 * q, o);
+          for (int k = 0; k < 8; ++k) acc[k] += p[k];
+        }
       }
+      bool finish = pslot < 0;
+      if (!finish) {
+        const u4v ib = gld_u4(item + 4);
+        const int sid = (int)ib.x, p0 = (int)ib.y, kparts = (int)ib.z;
+        if (g == 0) store8_coherent(sd.P + (size_t)pslot * D + 8 * q, acc);
+        wait_stores();
+        int old = 0;
+        if (lane == 0)
+          old = __hip_atomic_fetch_add((gint *)(sd.counters + sid), 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        old = __shfl(old, 0);
+        if (old == kparts - 1) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+          sum_partials<U>(sd, p0, kparts, NG, g, q, acc);
+          if (lane == 0)
+            __hip_atomic_store((gint *)(sd.counters + sid), 0, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+          finish = true;
+        }
+      }
+      if (finish) finish_row(sd, lds, has_neg && t < rep, t, xr, xd, acc, G, NG, g, q);
+    }
+  } else {
+    // wave work: packs of light rows (one row per lane group), then the
+    // medium rows (one row per wave), walked with a grid stride
+    const int n_light = sd.n_light;
+    const int n_wave_items = sd.n_items - sd.n_block_items - n_light;
+    const int32_t *witems = sd.items + (size_t)kItemInts * sd.n_block_items;
+    const int32_t *lrows = witems + (size_t)kItemInts * n_wave_items;
+    const int n_packs = (n_light + NG - 1) / NG;
+    const int W = (nb - sd.n_block_items) * wpb;
+    int u = (lb - sd.n_block_items) * wpb + wave;
+    for (; u < n_packs; u += W)
+      light_pack<U, MODE>(sd, lrows, n_light, u, lds, has_neg, rep, NG, g, q, lane);
+    for (int it = u - n_packs; it < n_wave_items; it += W) {
+      const u4v ia = gld_u4(witems + (size_t)kItemInts * it);
+      const int r = (int)ia.x, e0 = (int)ia.y, cnt = (int)ia.z;
+      const int t = r - sd.row_begin;
+      float xr[8], xd[8], acc[8];
+      load_own<MODE>(sd, r, q, xr, xd);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+      gather_edges<U, MODE>(sd, xd, e0, cnt, NG, g, q, lane, acc);
+      finish_row(sd, lds, has_neg && t < rep, t, xr, xd, acc, G, NG, g, q);
     }
   }
+  (void)ipw;
 
   // copy phase: the previous batch's new rows into X (and the shadow); no
   // thread of this launch reads those rows of X (load_row redirects them)
   if (sd.prev_rows > 0) {
-    const int nb = blk_prefix[si + 1] - blk_prefix[si];
     const int64_t total4 = (int64_t)sd.prev_rows * D / 4;
     const int64_t per = (total4 + nb - 1) / nb;
     const int64_t lo = (int64_t)lb * per;
@@ -372,7 +520,7 @@ __global__ __launch_bounds__(512) void force_kernel(
     for (int64_t idx = lo + threadIdx.x; idx < hi; idx += blockDim.x) {
       const float4 v = S4[idx];
       X4[base4 + idx] = v;
-      if (sd.bf16) {
+      if (use_bf16<MODE>(sd)) {
         uint2 w;
         w.x = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
         w.y = bf16_bits(v.z) | (bf16_bits(v.w) << 16);
@@ -551,16 +699,25 @@ __global__ __launch_bounds__(1024) void reduce_kernel(const double *__restrict__
 
 // ---- launch wrappers -------------------------------------------------------
 
-// lds_bytes: staged negative rows, max over the launch's steps of ns * D * 4.
+// lds_bytes: staged negative rows and sub-range sums (force_lds_bytes);
+// mode: 0 all steps gather FP32, 1 all gather BF16, 2 mixed.
+template <int U>
+static void launch_force_u(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
+                           const StepDesc *steps, const int *blk_prefix, int n_steps, int ipw) {
+  if (mode == 0) hipLaunchKernelGGL((force_kernel<U, 0>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
+  else if (mode == 1) hipLaunchKernelGGL((force_kernel<U, 1>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
+  else hipLaunchKernelGGL((force_kernel<U, 2>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
+}
+
 hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_steps,
-                        int n_blocks, int wpb, int ipw, int unroll, size_t lds_bytes,
+                        int n_blocks, int wpb, int ipw, int unroll, size_t lds_bytes, int mode,
                         hipStream_t st) {
   dim3 grid(n_blocks), block(64 * wpb);
   switch (unroll) {
-  case 1: hipLaunchKernelGGL(force_kernel<1>, grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw); break;
-  case 2: hipLaunchKernelGGL(force_kernel<2>, grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw); break;
-  case 8: hipLaunchKernelGGL(force_kernel<8>, grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw); break;
-  default: hipLaunchKernelGGL(force_kernel<4>, grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw); break;
+  case 1: launch_force_u<1>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw); break;
+  case 2: launch_force_u<2>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw); break;
+  case 8: launch_force_u<8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw); break;
+  default: launch_force_u<4>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw); break;
   }
   return hipGetLastError();
 }

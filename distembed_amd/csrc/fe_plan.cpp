@@ -7,8 +7,9 @@
 namespace fe {
 
 fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int B, int chunk,
-                     int max_chunks, int64_t c_limit, Plan *out) {
-  if (n <= 0 || !row_ptr || B <= 0 || chunk <= 0 || max_chunks <= 0 || !out) {
+                     int max_chunks, int light, int pack, int64_t c_limit, Plan *out) {
+  if (n <= 0 || !row_ptr || B <= 0 || chunk <= 0 || max_chunks <= 0 || light < 0 || pack < 1 ||
+      !out) {
     set_host_error("build_plan: invalid argument");
     return FE_ERR_INVALID_ARG;
   }
@@ -16,6 +17,8 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
   p.B = B;
   p.chunk = chunk;
   p.max_chunks = max_chunks;
+  p.light = std::min(light, chunk);
+  p.pack = pack;
   // cpp/algo/algo.hpp:80-88 with proc_row_width = n at world size 1.
   if (n % B == 0) {
     p.batches = (int)(n / B);
@@ -25,37 +28,69 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
     p.last = (int)(n - (int64_t)B * (p.batches - 1));
   }
   p.item_off.assign(p.batches + 1, 0);
+  p.block_items.assign(p.batches, 0);
+  p.light_rows.assign(p.batches, 0);
   p.split_off.assign(p.batches + 1, 0);
   p.pslots.assign(p.batches, 0);
   p.edges_eff.assign(p.batches, 0);
   p.items.reserve((size_t)kItemInts * (size_t)(n + (row_ptr[n] / chunk) + 16));
+  const int64_t block_edges = (int64_t)kSub * chunk;
+  std::vector<int32_t> waves;
+  std::vector<std::pair<int32_t, int32_t>> lights; // (degree, row)
   for (int j = 0; j < p.batches; ++j) {
     const int64_t r0 = (int64_t)j * B;
     const int64_t r1 = std::min<int64_t>(r0 + B, n);
-    int32_t pcount = 0, scount = 0;
+    int32_t pcount = 0, scount = 0, nblk = 0;
     int64_t eff = 0;
+    waves.clear();
+    lights.clear();
     for (int64_t r = r0; r < r1; ++r) {
       const int64_t e0 = row_ptr[r];
       const int64_t deg = row_ptr[r + 1] - e0;
       if (col) {
         for (int64_t e = e0; e < e0 + deg; ++e) eff += (col[e] < c_limit) ? 1 : 0;
       }
-      if (deg <= chunk) {
-        p.items.insert(p.items.end(), {(int32_t)r, (int32_t)e0, (int32_t)deg, -1, 0, 0, 0, 0});
+      if (deg <= p.light) {
+        lights.push_back({(int32_t)deg, (int32_t)r});
         continue;
       }
-      const int64_t per = std::max<int64_t>(chunk, (deg + max_chunks - 1) / max_chunks);
+      if (deg <= chunk) {
+        waves.insert(waves.end(), {(int32_t)r, (int32_t)e0, (int32_t)deg, -1, 0, 0, 0, 0});
+        continue;
+      }
+      if (deg <= block_edges) {
+        p.items.insert(p.items.end(), {(int32_t)r, (int32_t)e0, (int32_t)deg, -1, 0, 0, 0, 0});
+        ++nblk;
+        continue;
+      }
+      const int64_t per = std::max<int64_t>(block_edges, (deg + max_chunks - 1) / max_chunks);
       const int64_t k = (deg + per - 1) / per;
       for (int64_t i = 0; i < k; ++i) {
         const int64_t lo = deg * i / k, hi = deg * (i + 1) / k;
         p.items.insert(p.items.end(), {(int32_t)r, (int32_t)(e0 + lo), (int32_t)(hi - lo),
                                        (int32_t)(pcount + i), scount, pcount, (int32_t)k, 0});
       }
+      nblk += (int32_t)k;
       p.splits.insert(p.splits.end(), {(int32_t)r, pcount, (int32_t)k, 0});
       pcount += (int32_t)k;
       ++scount;
     }
+    p.items.insert(p.items.end(), waves.begin(), waves.end());
+    // light rows by decreasing degree (row id breaks ties), so the rows of a
+    // pack have similar degrees; each entry carries its pack's largest degree
+    std::sort(lights.begin(), lights.end(),
+              [](const std::pair<int32_t, int32_t> &a, const std::pair<int32_t, int32_t> &b) {
+                return a.first != b.first ? a.first > b.first : a.second < b.second;
+              });
+    for (size_t i = 0; i < lights.size(); ++i) {
+      const int32_t r = lights[i].second;
+      const int32_t pack_max = lights[(i / pack) * pack].first;
+      p.items.insert(p.items.end(),
+                     {r, (int32_t)row_ptr[r], lights[i].first, -1, pack_max, 0, 0, 0});
+    }
     p.item_off[j + 1] = (int64_t)p.items.size() / kItemInts;
+    p.block_items[j] = nblk;
+    p.light_rows[j] = (int32_t)lights.size();
     p.split_off[j + 1] = (int64_t)p.splits.size() / 4;
     p.pslots[j] = pcount;
     p.max_pslots = std::max(p.max_pslots, (int)pcount);

@@ -65,15 +65,19 @@ enum {
  * REDRAW keeps drawing from the same engine until the value is below N.   */
 enum { FE_NEG_STRICT = 0, FE_NEG_REDRAW = 1 };
 
-/* Tiles chosen by the autotuner. A zero field keeps the current value.
- * chunk and max_chunks change the split of heavy rows and therefore the
- * floating-point summation order; unroll and waves_per_block do not.      */
+/* Tiles chosen by the autotuner. A zero field keeps the current value
+ * (light: a negative value sets 0). chunk, max_chunks and light change how
+ * rows are cut into work and therefore the floating-point summation order;
+ * unroll, waves_per_block and items_per_wave do not.                       */
 typedef struct fe_tuning {
-  int chunk;           /* edges per work item for ordinary rows (<= 64)     */
-  int max_chunks;      /* most work items (partials) one heavy row may use  */
+  int chunk;           /* rows of degree <= chunk: one wave per row; up to
+                          4*chunk: one block per row (<= 64)                */
+  int max_chunks;      /* most block chunks (partials) one heavy row may use */
   int unroll;          /* rows in flight per lane group: 1, 2, 4 or 8       */
   int waves_per_block; /* 1, 2, 4 or 8                                      */
-  int items_per_wave;  /* work items a wave takes in turn (>= 1)            */
+  int items_per_wave;  /* wave work units per wave, on average (>= 1)       */
+  int light;           /* rows of degree <= light: one row per lane group,
+                          64/(d/8) rows per wave (rows per wavefront)       */
 } fe_tuning;
 
 typedef struct fe_ctx fe_ctx;
@@ -148,6 +152,18 @@ fe_status fe_epoch_units(fe_ctx *ctx, int ns, int64_t *edges,
  * the last reset, measured with HIP events on the context's stream.
  * Timing is enabled with fe_set_timing(ctx, 1).                            */
 fe_status fe_set_timing(fe_ctx *ctx, int enable);
+
+/* Autotuner (no counterpart in the reference): times `epochs` epochs of the
+ * captured chain for each candidate tiling -- unroll {2,4,8} x waves per
+ * block {2,4,8} x items per wave {1,2,4}, and with search_chunking also
+ * chunk {16,32,64} x max_chunks {64,256,1024} -- restores X after every
+ * trial and keeps the fastest. The launch-shape fields never change results;
+ * the chunking fields change the summation order of heavy rows. Up to
+ * max_results trials are reported in results / results_ms (ms per epoch).
+ * Needs negative lists covering epochs + batches - 1 lists.                */
+fe_status fe_autotune(fe_ctx *ctx, int epochs, int search_chunking,
+                      fe_tuning *results, double *results_ms, int max_results,
+                      int *n_results, fe_tuning *best, double *best_ms);
 fe_status fe_kernel_times(fe_ctx *ctx, double *force_ms, double *fixup_ms,
                           double *apply_ms, int64_t *launches);
 

@@ -94,9 +94,11 @@ def test_other_widths(d):
 
 def test_hub_rows_split_into_chunks():
     g = api.Graph.rmat(12, 16, seed=5)
-    # small chunks force heavy rows through partials + the fixup kernel
+    # small chunks force heavy rows through block partials reduced by the
+    # last chunk; light rows go one per lane group unless light = 0
     for tun in ({"chunk": 4, "max_chunks": 8}, {"chunk": 32, "max_chunks": 256},
-                {"chunk": 64, "max_chunks": 1}):
+                {"chunk": 64, "max_chunks": 1}, {"chunk": 32, "light": -1},
+                {"chunk": 64, "max_chunks": 64, "light": 16}):
         _batch_parity(g, 128, 512, [(0, 0), (0, 1), (3, 7)], tuning=tun)
 
 
