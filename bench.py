@@ -5,16 +5,21 @@ Workload: R-MAT scale 20 (edge factor 16, a=0.57, b=c=0.19; symmetrised,
 duplicates and self-loops removed, natural vertex order), d=128, batch 16384,
 5 reference negatives per batch, lr 0.02 -- the one-GPU configuration the
 metric is quoted on. A step is one epoch: the 64 dependent minibatches of
-algo_force2_vec_ns over the whole graph, replayed as one captured HIP graph.
-Inputs are resident in HBM before the timed region starts.
+algo_force2_vec_ns over the whole graph, replayed as captured HIP graphs.
+Inputs are resident in HBM before the timed region starts; the launch
+tiling is chosen by the library's autotuner during warm-up.
 
-value = (processed CSR entries + (row, negative) pairs) / second.
+value = (processed CSR entries + (row, negative) pairs) / second, FP32.
 roofline: the force kernel's algorithmic bytes per launch (its share of the
 per-epoch byte count: gathered rows d*4+4 per processed entry, own row read
-and new row written d*4 each, the batch's negative rows) divided by its
-average launch duration from HIP events recorded on its own stream over the
-timed region. cpu_baseline: the oracle's FP64 restatement of the reference
-loop (OpenMP, schedule(static)) on a bounded sample of the same workload.
+and new row written d*4 each, row pointers, the batch's negative rows)
+divided by its average launch duration from HIP events recorded on its own
+stream over the timed region. traffic: HBM bytes per launch from the
+committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when present.
+bf16_form: the same workload with BF16 gathers (FP32 master and
+accumulation), its speed-up and its final loss against the FP32 run.
+cpu_baseline: the oracle's FP64 restatement of the reference loop (OpenMP,
+schedule(static)) on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -69,6 +74,53 @@ def cpu_baseline(g, lists, d, B, ns, lr, budget_s: float, max_epochs: int):
                       f"VALUE_TYPE), {threads} OpenMP threads, {t_total:.1f} s"}
 
 
+def run_form(api, g, X0, lists, gather, args, tuning=None):
+    """Autotune, warm up, time args.steps epochs; returns the measurements."""
+    d, B, ns, lr = args.d, args.batch, args.ns, args.lr
+    ctx = api.ForceEmbed(g.n, d, gather=gather)
+    try:
+        ctx.upload_csr(g)
+        ctx.upload_x(X0)
+        ctx.set_negatives(lists)
+        ctx.configure(B, lr, 5.0)
+        if tuning:
+            ctx.set_tuning(**tuning)
+        elif not args.no_autotune:
+            best, best_ms, table = ctx.autotune(epochs=2, search_chunking=True)
+            log(f"autotune ({'bf16' if gather else 'fp32'}): {best} {best_ms:.3f} ms/epoch "
+                f"over {len(table)} trials")
+        tun = ctx.get_tuning()
+        edges, n0, n1 = ctx.epoch_units(ns)
+        use_graph = not args.no_graph
+        if args.warmup:
+            ctx.train(args.warmup, B, ns, lr, use_graph=use_graph)
+        # throughput pass: plain graph replays, every form from the same X0
+        ctx.upload_x(X0)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.train(args.steps, B, ns, lr, use_graph=use_graph)
+        ctx.synchronize()
+        dt = time.perf_counter() - t0
+        attr, rep = ctx.loss(0, ns)
+        # kernel pass: the same epochs again from X0 with a HIP event pair on
+        # the context's stream around every force launch (reading the events
+        # costs a sync per epoch, so this pass is not the throughput figure)
+        ctx.upload_x(X0)
+        ctx.set_timing(True)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        ctx.train(args.steps, B, ns, lr, use_graph=use_graph)
+        ctx.synchronize()
+        dt_events = time.perf_counter() - t1
+        kt = ctx.kernel_times()
+        ctx.set_timing(False)
+    finally:
+        ctx.close()
+    units = args.steps * edges + (n0 + (args.steps - 1) * n1 if args.steps else 0)
+    return {"dt": dt, "dt_events": dt_events, "units": units, "edges": edges, "kt": kt,
+            "tuning": tun, "loss": attr + rep, "loss_parts": (attr, rep), "use_graph": use_graph}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -79,11 +131,12 @@ def main() -> int:
     ap.add_argument("--batch", type=int, default=16384)
     ap.add_argument("--ns", type=int, default=5)
     ap.add_argument("--lr", type=float, default=0.02)
-    ap.add_argument("--gather", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--cpu-budget", type=float, default=15.0, help="seconds of CPU baseline")
     ap.add_argument("--cpu-epochs", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="", help="json dict of fe_tuning fields")
     args = ap.parse_args()
 
@@ -103,50 +156,23 @@ def main() -> int:
     d, B, ns, lr = args.d, args.batch, args.ns, args.lr
     batches = -(-g.n // B)
     X0 = api.init_uniform(g.n, d, -0.5, 0.5, seed=0)
-    epochs_needed = max(args.steps, args.warmup, args.cpu_epochs)
+    epochs_needed = max(args.steps, args.warmup, args.cpu_epochs, 3)
     lists, hits = api.reference_negatives(g.n, epochs_needed + batches - 1, ns)
-    gather = api.BF16 if args.gather == "bf16" else api.FP32
-    ctx = api.ForceEmbed(g.n, d, gather=gather)
-    ctx.upload_csr(g)
-    ctx.upload_x(X0)
-    ctx.set_negatives(lists)
-    if args.tuning:
-        ctx.set_tuning(**json.loads(args.tuning))
-    ctx.configure(B, lr, 5.0)
-    edges, n0, n1 = ctx.epoch_units(ns)
-    use_graph = not args.no_graph
+    tuning = json.loads(args.tuning) if args.tuning else None
 
-    log(f"warmup: {args.warmup} epochs")
-    if args.warmup:
-        ctx.train(args.warmup, B, ns, lr, use_graph=use_graph)
-    ctx.synchronize()
+    f32 = run_form(api, g, X0, lists, api.FP32, args, tuning)
+    kt = f32["kt"]
+    log(f"fp32: {args.steps} epochs in {f32['dt']:.3f} s; force {kt['force_ms'] * 1e3:.1f} us "
+        f"per launch ({kt['launches']} launches); loss {f32['loss']:.6e}")
+    value = f32["units"] / f32["dt"]
 
-    # timed region: K epochs, per-launch HIP events on the context's stream
-    ctx.set_timing(True)
-    ctx.synchronize()
-    t0 = time.perf_counter()
-    ctx.train(args.steps, B, ns, lr, use_graph=use_graph)
-    ctx.synchronize()
-    dt = time.perf_counter() - t0
-    kt = ctx.kernel_times()
-    ctx.set_timing(False)
-    log(f"timed: {args.steps} epochs in {dt:.3f} s; force {kt['force_ms'] * 1e3:.1f} us, "
-        f"fixup {kt['fixup_ms'] * 1e3:.1f} us, apply {kt['apply_ms'] * 1e3:.1f} us per batch "
-        f"({kt['launches']} launches)")
-    units = args.steps * edges + (n0 + (args.steps - 1) * n1 if args.steps else 0)
-    value = units / dt
-
-    # untimed check of the result: finite, FP64 loss reduction on the device
-    attr, rep = ctx.loss(0, ns)
-    ctx.close()
-
-    s_g = 2 if gather == api.BF16 else 4
-    per_epoch_bytes = (edges * (d * s_g + 4) + (g.n + 1) * 8 + 2 * g.n * d * 4
-                       + batches * ns * d * s_g)
+    edges = f32["edges"]
+    per_epoch_bytes = (edges * (d * 4 + 4) + (g.n + 1) * 8 + 2 * g.n * d * 4
+                       + batches * ns * d * 4)
     bytes_per_launch = per_epoch_bytes / batches
     force_s = kt["force_ms"] * 1e-3
     achieved = bytes_per_launch / force_s / 1e9 if force_s > 0 else None
-    traffic = traffic_from_profile(f"force_kernel_rmat20_{args.gather}_pmc.txt")
+    traffic = traffic_from_profile("force_kernel_rmat20_d128_fp32_pmc.txt")
 
     out = {
         "metric": METRIC,
@@ -155,19 +181,20 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": dt / max(args.steps, 1) * 1e3,
+        "ms_per_step": f32["dt"] / max(args.steps, 1) * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32" if gather == api.FP32 else "bf16-gather/fp32-accumulate",
+        "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": f"rmat{args.scale}_d{d}_{args.gather}",
+            "workload": f"rmat{args.scale}_d{d}_fp32",
             "graph": f"R-MAT scale {args.scale}, edge factor 16, a=0.57 b=c=0.19, seed 0, "
                      "symmetrised, deduplicated, no self-loops",
             "n": g.n, "nnz": g.nnz, "d": d, "batch": B, "ns": ns, "lr": lr,
-            "epochs_timed": args.steps, "step": "one epoch (64 dependent minibatches)",
-            "graph_replay": use_graph, "negative_draws_equal_to_n": hits,
+            "epochs_timed": args.steps, "step": f"one epoch ({batches} dependent minibatches)",
+            "graph_replay": f32["use_graph"], "negative_draws_equal_to_n": hits,
+            "tuning": f32["tuning"],
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
@@ -176,14 +203,30 @@ def main() -> int:
             "kernel": "force_kernel",
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": kt["force_ms"],
+            "launches_timed": kt["launches"],
+            "event_pass_ms_per_step": f32["dt_events"] / max(args.steps, 1) * 1e3,
         },
-        "epoch_kernel_ms": {"force": kt["force_ms"] * batches, "fixup": kt["fixup_ms"] * batches,
-                            "apply": kt["apply_ms"] * batches},
-        "loss_fp64": {"attractive": attr, "repulsive_eval": rep},
+        "loss_fp64_after_timed_epochs": f32["loss"],
     }
+    if not args.no_bf16:
+        b16 = run_form(api, g, X0, lists, api.BF16, args, None)
+        kb = b16["kt"]
+        log(f"bf16: {args.steps} epochs in {b16['dt']:.3f} s; force {kb['force_ms'] * 1e3:.1f} us "
+            f"per launch; loss {b16['loss']:.6e}")
+        bf16_bytes = (edges * (d * 2 + 4) + (g.n + 1) * 8 + 2 * g.n * d * 4 + g.n * d * 2
+                      + batches * ns * d * 2) / batches
+        out["bf16_form"] = {
+            "value": b16["units"] / b16["dt"],
+            "ms_per_step": b16["dt"] / max(args.steps, 1) * 1e3,
+            "speedup_vs_fp32": f32["dt"] / b16["dt"],
+            "force_avg_launch_ms": kb["force_ms"],
+            "achieved_GBps": bf16_bytes / (kb["force_ms"] * 1e-3) / 1e9 if kb["force_ms"] else None,
+            "loss_fp64_after_timed_epochs": b16["loss"],
+            "loss_rel_diff_vs_fp32": (b16["loss"] - f32["loss"]) / f32["loss"],
+            "tuning": b16["tuning"],
+        }
     if not args.no_cpu:
-        X_cpu_lists = lists
-        out["cpu_baseline"] = cpu_baseline(g, X_cpu_lists, d, B, ns, lr, args.cpu_budget,
+        out["cpu_baseline"] = cpu_baseline(g, lists, d, B, ns, lr, args.cpu_budget,
                                            args.cpu_epochs)
     print(json.dumps(out), flush=True)
     return 0

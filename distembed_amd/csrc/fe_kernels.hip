@@ -164,6 +164,57 @@ __device__ __forceinline__ void load_row(const StepDesc &sd, int c, int q, float
   }
 }
 
+// A gathered row slice held while its load is in flight: 8 FP32 registers,
+// or with BF16 gathers the 8 packed BF16 values (4 registers), unpacked only
+// when used -- twice the rows in flight for the same registers.
+template <int MODE> struct RowBuf {
+  float f[8];
+};
+template <> struct RowBuf<1> {
+  u4v w;
+};
+
+template <int MODE>
+__device__ __forceinline__ void load_buf(const StepDesc &sd, int c, int q, RowBuf<MODE> &b) {
+  if constexpr (MODE == 1) {
+    const unsigned off = (unsigned)(c - sd.prev_begin);
+    if (off < (unsigned)sd.prev_rows) {
+      float o[8];
+      load8_f32(sd.Sprev + (size_t)off * sd.D + 8 * q, o);
+      b.w.x = bf16_bits(o[0]) | (bf16_bits(o[1]) << 16);
+      b.w.y = bf16_bits(o[2]) | (bf16_bits(o[3]) << 16);
+      b.w.z = bf16_bits(o[4]) | (bf16_bits(o[5]) << 16);
+      b.w.w = bf16_bits(o[6]) | (bf16_bits(o[7]) << 16);
+    } else {
+      b.w = *(gcu4 *)(sd.Xs + (size_t)c * sd.D + 8 * q);
+    }
+  } else {
+    load_row<MODE>(sd, c, q, b.f);
+  }
+}
+
+template <int MODE> __device__ __forceinline__ void zero_buf(RowBuf<MODE> &b) {
+  if constexpr (MODE == 1) {
+    b.w = u4v{0u, 0u, 0u, 0u};
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) b.f[k] = 0.f;
+  }
+}
+
+template <int MODE>
+__device__ __forceinline__ void unpack_buf(const RowBuf<MODE> &b, float (&o)[8]) {
+  if constexpr (MODE == 1) {
+    o[0] = __uint_as_float(b.w.x << 16); o[1] = __uint_as_float(b.w.x & 0xFFFF0000u);
+    o[2] = __uint_as_float(b.w.y << 16); o[3] = __uint_as_float(b.w.y & 0xFFFF0000u);
+    o[4] = __uint_as_float(b.w.z << 16); o[5] = __uint_as_float(b.w.z & 0xFFFF0000u);
+    o[6] = __uint_as_float(b.w.w << 16); o[7] = __uint_as_float(b.w.w & 0xFFFF0000u);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) o[k] = b.f[k];
+  }
+}
+
 // Stage the `ns` negative rows of list `list` as FP32 in LDS.
 template <int MODE>
 __device__ __forceinline__ void stage_negatives(const StepDesc &sd, int list, float *lds) {
@@ -254,27 +305,24 @@ __device__ __forceinline__ void gather_edges(const StepDesc &sd, const float (&x
     const int cl = lane < m ? gld_int(sd.col + e0 + eb + lane) : 0;
     const int rounds = (m + NG - 1) / NG;
     for (int u0 = 0; u0 < rounds; u0 += U) {
-      float xc[U][8];
+      RowBuf<MODE> xb[U];
       bool ok[U];
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
         const int k = (u0 + uu) * NG + g;
         const int c = __shfl(cl, k & 63);
         ok[uu] = (k < m) && (c < c_limit);
-        if (ok[uu]) {
-          load_row<MODE>(sd, c, q, xc[uu]);
-        } else {
-#pragma unroll
-          for (int k2 = 0; k2 < 8; ++k2) xc[uu][k2] = 0.f;
-        }
+        if (ok[uu]) load_buf<MODE>(sd, c, q, xb[uu]);
+        else zero_buf<MODE>(xb[uu]);
       }
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
-        float v[8];
+        float xc[8], v[8];
+        unpack_buf<MODE>(xb[uu], xc);
         float p = 0.f;
 #pragma unroll
         for (int k2 = 0; k2 < 8; ++k2) {
-          v[k2] = ok[uu] ? xd[k2] - xc[uu][k2] : 0.f;
+          v[k2] = ok[uu] ? xd[k2] - xc[k2] : 0.f;
           p = fmaf(v[k2], v[k2], p);
         }
         const float s = group_sum(p, G);
@@ -337,27 +385,24 @@ __device__ __forceinline__ void light_pack(const StepDesc &sd, const int32_t *lr
     const int cl = (kb + q < deg) ? gld_int(sd.col + e0 + kb + q) : 0;
     const int kend = min(kmax, kb + G);
     for (int k0 = kb; k0 < kend; k0 += U) {
-      float xc[U][8];
+      RowBuf<MODE> xb[U];
       bool ok[U];
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
         const int k = k0 + uu;
         const int c = __shfl(cl, g * G + ((k - kb) & (G - 1)));
         ok[uu] = (k < kend) && (k < deg) && (c < c_limit);
-        if (ok[uu]) {
-          load_row<MODE>(sd, c, q, xc[uu]);
-        } else {
-#pragma unroll
-          for (int k2 = 0; k2 < 8; ++k2) xc[uu][k2] = 0.f;
-        }
+        if (ok[uu]) load_buf<MODE>(sd, c, q, xb[uu]);
+        else zero_buf<MODE>(xb[uu]);
       }
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
-        float v[8];
+        float xc[8], v[8];
+        unpack_buf<MODE>(xb[uu], xc);
         float p = 0.f;
 #pragma unroll
         for (int k2 = 0; k2 < 8; ++k2) {
-          v[k2] = ok[uu] ? xd[k2] - xc[uu][k2] : 0.f;
+          v[k2] = ok[uu] ? xd[k2] - xc[k2] : 0.f;
           p = fmaf(v[k2], v[k2], p);
         }
         const float s = group_sum(p, G);

@@ -1,0 +1,124 @@
+"""Summarise rocprofv3 CSV output into the short text kept under profiles/.
+
+  python -m distembed_amd.profile_summary trace  <kernel_stats.csv> <kernel_trace.csv> <out> <command>
+  python -m distembed_amd.profile_summary pmc <fetch csv> <write csv> <out> <command> <copyprobe fetch csv> <copyprobe write csv>
+  python -m distembed_amd.profile_summary copyprobe
+
+trace: per-kernel launch count, total and average duration (from the
+--kernel-trace --stats run), plus the force kernel's per-launch average.
+pmc: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the force kernel,
+scaled by what the same counters read for a cold 1 GiB device-to-device
+copy (copyprobe), and the resulting HBM bytes per force launch
+(hbm_bytes_per_launch).
+"""
+from __future__ import annotations
+
+import csv
+import statistics
+import sys
+from collections import defaultdict
+
+
+def _short(name: str) -> str:
+    return name.split("(")[0].replace("void ", "").strip()
+
+
+def trace(stats_csv: str, trace_csv: str, out: str, command: str) -> None:
+    rows = list(csv.DictReader(open(stats_csv)))
+    lines = [f"# rocprofv3 kernel statistics", f"command: {command}", ""]
+    lines.append(f"{'kernel':60s} {'calls':>7s} {'total_ms':>10s} {'avg_us':>9s}")
+    for r in rows:
+        name = _short(r.get("Name", r.get("KernelName", "?")))
+        calls = int(float(r.get("Calls", 0)))
+        tot = float(r.get("TotalDurationNs", 0)) / 1e6
+        avg = float(r.get("AverageNs", 0)) / 1e3
+        lines.append(f"{name[:60]:60s} {calls:7d} {tot:10.3f} {avg:9.2f}")
+    durs = defaultdict(list)
+    for r in csv.DictReader(open(trace_csv)):
+        durs[_short(r["Kernel_Name"])].append(
+            (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    lines.append("")
+    for k, v in sorted(durs.items()):
+        if "force_kernel" in k:
+            lines.append(f"force launches {k}: n={len(v)} avg_us={statistics.mean(v):.2f} "
+                         f"median_us={statistics.median(v):.2f} max_us={max(v):.2f}")
+    open(out, "w").write("\n".join(lines) + "\n")
+
+
+def _per_dispatch(path: str, counter: str):
+    vals = defaultdict(dict)  # kernel -> dispatch -> value
+    for r in csv.DictReader(open(path)):
+        if r.get("Counter_Name") != counter:
+            continue
+        vals[_short(r["Kernel_Name"])][int(r["Dispatch_Id"])] = float(r["Counter_Value"])
+    return vals
+
+
+def pmc(fetch_csv: str, write_csv: str, out: str, command: str, cal_fetch_csv: str,
+        cal_write_csv: str) -> None:
+    fetch = _per_dispatch(fetch_csv, "FETCH_SIZE")
+    write = _per_dispatch(write_csv, "WRITE_SIZE")
+    cal_f = calibration(cal_fetch_csv, "FETCH_SIZE")
+    cal_w = calibration(cal_write_csv, "WRITE_SIZE")
+    lines = ["# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch)",
+             f"command (fetch): rocprofv3 --pmc FETCH_SIZE --output-format csv -- {command}",
+             f"command (write): rocprofv3 --pmc WRITE_SIZE --output-format csv -- {command}",
+             "calibration: rocprofv3 --pmc <counter> -- python3 -m distembed_amd.profile_summary "
+             "copyprobe (a cold 1 GiB device-to-device copy)", ""]
+    lines.append(f"calibration factors: FETCH_SIZE reads {cal_f:.4f} x the copied KiB, "
+                 f"WRITE_SIZE reads {cal_w:.4f} x")
+    for k in sorted(fetch):
+        if "force_kernel" not in k:
+            continue
+        f = statistics.mean(fetch[k].values())
+        w = statistics.mean(write.get(k, {0: 0.0}).values())
+        fb = f * 1024 / (cal_f or 1.0)
+        wb = w * 1024 / (cal_w or 1.0)
+        lines.append(f"{k}: dispatches={len(fetch[k])} fetch_KiB={f:.1f} write_KiB={w:.1f}")
+        lines.append(f"hbm_bytes_per_launch = {fb + wb:.6e}")
+        lines.append(f"hbm_read_bytes_per_launch = {fb:.6e}")
+        lines.append(f"hbm_write_bytes_per_launch = {wb:.6e}")
+    open(out, "w").write("\n".join(lines) + "\n")
+
+
+def copyprobe(nbytes: int = 1 << 30) -> None:
+    """A plain device-to-device copy of `nbytes` cold bytes (the HIP runtime's
+    copy kernel), run under rocprofv3 --pmc to calibrate FETCH_SIZE and
+    WRITE_SIZE: the counters should read nbytes / 1024 KiB each."""
+    import ctypes as C
+    from . import _lib
+    _lib.load()
+    hip = C.CDLL("libamdhip64.so.7")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    # flush the caches between the fill and the copy with an unrelated copy
+    c, d2 = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(c), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(d2), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemcpy(d2, c, C.c_size_t(nbytes), 3) == 0
+    assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    print(f"copyprobe: copied {nbytes} bytes device to device", flush=True)
+
+
+def calibration(path: str, counter: str, nbytes: int = 1 << 30) -> float:
+    """Counter KiB of the last runtime copy kernel / the KiB it copied."""
+    vals = []
+    for r in csv.DictReader(open(path)):
+        if r.get("Counter_Name") == counter and "copyBuffer" in r["Kernel_Name"]:
+            vals.append((int(r["Dispatch_Id"]), float(r["Counter_Value"])))
+    vals.sort()
+    return vals[-1][1] / (nbytes / 1024.0)
+
+
+if __name__ == "__main__":
+    mode = sys.argv[1]
+    if mode == "trace":
+        trace(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5])
+    elif mode == "copyprobe":
+        copyprobe()
+    else:
+        pmc(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5], sys.argv[6], sys.argv[7])
