@@ -27,7 +27,6 @@ struct fe_ctx {
   std::string err;
 };
 
-static thread_local std::string g_create_error;
 
 static fe_status fail(fe_ctx *c, fe_status s, const std::string &msg) {
   if (c) c->err = msg;
@@ -60,7 +59,7 @@ static void drop_graph(fe_ctx *c) {
 extern "C" fe_status fe_create(fe_ctx **out, int device, int64_t n, int d, int gather_dtype,
                                uint32_t quirks) {
   if (!out) {
-    g_create_error = "fe_create: out is NULL";
+    fe::set_host_error("fe_create: out is NULL");
     return FE_ERR_INVALID_ARG;
   }
   *out = nullptr;
@@ -68,16 +67,16 @@ extern "C" fe_status fe_create(fe_ctx **out, int device, int64_t n, int d, int g
   hipError_t e = hipGetDeviceCount(&count);
   if (e != hipSuccess || count <= 0) {
     (void)hipGetLastError();
-    g_create_error = "fe_create: no HIP device available";
+    fe::set_host_error("fe_create: no HIP device available");
     return FE_ERR_NO_DEVICE;
   }
   if (device < 0 || device >= count) {
-    g_create_error = "fe_create: device index out of range";
+    fe::set_host_error("fe_create: device index out of range");
     return FE_ERR_NO_DEVICE;
   }
   fe_ctx *c = new (std::nothrow) fe_ctx();
   if (!c) {
-    g_create_error = "fe_create: out of host memory";
+    fe::set_host_error("fe_create: out of host memory");
     return FE_ERR_OOM;
   }
   c->device = device;
@@ -89,7 +88,7 @@ extern "C" fe_status fe_create(fe_ctx **out, int device, int64_t n, int d, int g
     s = hip_ok(c, hipMemcpy(c->d_epoch_list, &c->job.epoch, sizeof(int *), hipMemcpyHostToDevice),
                "hipMemcpy");
   if (s != FE_OK) {
-    g_create_error = "fe_create: " + c->err;
+    fe::set_host_error("fe_create: " + c->err);
     job_free(c->job);
     if (c->d_epoch_list) hipFree(c->d_epoch_list);
     if (c->stream) hipStreamDestroy(c->stream);
@@ -113,7 +112,7 @@ extern "C" void fe_destroy(fe_ctx *c) {
 }
 
 extern "C" const char *fe_last_error(const fe_ctx *c) {
-  if (!c) return g_create_error.c_str();
+  if (!c) return fe::host_error();
   return c->err.c_str();
 }
 

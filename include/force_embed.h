@@ -89,8 +89,9 @@ typedef struct fe_ctx fe_ctx;
 fe_status fe_create(fe_ctx **out, int device, int64_t n, int d,
                     int gather_dtype, uint32_t quirks);
 void fe_destroy(fe_ctx *ctx);
-/* Message of the last failure on `ctx` (or of the last failed fe_create when
- * ctx is NULL). Never NULL.                                                */
+/* Message of the last failure on `ctx`; with ctx NULL, of the calling
+ * thread's last failed call that takes no context (fe_create and the host
+ * helpers below). Never NULL.                                              */
 const char *fe_last_error(const fe_ctx *ctx);
 
 /* ---- data (replaces SpMat/CSRLocal native CSR, cpp/core/csr_local.hpp:37-125,

@@ -19,6 +19,11 @@
 // fused multiply-add changes the rounding. VALUE_TYPE = double is the
 // reference's own precision (cpp/core/common.h:35) and the FP64 oracle.
 //
+// The BF16-shadow form (orc_train_bf16) has no counterpart in the reference:
+// every read of X inside a batch sees X rounded to bfloat16 (round to nearest
+// even), the forces are computed from those values in float exactly as above,
+// and the batch's update is added to the float master X.
+//
 // Pinning (tests/test_oracle_*.py): the sampler against std:: compiled with
 // g++; the per-batch update against torch autograd of the force-model loss in
 // FP64 (an independent route to the same update) and against hand-computed
@@ -29,6 +34,7 @@
 #include <stdint.h>
 
 #include <cmath>
+#include <cstring>
 #include <random>
 #include <vector>
 
@@ -180,6 +186,50 @@ int train_impl(int64_t n, int d, const int64_t *rp, const int32_t *col, T *X, in
   return 0;
 }
 
+// float -> bfloat16 -> float, round to nearest even; NaN stays NaN.
+inline float bf16_round(float f) {
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  if ((u & 0x7F800000u) == 0x7F800000u && (u & 0x007FFFFFu)) {
+    u |= 0x00400000u;
+  } else {
+    u += 0x7FFFu + ((u >> 16) & 1u);
+  }
+  u &= 0xFFFF0000u;
+  float r;
+  std::memcpy(&r, &u, 4);
+  return r;
+}
+
+// The BF16-shadow form of train_impl<float>: each batch computes its forces
+// on a rounded copy of X and adds them to X.
+int train_bf16_impl(int64_t n, int d, const int64_t *rp, const int32_t *col, float *X, int B,
+                    int iterations, const int32_t *lists, int n_lists, int ns, float lr,
+                    float bound, unsigned quirks, int threads) {
+  if (n <= 0 || d <= 0 || B <= 0 || iterations < 0 || ns < 0) return -1;
+  const Shape sh = make_shape(n, d, B);
+  if (ns > 0 && (int64_t)iterations + sh.batches - 1 > n_lists) return -1;
+  if (ns > 0 && check_lists<float>(n, lists, (int64_t)(iterations + sh.batches - 1) * ns))
+    return -2;
+  if (threads > 0) omp_set_num_threads(threads);
+  std::vector<float> prev((size_t)sh.B * d), R((size_t)n * d);
+  int considering_batch_size = sh.B;
+  for (int i = 0; i < iterations; i++) {
+    for (int j = 0; j < sh.batches; j++) {
+      if (j == sh.batches - 1) considering_batch_size = sh.last;
+      else if (!(quirks & 1u)) considering_batch_size = sh.B;
+      const int32_t *neg = ns > 0 ? lists + (int64_t)(i + j) * ns : nullptr;
+      for (int64_t k = 0; k < n * d; ++k) R[k] = bf16_round(X[k]);
+      batch_step<float>(sh, rp, col, R.data(), j, considering_batch_size, neg, ns, lr, bound,
+                        quirks, prev.data(), false);
+      const int64_t r0 = (int64_t)j * sh.B;
+      const int64_t rows = std::min<int64_t>(r0 + sh.B, n) - r0;
+      for (int64_t k = 0; k < rows * d; ++k) X[r0 * d + k] += prev[k];
+    }
+  }
+  return 0;
+}
+
 uint64_t mix64(uint64_t z) {
   z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
   z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
@@ -229,6 +279,17 @@ int orc_train_f64(int64_t n, int d, const int64_t *rp, const int32_t *col, doubl
                   double bound, unsigned quirks, int threads) {
   return train_impl<double>(n, d, rp, col, X, B, iterations, lists, n_lists, ns, lr, bound,
                             quirks, threads);
+}
+
+void orc_bf16_round(const float *in, float *out, int64_t count) {
+  for (int64_t i = 0; i < count; ++i) out[i] = bf16_round(in[i]);
+}
+
+int orc_train_bf16(int64_t n, int d, const int64_t *rp, const int32_t *col, float *X, int B,
+                   int iterations, const int32_t *lists, int n_lists, int ns, float lr,
+                   float bound, unsigned quirks, int threads) {
+  return train_bf16_impl(n, d, rp, col, X, B, iterations, lists, n_lists, ns, lr, bound, quirks,
+                         threads);
 }
 
 // Force-model loss in FP64: attractive log1p(s) over CSR entries with

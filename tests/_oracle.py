@@ -41,9 +41,13 @@ def load():
                                   C.c_int, C.c_int, C.c_float, C.c_float, C.c_uint, C.c_int]
     lib.orc_train_f64.argtypes = [C.c_int64, C.c_int, I64P, I32P, DP, C.c_int, C.c_int, I32P,
                                   C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint, C.c_int]
+    lib.orc_train_bf16.argtypes = lib.orc_train_f32.argtypes
+    lib.orc_bf16_round.argtypes = [FP, FP, C.c_int64]
+    lib.orc_bf16_round.restype = None
     lib.orc_loss_f64.argtypes = [C.c_int64, C.c_int, I64P, I32P, DP, C.c_uint, C.c_uint64, C.c_int,
                                  DP, DP, C.c_int]
-    for f in ("orc_step_f32", "orc_step_f64", "orc_train_f32", "orc_train_f64", "orc_loss_f64"):
+    for f in ("orc_step_f32", "orc_step_f64", "orc_train_f32", "orc_train_f64", "orc_train_bf16",
+              "orc_loss_f64"):
         getattr(lib, f).restype = C.c_int
     _lib = lib
     return lib
@@ -96,13 +100,28 @@ def step(g, X, B, epoch, batch, lists, lr=0.02, bound=5.0, quirks=3, apply=True,
     return delta
 
 
-def train(g, X, B, iterations, lists, lr=0.02, bound=5.0, quirks=3, threads=0):
+def bf16_round(X):
+    """X rounded to bfloat16 (nearest even), returned as float32."""
+    X = np.ascontiguousarray(X, dtype=np.float32)
+    out = np.empty_like(X)
+    load().orc_bf16_round(_p(X, C.c_float), _p(out, C.c_float), X.size)
+    return out
+
+
+def train(g, X, B, iterations, lists, lr=0.02, bound=5.0, quirks=3, threads=0, bf16=False):
+    """Training in place. bf16: the BF16-shadow form (float X only): each
+    batch's forces come from X rounded to bfloat16, added to float X."""
     lib = load()
     lists, n_lists, ns = _lists_arg(lists)
     rp = np.ascontiguousarray(g.row_ptr, dtype=np.int64)
     ci = np.ascontiguousarray(g.col_idx, dtype=np.int32)
     d = X.shape[1]
-    if X.dtype == np.float32:
+    if bf16:
+        assert X.dtype == np.float32
+        st = lib.orc_train_bf16(g.n, d, _p(rp, C.c_int64), _p(ci, C.c_int32), _p(X, C.c_float), B,
+                                iterations, _p(lists, C.c_int32), n_lists, ns, lr, bound, quirks,
+                                threads)
+    elif X.dtype == np.float32:
         st = lib.orc_train_f32(g.n, d, _p(rp, C.c_int64), _p(ci, C.c_int32), _p(X, C.c_float), B,
                                iterations, _p(lists, C.c_int32), n_lists, ns, lr, bound, quirks,
                                threads)

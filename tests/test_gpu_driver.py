@@ -1,0 +1,70 @@
+"""The dist_embed driver (C++ mirror of the reference API over the C ABI) on
+the GPU: its embedding.txt against the oracle run on the same graph, X0 and
+negative lists."""
+import subprocess
+
+import numpy as np
+import pytest
+
+import _oracle as orc
+from distembed_amd import api, build
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(tmp_path, *args):
+    if not build.DRIVER.exists():
+        pytest.fail("dist_embed driver is not built")
+    out = tmp_path / "out"
+    r = subprocess.run([str(build.DRIVER), *args, "-output", str(out)], capture_output=True,
+                       text=True, cwd=tmp_path, timeout=300)
+    assert r.returncode == 0, r.stderr
+    rows = np.loadtxt(out / "embedding.txt", dtype=np.float64, ndmin=2)
+    assert np.array_equal(rows[:, 0], np.arange(1, rows.shape[0] + 1))
+    return rows[:, 1:], r.stdout
+
+
+def _oracle_x(g, d, B, T, ns, bf16=False):
+    X = api.init_uniform(g.n, d, -1.0, 1.0, seed=0)  # DenseMat's U(-1, 1), seed 0
+    batches = -(-g.n // B)
+    lists = api.reference_negatives(g.n, T + batches - 1, ns)[0]
+    orc.train(g, X, B, T, lists, bf16=bf16)
+    return X
+
+
+def _sbm(n):
+    nb = n / 10
+    return api.Graph.sbm(n, 10, min(1.0, 18 / (nb - 1)), min(1.0, 2 / (n - nb)), seed=0)
+
+
+def test_driver_sbm_matches_oracle(tmp_path):
+    n, d, B, T = 900, 64, 128, 4
+    got, stdout = _run(tmp_path, "-sbm", str(n), "-dim", str(d), "-batch", str(B), "-iter",
+                       str(T), "-nsamples", "5", "-lr", "0.02", "-dataset", "sbm900")
+    want = _oracle_x(_sbm(n), d, B, T, 5)
+    # the file holds 5 decimals: half a unit of the last place, plus FP32 rounding
+    np.testing.assert_allclose(got, want, rtol=0, atol=5e-6 + 1e-4 * np.abs(want).max())
+    perf = (tmp_path / "perf_output").read_text()
+    assert '"algo": "Embedding"' in perf and '"Total Time"' in perf and perf.rstrip().endswith(",")
+
+
+def test_driver_mtx_input_equals_generated_graph(tmp_path):
+    n, d, B, T = 700, 32, 100, 3
+    g = _sbm(n)
+    rows = np.repeat(np.arange(n), np.diff(g.row_ptr))
+    keep = rows > g.col_idx  # each undirected edge once, as a symmetric file lists it
+    lines = [f"{c + 1} {r + 1}" for r, c in zip(rows[keep], g.col_idx[keep])]
+    mtx = tmp_path / "g.mtx"
+    mtx.write_text("%%MatrixMarket matrix coordinate pattern symmetric\n"
+                   f"{n} {n} {len(lines)}\n" + "\n".join(lines) + "\n")
+    a, _ = _run(tmp_path, "-input", str(mtx), "-dim", str(d), "-batch", str(B), "-iter", str(T))
+    b, _ = _run(tmp_path, "-sbm", str(n), "-dim", str(d), "-batch", str(B), "-iter", str(T))
+    np.testing.assert_array_equal(a, b)
+
+
+def test_driver_bf16_gather_matches_bf16_oracle(tmp_path):
+    n, d, B, T = 800, 128, 200, 3
+    got, _ = _run(tmp_path, "-sbm", str(n), "-dim", str(d), "-batch", str(B), "-iter", str(T),
+                  "-gather", "bf16")
+    want = _oracle_x(_sbm(n), d, B, T, 5, bf16=True)
+    np.testing.assert_allclose(got, want, rtol=0, atol=5e-6 + 1e-3 * np.abs(want).max())

@@ -231,6 +231,51 @@ def test_train_equals_sequence_of_steps(quirks):
     np.testing.assert_array_equal(a, b)
 
 
+def test_bf16_round_matches_torch_bfloat16():
+    # torch's float -> bfloat16 conversion rounds to nearest even: an
+    # independent implementation of the same rounding
+    torch = pytest.importorskip("torch")
+    rng = np.random.default_rng(3)
+    base = rng.standard_normal(4096).astype(np.float32) * np.float32(3.0)
+    u = base.view(np.uint32)
+    ties_even = ((u & np.uint32(0xFFFE0000)) | np.uint32(0x8000)).view(np.float32)
+    ties_odd = ((u & np.uint32(0xFFFE0000)) | np.uint32(0x18000)).view(np.float32)
+    special = np.array([0.0, -0.0, 1e-40, -3e-39, 3.3e38, -3.3e38, 1.0, 0.5 + 2 ** -9,
+                        np.inf, -np.inf], dtype=np.float32)
+    x = np.concatenate([base, ties_even, ties_odd, special])
+    want = torch.from_numpy(x).to(torch.bfloat16).to(torch.float32).numpy()
+    np.testing.assert_array_equal(orc.bf16_round(x).view(np.uint32), want.view(np.uint32))
+    assert np.isnan(orc.bf16_round(np.array([np.nan], dtype=np.float32))[0])
+
+
+def test_bf16_train_is_fp32_forces_at_rounded_rows():
+    n, d, B, T = 37, 16, 8, 3
+    g = _rand_graph(n, 90, seed=4)
+    rng = np.random.default_rng(9)
+    X = rng.uniform(-0.5, 0.5, (n, d)).astype(np.float32)
+    batches = 5
+    lists, _ = api.reference_negatives(n, T + batches - 1, 5)
+    # on a matrix that BF16 holds exactly, the first batch is the FP32 batch
+    Xe = orc.bf16_round(X)
+    a, b = Xe.copy(), Xe.copy()
+    g1 = _graph_from_pairs(n, [(r, c) for r in range(B) for c in
+                               g.col_idx[g.row_ptr[r]:g.row_ptr[r + 1]]])
+    orc.train(g1, a, n, 1, lists, bf16=True)
+    orc.train(g1, b, n, 1, lists)
+    np.testing.assert_array_equal(a, b)
+    # every batch re-reads the rounded matrix, including rows that the
+    # previous batch just moved, and adds its forces to the float master
+    a = X.copy()
+    orc.train(g, a, B, T, lists, bf16=True)
+    b = X.copy()
+    for i in range(T):
+        for j in range(batches):
+            delta = orc.step(g, orc.bf16_round(b), B, i, j, lists, apply=False, want_delta=True)
+            b[j * B:j * B + delta.shape[0]] += delta
+    np.testing.assert_array_equal(a, b)
+    assert not np.array_equal(a, orc.bf16_round(a))  # the master keeps float precision
+
+
 def test_rejects_out_of_range_negative():
     g = _rand_graph(8, 10, seed=0)
     X = np.zeros((8, 8), dtype=np.float32)
