@@ -20,6 +20,9 @@ bf16_form: the same workload with BF16 gathers (FP32 master and
 accumulation), its speed-up and its final loss against the FP32 run.
 cpu_baseline: the oracle's FP64 restatement of the reference loop (OpenMP,
 schedule(static)) on a bounded sample of the same workload.
+
+--form precision | batching | serving runs the other workloads instead
+(distembed_amd/bench_forms.py) and prints their results as one JSON line.
 """
 from __future__ import annotations
 
@@ -138,6 +141,13 @@ def main() -> int:
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--tuning", type=str, default="", help="json dict of fe_tuning fields")
+    ap.add_argument("--form", choices=["plain", "precision", "batching", "serving"],
+                    default="plain")
+    ap.add_argument("--epochs", type=int, default=50, help="epochs of the batching/serving jobs")
+    ap.add_argument("--graphs", type=int, default=512, help="graphs of the batching form")
+    ap.add_argument("--jobs", type=int, default=48, help="jobs per load level (serving form)")
+    ap.add_argument("--qps", type=str, default="4,8,16", help="offered loads (serving form)")
+    ap.add_argument("--budget-gib", type=float, default=0.0, help="engine HBM budget")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -149,6 +159,19 @@ def main() -> int:
     if _lib.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 3
+
+    if args.form != "plain":
+        from distembed_amd import bench_forms as bf
+        if args.form == "precision":
+            res = bf.precision(args.steps, args.warmup)
+        elif args.form == "batching":
+            res = bf.batching(args.epochs, count=args.graphs, budget_gib=args.budget_gib or 96.0)
+        else:
+            res = bf.serving(tuple(float(q) for q in args.qps.split(",")), args.jobs,
+                             args.epochs, budget_gib=args.budget_gib or 8.0)
+        res.update({"n_gpus": 1, "data": "synthetic", "unit": "edges+negatives/s"})
+        print(json.dumps(res), flush=True)
+        return 0
 
     t0 = time.perf_counter()
     g = api.Graph.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=0)

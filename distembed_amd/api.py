@@ -26,6 +26,13 @@ def _ptr(a: np.ndarray, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
+def _tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light):
+    """fe_tuning from optional fields: 0 keeps a field, light -1 sets 0."""
+    z = lambda v: 0 if v is None else int(v)
+    lt = 0 if light is None else (-1 if int(light) == 0 else int(light))
+    return fe_tuning(z(chunk), z(max_chunks), z(unroll), z(waves_per_block), z(items_per_wave), lt)
+
+
 def _host_err() -> str:
     return _lib_or_raise().fe_last_error(None).decode()
 
@@ -172,9 +179,10 @@ class ForceEmbed:
     def configure(self, batch_size: int, lr: float = 0.02, max_bound: float = 5.0) -> None:
         self._check(self._lib.fe_configure(self._ctx, batch_size, lr, max_bound))
 
-    def set_tuning(self, chunk=0, max_chunks=0, unroll=0, waves_per_block=0,
-                   items_per_wave=0, light=0) -> None:
-        t = fe_tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light)
+    def set_tuning(self, chunk=None, max_chunks=None, unroll=None, waves_per_block=None,
+                   items_per_wave=None, light=None) -> None:
+        """Fields left at None keep their value; light=0 turns light rows off."""
+        t = _tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light)
         self._check(self._lib.fe_set_tuning(self._ctx, C.byref(t)))
 
     def get_tuning(self) -> dict:
@@ -323,9 +331,10 @@ class Engine:
         self._check(self._lib.fe_engine_collect(self._e, job_id, _ptr(x, C.c_float)))
         return x
 
-    def set_tuning(self, chunk=0, max_chunks=0, unroll=0, waves_per_block=0,
-                   items_per_wave=0, light=0) -> None:
-        t = fe_tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light)
+    def set_tuning(self, chunk=None, max_chunks=None, unroll=None, waves_per_block=None,
+                   items_per_wave=None, light=None) -> None:
+        """As ForceEmbed.set_tuning; chunking applies to jobs submitted later."""
+        t = _tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light)
         self._check(self._lib.fe_engine_set_tuning(self._e, C.byref(t)))
 
     def stats(self):

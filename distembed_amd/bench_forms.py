@@ -1,0 +1,226 @@
+"""Workloads of the precision, batching and serving forms (bench.py --form).
+
+precision: R-MAT scale 20 and 22, d=128, FP32 and BF16-shadow gathers,
+  5 and 10 negatives: epoch time, BF16 speed-up and final loss of each.
+batching: 512 independent SBM graphs (n log-uniform in [1e3, 2e5], 4-32
+  blocks, average degree uniform in [8, 40] with 90 % of it inside the
+  block), d=64, B=1024, 5 negatives, 50 epochs, FP32 and BF16; all graphs
+  resident, advanced together round by round, the rounds captured in HIP
+  graphs (fe_engine_run_graph), tiles chosen by the autotuner.
+serving: Poisson arrivals of R-MAT jobs (edge count log-uniform in
+  [2e4, 2e7], d from {32, 64, 128}), 50 epochs, 5 negatives, admitted under
+  an HBM budget and interleaved round by round (fe_engine_run); job latency
+  is arrival -> result on the host, reported as p50/p99 with the aggregate
+  edges+negatives/s, at three offered loads.
+"""
+from __future__ import annotations
+
+import math
+import sys
+import time
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+
+from . import api
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _units(g: api.Graph, B: int, ns: int, epochs: int, quirks: int = 3) -> int:
+    """Edge plus negative interactions of `epochs` epochs (reference loop)."""
+    n = g.n
+    edges = int((g.col_idx < n - 1).sum()) if quirks & 2 else g.nnz
+    batches = -(-n // B)
+    last = n - B * (batches - 1)
+    first = (B * (batches - 1) + last) * ns
+    later = (batches * last if (quirks & 1) else B * (batches - 1) + last) * ns
+    return epochs * edges + (first + (epochs - 1) * later if epochs else 0)
+
+
+# ---- precision form --------------------------------------------------------
+
+def precision(steps: int, warmup: int) -> dict:
+    out = []
+    for scale in (20, 22):
+        t0 = time.perf_counter()
+        g = api.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=0)
+        log(f"precision: R-MAT {scale}: n={g.n} nnz={g.nnz} ({time.perf_counter() - t0:.1f} s)")
+        d, B, lr = 128, 16384, 0.02
+        X0 = api.init_uniform(g.n, d, -0.5, 0.5, seed=0)
+        batches = -(-g.n // B)
+        for ns in (5, 10):
+            lists = api.reference_negatives(g.n, steps + warmup + batches, ns)[0]
+            row = {"graph": f"rmat{scale}", "n": g.n, "nnz": g.nnz, "ns": ns}
+            for name, gather in (("fp32", api.FP32), ("bf16", api.BF16)):
+                with api.ForceEmbed(g.n, d, gather=gather) as ctx:
+                    ctx.upload_csr(g)
+                    ctx.upload_x(X0)
+                    ctx.set_negatives(lists)
+                    ctx.configure(B, lr, 5.0)
+                    best, best_ms, _ = ctx.autotune(epochs=2, search_chunking=True)
+                    ctx.upload_x(X0)
+                    if warmup:
+                        ctx.train(warmup, B, ns, lr)
+                    ctx.upload_x(X0)
+                    ctx.synchronize()
+                    t1 = time.perf_counter()
+                    ctx.train(steps, B, ns, lr)
+                    ctx.synchronize()
+                    dt = time.perf_counter() - t1
+                    a, r = ctx.loss(0, 5)
+                    e, n0, n1 = ctx.epoch_units(ns)
+                units = steps * e + n0 + (steps - 1) * n1
+                row[name] = {"ms_per_epoch": dt / steps * 1e3, "units_per_s": units / dt,
+                             "loss": a + r, "tuning": best}
+                log(f"precision: rmat{scale} ns={ns} {name}: {dt / steps * 1e3:.3f} ms/epoch, "
+                    f"loss {a + r:.6e}")
+            row["bf16_speedup"] = row["fp32"]["ms_per_epoch"] / row["bf16"]["ms_per_epoch"]
+            row["bf16_loss_rel_diff"] = (row["bf16"]["loss"] - row["fp32"]["loss"]) / row["fp32"]["loss"]
+            out.append(row)
+        del g
+    return {"form": "precision", "epochs_timed": steps, "d": 128, "batch": 16384, "lr": 0.02,
+            "results": out}
+
+
+# ---- batching form ---------------------------------------------------------
+
+def sbm_pool(count: int, seed: int = 0):
+    """`count` SBM graphs: n log-uniform in [1e3, 2e5], blocks in [4, 32],
+    average degree uniform in [8, 40], 90 % of it inside the block."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        n = int(round(math.exp(rng.uniform(math.log(1e3), math.log(2e5)))))
+        blocks = int(rng.integers(4, 33))
+        deg = float(rng.uniform(8, 40))
+        nb = n / blocks
+        p_in = min(1.0, 0.9 * deg / max(nb - 1, 1))
+        p_out = min(1.0, 0.1 * deg / max(n - nb, 1))
+        specs.append((n, blocks, p_in, p_out, 1000 + i))
+    with ThreadPoolExecutor(8) as ex:
+        return list(ex.map(lambda s: api.Graph.sbm(*s), specs))
+
+
+def batching(epochs: int, count: int = 512, d: int = 64, B: int = 1024, ns: int = 5,
+             budget_gib: float = 96.0) -> dict:
+    t0 = time.perf_counter()
+    graphs = sbm_pool(count)
+    n_tot = sum(g.n for g in graphs)
+    nnz_tot = sum(g.nnz for g in graphs)
+    log(f"batching: {count} SBM graphs, {n_tot} vertices, {nnz_tot} CSR entries "
+        f"({time.perf_counter() - t0:.1f} s)")
+    X0s = [api.init_uniform(g.n, d, -0.5, 0.5, seed=i) for i, g in enumerate(graphs)]
+    units = sum(_units(g, B, ns, epochs) for g in graphs)
+    # tiles for this shape: autotune on the median-sized graph
+    med = sorted(range(count), key=lambda i: graphs[i].n)[count // 2]
+    res = {"form": "batching", "graphs": count, "vertices": n_tot, "csr_entries": nnz_tot,
+           "d": d, "batch": B, "ns": ns, "epochs": epochs, "units": units}
+    for name, gather in (("fp32", api.FP32), ("bf16", api.BF16)):
+        with api.ForceEmbed(graphs[med].n, d, gather=gather) as ctx:
+            ctx.upload_csr(graphs[med])
+            ctx.upload_x(X0s[med])
+            ctx.configure(B, 0.02, 5.0)
+            ctx.set_negatives(api.reference_negatives(graphs[med].n, 2 + -(-graphs[med].n // B), ns)[0])
+            best, best_ms, _ = ctx.autotune(epochs=2, search_chunking=True)
+        log(f"batching {name}: autotuned on n={graphs[med].n}: {best}")
+        with api.Engine(int(budget_gib * 2 ** 30)) as eng:
+            eng.set_tuning(**best)
+            t1 = time.perf_counter()
+            ids = [eng.submit(g, x, B, epochs, ns=ns, gather=gather) for g, x in zip(graphs, X0s)]
+            t_sub = time.perf_counter() - t1
+            peak = eng.stats()["bytes_in_use"]
+            t2 = time.perf_counter()
+            eng.run_graph()
+            wall = time.perf_counter() - t2
+            gt = eng.graph_timing()
+            xs = [eng.collect(i) for i in ids[:4]]
+            assert all(np.isfinite(x).all() for x in xs)
+        res[name] = {"units_per_s": units / (gt["exec_ms"] * 1e-3), "exec_s": gt["exec_ms"] * 1e-3,
+                     "prepare_s": gt["prepare_ms"] * 1e-3, "wall_s": wall, "submit_s": t_sub,
+                     "rounds": gt["rounds"], "hbm_bytes_resident": peak, "tuning": best}
+        log(f"batching {name}: {units / (gt['exec_ms'] * 1e-3) / 1e9:.2f} G units/s "
+            f"(exec {gt['exec_ms'] / 1e3:.2f} s, prepare {gt['prepare_ms'] / 1e3:.2f} s, "
+            f"{gt['rounds']} rounds, {peak / 2 ** 30:.1f} GiB resident)")
+    res["bf16_speedup"] = res["fp32"]["exec_s"] / res["bf16"]["exec_s"]
+    return res
+
+
+# ---- serving form ----------------------------------------------------------
+
+def rmat_job_pool(count: int, seed: int = 1):
+    """R-MAT jobs: generated edge count log-uniform in [2e4, 2e7] (edge factor
+    16, scale from the edge count), d from {32, 64, 128}."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for i in range(count):
+        m = math.exp(rng.uniform(math.log(2e4), math.log(2e7)))
+        scale = max(6, int(round(math.log2(m / 16))))
+        ef = max(1, int(round(m / 2 ** scale)))
+        d = int(rng.choice([32, 64, 128]))
+        specs.append((scale, ef, d, 2000 + i))
+    with ThreadPoolExecutor(8) as ex:
+        graphs = list(ex.map(lambda s: api.Graph.rmat(s[0], s[1], 0.57, 0.19, 0.19, seed=s[3]),
+                             specs))
+    return [(g, s[2]) for g, s in zip(graphs, specs)]
+
+
+def serving(qps_levels=(4.0, 8.0, 16.0), jobs_per_level: int = 48, epochs: int = 50,
+            B: int = 16384, ns: int = 5, budget_gib: float = 32.0,
+            rounds_per_call: int = 4) -> dict:
+    pool = rmat_job_pool(jobs_per_level)
+    X0s = [api.init_uniform(g.n, d, -0.5, 0.5, seed=i) for i, (g, d) in enumerate(pool)]
+    units = [_units(g, min(B, g.n), ns, epochs) for g, _ in pool]
+    log(f"serving: {len(pool)} jobs, {sum(g.nnz for g, _ in pool)} CSR entries, "
+        f"{sum(units) / 1e9:.2f} G units")
+    levels = []
+    for qps in qps_levels:
+        rng = np.random.default_rng(int(qps * 1000))
+        arrivals = np.cumsum(rng.exponential(1.0 / qps, len(pool)))
+        arrivals -= arrivals[0]
+        lat = [None] * len(pool)
+        with api.Engine(int(budget_gib * 2 ** 30)) as eng:
+            nxt = 0
+            running = {}  # job id -> pool index
+            waiting = []  # arrived, not admitted
+            done_units = 0
+            t0 = time.perf_counter()
+            while nxt < len(pool) or waiting or running:
+                now = time.perf_counter() - t0
+                while nxt < len(pool) and arrivals[nxt] <= now:
+                    waiting.append(nxt)
+                    nxt += 1
+                while waiting:
+                    i = waiting[0]
+                    g, d = pool[i]
+                    try:
+                        jid = eng.submit(g, X0s[i], min(B, g.n), epochs, ns=ns)
+                    except api.FeError as e:
+                        if e.status == 4:  # over budget: wait for memory to free up
+                            break
+                        raise
+                    running[jid] = i
+                    waiting.pop(0)
+                if running:
+                    eng.run(rounds_per_call)
+                    for jid in [j for j in running if eng.done(j)]:
+                        i = running.pop(jid)
+                        eng.collect(jid)
+                        lat[i] = time.perf_counter() - t0 - arrivals[i]
+                        done_units += units[i]
+                elif nxt < len(pool):
+                    time.sleep(max(0.0, min(0.01, arrivals[nxt] - (time.perf_counter() - t0))))
+            span = time.perf_counter() - t0
+        lat_ms = np.array(lat) * 1e3
+        levels.append({"offered_qps": qps, "jobs": len(pool), "span_s": span,
+                       "achieved_qps": len(pool) / span,
+                       "p50_ms": float(np.percentile(lat_ms, 50)),
+                       "p99_ms": float(np.percentile(lat_ms, 99)),
+                       "mean_ms": float(lat_ms.mean()),
+                       "units_per_s": done_units / span})
+        log(f"serving qps={qps}: p50 {levels[-1]['p50_ms']:.1f} ms, p99 {levels[-1]['p99_ms']:.1f} ms, "
+            f"{done_units / span / 1e9:.2f} G units/s over {span:.1f} s")
+    return {"form": "serving", "epochs": epochs, "batch": B, "ns": ns, "budget_gib": budget_gib,
+            "levels": levels}

@@ -12,7 +12,6 @@ struct fe_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
   Job job;
-  int **d_epoch_list = nullptr; // {job.epoch} for the tick kernel
   // captured epochs: [0] first epoch of a run, [1] every later epoch
   hipGraph_t graph[2] = {nullptr, nullptr};
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
@@ -83,14 +82,9 @@ extern "C" fe_status fe_create(fe_ctx **out, int device, int64_t n, int d, int g
   fe_status s = set_device(c);
   if (s == FE_OK) s = hip_ok(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
   if (s == FE_OK) s = from_job(c, job_init(c->job, n, d, gather_dtype, quirks));
-  if (s == FE_OK) s = hip_ok(c, hipMalloc((void **)&c->d_epoch_list, sizeof(int *)), "hipMalloc");
-  if (s == FE_OK)
-    s = hip_ok(c, hipMemcpy(c->d_epoch_list, &c->job.epoch, sizeof(int *), hipMemcpyHostToDevice),
-               "hipMemcpy");
   if (s != FE_OK) {
     fe::set_host_error("fe_create: " + c->err);
     job_free(c->job);
-    if (c->d_epoch_list) hipFree(c->d_epoch_list);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
     return s;
@@ -106,7 +100,6 @@ extern "C" void fe_destroy(fe_ctx *c) {
   drop_graph(c);
   for (hipEvent_t ev : c->events) hipEventDestroy(ev);
   job_free(c->job);
-  if (c->d_epoch_list) hipFree(c->d_epoch_list);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
 }
@@ -229,7 +222,7 @@ extern "C" fe_status fe_step(fe_ctx *c, int epoch, int batch) {
   fe_status s = from_job(c, job_prepare(j, c->stream));
   if (s != FE_OK) return s;
   if ((s = check_batch(c, epoch, batch)) != FE_OK) return s;
-  if ((s = from_job(c, job_check(j, launch_set_int(j.epoch, epoch, c->stream), "epoch set"))) != FE_OK)
+  if ((s = from_job(c, job_set_epoch(j, epoch, c->stream))) != FE_OK)
     return s;
   const size_t ev0 = c->events_used;
   if ((s = from_job(c, job_launch_force(j, batch, DESC_SINGLE, c->stream, next_events(c)))) != FE_OK)
@@ -250,7 +243,7 @@ extern "C" fe_status fe_step_delta(fe_ctx *c, int epoch, int batch, float *delta
   fe_status s = from_job(c, job_prepare(j, c->stream));
   if (s != FE_OK) return s;
   if ((s = check_batch(c, epoch, batch)) != FE_OK) return s;
-  if ((s = from_job(c, job_check(j, launch_set_int(j.epoch, epoch, c->stream), "epoch set"))) != FE_OK)
+  if ((s = from_job(c, job_set_epoch(j, epoch, c->stream))) != FE_OK)
     return s;
   if ((s = from_job(c, job_launch_force(j, batch, DESC_DELTA, c->stream, nullptr))) != FE_OK) return s;
   const size_t count = (size_t)rows_of_batch(j, batch) * j.d;
@@ -264,9 +257,9 @@ extern "C" fe_status fe_step_delta(fe_ctx *c, int epoch, int batch, float *delta
 // One epoch of launches on the context stream: the epoch tick, then either
 // the fused chain (each batch hands its rows to the next launch) or, with a
 // single batch, force + apply.
-static fe_status launch_epoch(fe_ctx *c, bool first) {
+static fe_status enqueue_epoch(fe_ctx *c, bool first) {
   Job &j = c->job;
-  fe_status s = from_job(c, job_check(j, launch_tick(c->d_epoch_list, 1, c->stream), "tick"));
+  fe_status s = from_job(c, job_tick(j, c->stream));
   if (s != FE_OK) return s;
   if (!job_can_fuse(j)) {
     if ((s = from_job(c, job_launch_force(j, 0, DESC_SINGLE, c->stream, next_events(c)))) != FE_OK)
@@ -287,7 +280,7 @@ static fe_status capture_epochs(fe_ctx *c) {
                          "hipStreamBeginCapture");
     if (s != FE_OK) return s;
     const size_t ev0 = c->events_used;
-    s = launch_epoch(c, k == 0);
+    s = enqueue_epoch(c, k == 0);
     hipGraph_t g = nullptr;
     hipError_t e = hipStreamEndCapture(c->stream, &g);
     if (s == FE_OK) s = hip_ok(c, e, "hipStreamEndCapture");
@@ -329,7 +322,7 @@ extern "C" fe_status fe_train(fe_ctx *c, int iterations, int batch_size, int ns,
   if (iterations == 0) return FE_OK;
   if ((s = from_job(c, job_reset_counters(j, c->stream))) != FE_OK) return s;
   // the tick at the start of every epoch brings the counter to 0, 1, ...
-  if ((s = from_job(c, job_check(j, launch_set_int(j.epoch, -1, c->stream), "epoch set"))) != FE_OK)
+  if ((s = from_job(c, job_set_epoch(j, -1, c->stream))) != FE_OK)
     return s;
   if (use_graph) {
     if (!c->graph_valid || c->graph_timed != c->timing) {
@@ -351,7 +344,7 @@ extern "C" fe_status fe_train(fe_ctx *c, int iterations, int batch_size, int ns,
   } else {
     c->events_used = 0;
     for (int it = 0; it < iterations; ++it) {
-      if ((s = launch_epoch(c, it == 0)) != FE_OK) return s;
+      if ((s = enqueue_epoch(c, it == 0)) != FE_OK) return s;
       if (c->timing && c->events_used >= 4096) {
         if ((s = hip_ok(c, hipStreamSynchronize(c->stream), "sync")) != FE_OK) return s;
         if ((s = harvest_events(c)) != FE_OK) return s;

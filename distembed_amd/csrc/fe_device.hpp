@@ -32,8 +32,12 @@ struct alignas(16) StepDesc {
   float *P;              // partial sums of heavy-row chunks
   int *counters;         // per split row of the batch: chunks finished
   const int32_t *negs;   // negative lists, list l at negs + l*ns
+  const int32_t *neg_ids; // this step's ns negative ids (list epoch + batch), read
+                          // without waiting for the epoch counter
   const int *epoch;      // device epoch counter of the job
   const float *Sprev;    // new rows of the previous batch, not yet in X
+  uint16_t *Sb;          // BF16 gathers: this batch's new rows rounded, beside S
+  const uint16_t *Sbprev; // ... of the previous batch (read instead of Xs)
   int n_items;           // block items + wave items + light rows
   int n_block_items;
   int n_light;           // light rows (one per lane group), last in items
@@ -53,6 +57,19 @@ struct alignas(16) StepDesc {
   int prev_rows;         // 0: no pending previous batch
   float lr;
   float bound;           // MAX_BOUND of cpp/algo/algo.hpp:65-72
+};
+
+// Epoch counter of one job and the per-batch copies of the negative ids of
+// the current epoch: negcur[b*ns + k] = negs[(epoch + b)*ns + k] (0 where the
+// list does not exist), refreshed whenever the counter changes.
+struct alignas(16) EpochDesc {
+  int *epoch;
+  const int32_t *negs;
+  int32_t *negcur;
+  int batches;
+  int ns;
+  int n_lists;
+  int pad;
 };
 
 // Loss over one job's graph (FP64 reduction).

@@ -373,6 +373,14 @@ static int round_mode(const std::vector<EngineJob *> &run) {
   return (f && h) ? 2 : (h ? 1 : 0);
 }
 
+// Lanes per row shared by every job of the round, or 0 if they differ.
+static int round_g(const std::vector<EngineJob *> &run) {
+  int g = run.empty() ? 0 : run[0]->job.G;
+  for (const EngineJob *ej : run)
+    if (ej->job.G != g) return 0;
+  return g;
+}
+
 static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *host, int &K,
                              size_t &lds, std::vector<EngineJob *> &finished) {
   K = (int)run.size();
@@ -391,6 +399,9 @@ static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *
     sd[k] = j.host_desc[(size_t)j.plan.batches * DESC_SINGLE + b];
     sd[k].epoch = dep + k;
     ep[k] = ej->epoch;
+    // this round's negative ids, straight from the job's lists
+    const int list = ej->epoch + b;
+    sd[k].neg_ids = (j.ns > 0 && list < j.n_lists) ? j.negs + (size_t)list * j.ns : j.negs;
     pf[k + 1] = pf[k] + force_blocks_for(sd[k], ipb);
     pa[k + 1] = pa[k] + j.apply_blocks[b];
     lds = std::max(lds, force_lds_bytes(j.ns, j.d));
@@ -448,7 +459,7 @@ extern "C" fe_status fe_engine_run(fe_engine *e, int rounds, int *running) {
     const int *dpa = dpf + K + 1;
     if ((s = ehip(e, launch_force(dsd, dpf, K, fblocks, e->tun.waves_per_block,
                                   e->tun.items_per_wave, e->tun.unroll, lds, round_mode(run),
-                                  e->stream),
+                                  round_g(run), e->stream),
                   "force launch")) != FE_OK)
       return s;
     if ((s = ehip(e, launch_apply(dsd, dpa, K, ablocks, e->stream), "apply launch")) != FE_OK)
@@ -540,6 +551,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
     int fblocks, ablocks;
     size_t lds;
     int mode;
+    int g_const;
   };
   std::vector<RoundInfo> rounds;
   std::vector<char> host;
@@ -559,7 +571,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
     char *h = host.data() + total;
     build_round(e, run, h, K, lds, finished);
     const int *hpf = reinterpret_cast<const int *>(h + K * sizeof(StepDesc)) + K;
-    rounds.push_back({total, K, hpf[K], hpf[2 * K + 1], lds, round_mode(run)});
+    rounds.push_back({total, K, hpf[K], hpf[2 * K + 1], lds, round_mode(run), round_g(run)});
     for (EngineJob *ej : finished) finish_at.push_back({(int)rounds.size() - 1, ej});
     total += rb;
   }
@@ -589,7 +601,8 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
       const int *dpf = reinterpret_cast<const int *>(dsched + ri.offset + ri.K * sizeof(StepDesc)) + ri.K;
       const int *dpa = dpf + ri.K + 1;
       ls = ehip(e, launch_force(dsd, dpf, ri.K, ri.fblocks, e->tun.waves_per_block,
-                                e->tun.items_per_wave, e->tun.unroll, ri.lds, ri.mode, e->stream),
+                                e->tun.items_per_wave, e->tun.unroll, ri.lds, ri.mode, ri.g_const,
+                                e->stream),
                 "force launch");
       if (ls == FE_OK) ls = ehip(e, launch_apply(dsd, dpa, ri.K, ri.ablocks, e->stream), "apply launch");
     }

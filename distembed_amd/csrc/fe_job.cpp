@@ -112,17 +112,19 @@ fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks) {
 }
 
 void job_free(Job &j) {
-  void *bufs[] = {j.X, j.Xs, j.col, j.items, j.S, j.P, j.counters, j.negs, j.epoch, j.desc,
-                  j.prefix};
+  void *bufs[] = {j.X, j.Xs, j.col, j.items, j.S, j.Sb, j.P, j.counters, j.negs, j.epoch, j.desc,
+                  j.prefix, j.negcur, j.edesc};
   for (void *p : bufs) dev_free(j, p);
   j.X = nullptr;
   j.Xs = nullptr;
-  j.col = j.items = j.negs = nullptr;
+  j.col = j.items = j.negs = j.negcur = nullptr;
+  j.edesc = nullptr;
   j.S = j.P = nullptr;
+  j.Sb = nullptr;
   j.counters = j.epoch = j.prefix = nullptr;
   j.desc = nullptr;
-  j.cap_items = j.cap_S = j.cap_P = j.cap_ctr = j.cap_negs = j.cap_desc = 0;
-  j.cap_prefix = j.cap_col = j.cap_X = j.cap_Xs = j.cap_epoch = 0;
+  j.cap_items = j.cap_S = j.cap_Sb = j.cap_P = j.cap_ctr = j.cap_negs = j.cap_desc = 0;
+  j.cap_prefix = j.cap_negcur = j.cap_edesc = j.cap_col = j.cap_X = j.cap_Xs = j.cap_epoch = 0;
   j.device_bytes = 0;
 }
 
@@ -276,6 +278,7 @@ static StepDesc make_desc(const Job &j, int b, DescKind kind) {
   sd.P = j.P;
   sd.counters = j.counters;
   sd.negs = j.negs;
+  sd.neg_ids = j.negcur ? j.negcur + (size_t)b * j.ns : nullptr;
   sd.epoch = j.epoch;
   sd.n_items = (int)(j.plan.item_off[b + 1] - j.plan.item_off[b]);
   sd.n_block_items = j.plan.block_items[b];
@@ -295,9 +298,11 @@ static StepDesc make_desc(const Job &j, int b, DescKind kind) {
   sd.delta_mode = kind == DESC_DELTA ? 1 : 0;
   if (kind == DESC_FUSED || kind == DESC_FIRST) {
     sd.S = j.S + slot * out_slot(j, b);
+    if (j.Sb) sd.Sb = j.Sb + slot * out_slot(j, b);
     if (kind == DESC_FUSED) {
       const int pb = b > 0 ? b - 1 : j.plan.batches - 1;
       sd.Sprev = j.S + slot * out_slot(j, pb);
+      if (j.Sb) sd.Sbprev = j.Sb + slot * out_slot(j, pb);
       sd.prev_begin = (int)((int64_t)pb * j.plan.B);
       sd.prev_rows = rows_of_batch(j, pb);
     }
@@ -358,6 +363,9 @@ fe_status job_prepare(Job &j, hipStream_t st) {
       return s;
     if ((s = ensure_cap(j, j.S, j.cap_S, (size_t)3 * j.B * j.d, "staging rows")) != FE_OK)
       return s;
+    if (j.dtype == FE_GATHER_BF16 &&
+        (s = ensure_cap(j, j.Sb, j.cap_Sb, (size_t)3 * j.B * j.d, "BF16 staging rows")) != FE_OK)
+      return s;
     if ((s = ensure_cap(j, j.P, j.cap_P, (size_t)std::max(j.plan.max_pslots, 1) * j.d,
                         "partials")) != FE_OK)
       return s;
@@ -379,6 +387,19 @@ fe_status job_prepare(Job &j, hipStream_t st) {
       return s;
     if ((s = ensure_cap(j, j.prefix, j.cap_prefix, (size_t)nb * 4, "block prefixes")) != FE_OK)
       return s;
+    if ((s = ensure_cap(j, j.negcur, j.cap_negcur, (size_t)nb * std::max(j.ns, 1),
+                        "current negative ids")) != FE_OK)
+      return s;
+    if ((s = ensure_cap(j, j.edesc, j.cap_edesc, 1, "epoch state")) != FE_OK) return s;
+    EpochDesc ed{};
+    ed.epoch = j.epoch;
+    ed.negs = j.negs;
+    ed.negcur = j.negcur;
+    ed.batches = nb;
+    ed.ns = j.ns;
+    ed.n_lists = j.n_lists;
+    FE_HIP(j, hipMemcpyAsync(j.edesc, &ed, sizeof ed, hipMemcpyHostToDevice, st),
+           "epoch state copy");
     std::vector<StepDesc> hd((size_t)3 * nb + 1);
     std::vector<int> pf((size_t)nb * 4, 0);
     j.force_blocks.assign(nb, 0);
@@ -408,12 +429,30 @@ fe_status job_prepare(Job &j, hipStream_t st) {
   return FE_OK;
 }
 
+fe_status job_set_epoch(Job &j, int value, hipStream_t st) {
+  if (!j.edesc) {
+    j.err = "epoch state not prepared";
+    return FE_ERR_STATE;
+  }
+  FE_HIP(j, launch_epoch(j.edesc, 1, 1, value, st), "epoch set");
+  return FE_OK;
+}
+
+fe_status job_tick(Job &j, hipStream_t st) {
+  if (!j.edesc) {
+    j.err = "epoch state not prepared";
+    return FE_ERR_STATE;
+  }
+  FE_HIP(j, launch_epoch(j.edesc, 1, 0, 0, st), "epoch tick");
+  return FE_OK;
+}
+
 fe_status job_launch_force(Job &j, int b, DescKind kind, hipStream_t st, hipEvent_t *ev) {
   const size_t lds = force_lds_bytes(j.ns, j.d);
   if (ev) FE_HIP(j, record_event(ev[0], st), "event");
   FE_HIP(j, launch_force(job_desc(j, b, kind), j.prefix + 4 * b, 1, j.force_blocks[b],
                          j.tun.waves_per_block, j.tun.items_per_wave, j.tun.unroll, lds,
-                         j.dtype == FE_GATHER_BF16 ? 1 : 0, st),
+                         j.dtype == FE_GATHER_BF16 ? 1 : 0, j.G, st),
          "force kernel launch");
   if (ev) FE_HIP(j, record_event(ev[1], st), "event");
   return FE_OK;
@@ -485,10 +524,12 @@ int64_t job_bytes_estimate(const Job &j) {
   int64_t b = nd * 4 + (j.dtype == FE_GATHER_BF16 ? nd * 2 : 0);
   b += std::max<int64_t>(j.nnz, 1) * 4;
   b += (int64_t)j.plan.items.size() * 4;
-  b += 3 * (int64_t)j.plan.B * j.d * 4 + (int64_t)std::max(j.plan.max_pslots, 1) * j.d * 4;
+  b += 3 * (int64_t)j.plan.B * j.d * (j.dtype == FE_GATHER_BF16 ? 6 : 4);
+  b += (int64_t)std::max(j.plan.max_pslots, 1) * j.d * 4;
   b += (int64_t)std::max(j.plan.max_splits, 1) * 4;
   b += (int64_t)j.n_lists * j.ns * 4;
   b += (int64_t)(3 * j.plan.batches + 1) * (int64_t)sizeof(StepDesc) + 16LL * j.plan.batches + 4;
+  b += (int64_t)j.plan.batches * std::max(j.ns, 1) * 4 + (int64_t)sizeof(EpochDesc);
   return b;
 }
 

@@ -14,12 +14,14 @@ namespace fe {
 
 // Launch wrappers (fe_kernels.hip).
 // mode: 0 every step gathers FP32, 1 every step gathers BF16, 2 mixed.
+// g_const: lanes per row shared by every step of the launch (0: they differ).
 hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
-                        int wpb, int ipw, int unroll, size_t lds_bytes, int mode, hipStream_t st);
+                        int wpb, int ipw, int unroll, size_t lds_bytes, int mode, int g_const,
+                        hipStream_t st);
 hipError_t launch_apply(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
                         hipStream_t st);
-hipError_t launch_tick(int *const *ctrs, int n, hipStream_t st);
-hipError_t launch_set_int(int *p, int v, hipStream_t st);
+// set != 0: every job's epoch counter := value; else += 1. Refreshes negcur.
+hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, hipStream_t st);
 hipError_t launch_to_bf16(const float *x, uint16_t *xs, int64_t count, hipStream_t st);
 hipError_t launch_loss_edges(const LossDesc &ld, double *block_out, int n_blocks, int ipw,
                              hipStream_t st);
@@ -82,15 +84,18 @@ struct Job {
   int32_t *col = nullptr;
   int32_t *items = nullptr;
   float *S = nullptr;        // 3 staging slots of B x d
+  uint16_t *Sb = nullptr;    // BF16 gathers: the same slots rounded to BF16
   float *P = nullptr;
   int *counters = nullptr;
   int32_t *negs = nullptr;
+  int32_t *negcur = nullptr;   // batches x ns: ids of the current epoch's lists
+  EpochDesc *edesc = nullptr;  // one entry: this job's epoch state
   int *epoch = nullptr;
   StepDesc *desc = nullptr;  // 3 * batches + 1: FUSED[b], SINGLE[b], DELTA[b], FIRST
   int *prefix = nullptr;     // per batch: {0, force blocks}, {0, apply blocks}
   std::vector<StepDesc> host_desc;
-  size_t cap_items = 0, cap_S = 0, cap_P = 0, cap_ctr = 0, cap_negs = 0, cap_desc = 0;
-  size_t cap_prefix = 0, cap_col = 0, cap_X = 0, cap_Xs = 0, cap_epoch = 0;
+  size_t cap_items = 0, cap_S = 0, cap_Sb = 0, cap_P = 0, cap_ctr = 0, cap_negs = 0, cap_desc = 0;
+  size_t cap_prefix = 0, cap_negcur = 0, cap_edesc = 0, cap_col = 0, cap_X = 0, cap_Xs = 0, cap_epoch = 0;
   bool plan_dirty = true;  // plan must be rebuilt (batch size, chunking, CSR)
   bool steps_dirty = true; // descriptors must be rebuilt
   int64_t device_bytes = 0;
@@ -109,6 +114,10 @@ fe_status job_configure(Job &j, int B, float lr, float bound);
 fe_status job_set_tuning(Job &j, const Tuning &t);
 // Builds the plan / descriptors if dirty and uploads them.
 fe_status job_prepare(Job &j, hipStream_t st);
+// Epoch counter := value (and the current negative ids), on stream st.
+fe_status job_set_epoch(Job &j, int value, hipStream_t st);
+// Epoch counter += 1 (start of an epoch; may be captured in a graph).
+fe_status job_tick(Job &j, hipStream_t st);
 // Zeroes the split counters (outside any graph capture).
 fe_status job_reset_counters(Job &j, hipStream_t st);
 const StepDesc *job_desc(const Job &j, int b, DescKind kind);

@@ -174,31 +174,52 @@ template <> struct RowBuf<1> {
   u4v w;
 };
 
+// Where a lane reads gathered rows from: its 8-component slice of X (or of
+// the BF16 shadow) and of the previous batch's staging slot. A row of the
+// previous batch is read from the slot (its new values are not in X yet);
+// the source is selected per lane without a branch and the address is one
+// 32 x 32 -> 64-bit multiply-add, so the U loads of a round issue back to
+// back.
+struct RowSrc {
+  const char *xq;      // X or Xs, plus this lane's slice offset
+  const char *sq;      // Sprev or Sbprev, plus the slice offset
+  uint32_t row_bytes;  // bytes per gathered row
+  uint32_t prev_begin;
+  uint32_t prev_rows;
+  bool bf;             // gathers read BF16
+};
+
 template <int MODE>
-__device__ __forceinline__ void load_buf(const StepDesc &sd, int c, int q, RowBuf<MODE> &b) {
-  if constexpr (MODE == 1) {
-    const unsigned off = (unsigned)(c - sd.prev_begin);
-    if (off < (unsigned)sd.prev_rows) {
-      float o[8];
-      load8_f32(sd.Sprev + (size_t)off * sd.D + 8 * q, o);
-      b.w.x = bf16_bits(o[0]) | (bf16_bits(o[1]) << 16);
-      b.w.y = bf16_bits(o[2]) | (bf16_bits(o[3]) << 16);
-      b.w.z = bf16_bits(o[4]) | (bf16_bits(o[5]) << 16);
-      b.w.w = bf16_bits(o[6]) | (bf16_bits(o[7]) << 16);
-    } else {
-      b.w = *(gcu4 *)(sd.Xs + (size_t)c * sd.D + 8 * q);
-    }
+__device__ __forceinline__ RowSrc make_src(const StepDesc &sd, int q) {
+  RowSrc r;
+  r.bf = use_bf16<MODE>(sd);
+  if (r.bf) {
+    r.xq = reinterpret_cast<const char *>(sd.Xs + 8 * q);
+    r.sq = reinterpret_cast<const char *>(sd.Sbprev + 8 * q);
+    r.row_bytes = (uint32_t)sd.D * 2u;
   } else {
-    load_row<MODE>(sd, c, q, b.f);
+    r.xq = reinterpret_cast<const char *>(sd.X + 8 * q);
+    r.sq = reinterpret_cast<const char *>(sd.Sprev + 8 * q);
+    r.row_bytes = (uint32_t)sd.D * 4u;
   }
+  r.prev_begin = (uint32_t)sd.prev_begin;
+  r.prev_rows = (uint32_t)sd.prev_rows;
+  return r;
 }
 
-template <int MODE> __device__ __forceinline__ void zero_buf(RowBuf<MODE> &b) {
+template <int MODE>
+__device__ __forceinline__ void load_buf(const RowSrc &src, int c, RowBuf<MODE> &b) {
+  const uint32_t off = (uint32_t)c - src.prev_begin;
+  const bool prev = off < src.prev_rows;
+  const uint32_t idx = prev ? off : (uint32_t)c;
+  const char *p = (prev ? src.sq : src.xq) + (uint64_t)idx * src.row_bytes;
   if constexpr (MODE == 1) {
-    b.w = u4v{0u, 0u, 0u, 0u};
+    b.w = *(gcu4 *)p;
+  } else if constexpr (MODE == 0) {
+    load8_f32(reinterpret_cast<const float *>(p), b.f);
   } else {
-#pragma unroll
-    for (int k = 0; k < 8; ++k) b.f[k] = 0.f;
+    if (src.bf) load8_bf16(reinterpret_cast<const uint16_t *>(p), b.f);
+    else load8_f32(reinterpret_cast<const float *>(p), b.f);
   }
 }
 
@@ -215,19 +236,51 @@ __device__ __forceinline__ void unpack_buf(const RowBuf<MODE> &b, float (&o)[8])
   }
 }
 
-// Stage the `ns` negative rows of list `list` as FP32 in LDS.
-template <int MODE>
-__device__ __forceinline__ void stage_negatives(const StepDesc &sd, int list, float *lds) {
-  const int G = sd.G;
-  const int total8 = sd.ns * G; // 8-component pieces
-  const int32_t *nl = sd.negs + (size_t)list * sd.ns;
-  for (int p = threadIdx.x; p < total8; p += blockDim.x) {
-    const int k = p / G;
-    const int q = p - k * G;
-    float v[8];
-    load_row<MODE>(sd, gld_int(nl + k), q, v);
-    store8_lds(lds + (size_t)k * sd.D + 8 * q, v);
+// acc += lr * clamp(v * d1, +-bound) for one interaction whose squared
+// distance s the group has reduced. d1 = -2/(1+s) (attractive,
+// cpp/algo/algo.hpp:573) or 2/((s+1e-6)(1+s)) (repulsive, :641).
+// While every s in the wave is finite, so is every v*d1: d1 takes the
+// hardware reciprocal (1 ulp) and the clamp is one med3, which for finite
+// values equals scale(). A non-finite s anywhere in the wave takes the exact
+// expressions instead, whose clamp passes a NaN through as scale() does.
+template <bool ATTR>
+__device__ __forceinline__ void add_force(const float (&v)[8], float s, float lr, float bound,
+                                          float (&acc)[8]) {
+  if (__builtin_amdgcn_ballot_w64(!__builtin_isfinite(s)) == 0) {
+    const float d1 = ATTR ? -2.0f * __builtin_amdgcn_rcpf(1.0f + s)
+                          : 2.0f * __builtin_amdgcn_rcpf((s + 1e-6f) * (1.0f + s));
+#pragma unroll
+    for (int k = 0; k < 8; ++k)
+      acc[k] = fmaf(lr, __builtin_amdgcn_fmed3f(v[k] * d1, -bound, bound), acc[k]);
+  } else {
+    const float d1 = ATTR ? -2.0f / (1.0f + s) : 2.0f / ((s + 1e-6f) * (1.0f + s));
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = fmaf(lr, clampf(v[k] * d1, bound), acc[k]);
   }
+}
+
+// One gathered slice against the row's own slice xd.
+template <bool ATTR>
+__device__ __forceinline__ void interact(const float (&xd)[8], const float (&xc)[8], int G,
+                                         float lr, float bound, float (&acc)[8]) {
+  float v[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) v[k] = xd[k] - xc[k];
+  float p = v[0] * v[0];
+#pragma unroll
+  for (int k = 1; k < 8; ++k) p = fmaf(v[k], v[k], p);
+  add_force<ATTR>(v, group_sum(p, G), lr, bound, acc);
+}
+
+// Staging piece p (negative p / G, slice p % G) of the step's negative rows,
+// as FP32 in LDS; id is that negative's row.
+template <int MODE>
+__device__ __forceinline__ void stage_piece(const StepDesc &sd, int p, int id, int G, float *lds) {
+  const int k = p / G;
+  const int q = p - k * G;
+  float v[8];
+  load_row<MODE>(sd, id, q, v);
+  store8_lds(lds + (size_t)k * sd.D + 8 * q, v);
 }
 
 // Repulsive forces of the staged negatives on x_r, spread over the groups.
@@ -240,21 +293,19 @@ __device__ __forceinline__ void repulse(const StepDesc &sd, const float *lds,
   for (int u = 0; u < rounds; ++u) {
     const int k = u * NG + g;
     const bool ok = k < ns;
-    const float *pn = lds + (ok ? k : 0) * sd.D + 8 * q;
-    const float4 a = *reinterpret_cast<const float4 *>(pn);
-    const float4 b = *reinterpret_cast<const float4 *>(pn + 4);
-    const float xn[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-    float v[8];
-    float p = 0.f;
+    // a group without a negative this round meets the row itself: v = 0
+    float xn[8];
+    if (ok) {
+      const float *pn = lds + k * sd.D + 8 * q;
+      const float4 a = *reinterpret_cast<const float4 *>(pn);
+      const float4 b = *reinterpret_cast<const float4 *>(pn + 4);
+      xn[0] = a.x; xn[1] = a.y; xn[2] = a.z; xn[3] = a.w;
+      xn[4] = b.x; xn[5] = b.y; xn[6] = b.z; xn[7] = b.w;
+    } else {
 #pragma unroll
-    for (int k2 = 0; k2 < 8; ++k2) {
-      v[k2] = ok ? xr[k2] - xn[k2] : 0.f;
-      p = fmaf(v[k2], v[k2], p);
+      for (int k2 = 0; k2 < 8; ++k2) xn[k2] = xr[k2];
     }
-    const float s = group_sum(p, G);
-    const float d1 = 2.0f / ((s + 1e-6f) * (1.0f + s));
-#pragma unroll
-    for (int k2 = 0; k2 < 8; ++k2) acc[k2] = fmaf(lr, clampf(v[k2] * d1, bound), acc[k2]);
+    interact<false>(xr, xn, G, lr, bound, acc);
   }
 }
 
@@ -291,44 +342,36 @@ __device__ __forceinline__ void sum_partials(const StepDesc &sd, int p0, int k, 
   }
 }
 
-// Attractive forces of edges [e0, e0+cnt) of the row on the per-group
+// Attractive forces of edges [e0, e0+cnt) of row r on the per-group
 // accumulators: group g takes edges g, g+NG, ... in order, U rows in flight.
+// A slot past the end, or holding a column the quirk skips (c >= c_limit),
+// reads row r itself, whose distance to itself is exactly zero: no force.
 template <int U, int MODE>
-__device__ __forceinline__ void gather_edges(const StepDesc &sd, const float (&xd)[8], int e0,
-                                             int cnt, int NG, int g, int q, int lane,
-                                             float (&acc)[8]) {
-  const int G = sd.G;
+__device__ __forceinline__ void gather_edges(const StepDesc &sd, const RowSrc &src,
+                                             const float (&xd)[8], int r, int e0, int cnt,
+                                             int G, int NG, int g, int lane, float (&acc)[8]) {
   const float lr = sd.lr, bound = sd.bound;
   const int c_limit = sd.c_limit;
   for (int eb = 0; eb < cnt; eb += 64) {
     const int m = min(64, cnt - eb);
-    const int cl = lane < m ? gld_int(sd.col + e0 + eb + lane) : 0;
+    int cl = lane < m ? gld_int(sd.col + e0 + eb + lane) : r;
+    cl = cl < c_limit ? cl : r;
     const int rounds = (m + NG - 1) / NG;
     for (int u0 = 0; u0 < rounds; u0 += U) {
       RowBuf<MODE> xb[U];
-      bool ok[U];
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
         const int k = (u0 + uu) * NG + g;
         const int c = __shfl(cl, k & 63);
-        ok[uu] = (k < m) && (c < c_limit);
-        if (ok[uu]) load_buf<MODE>(sd, c, q, xb[uu]);
-        else zero_buf<MODE>(xb[uu]);
+        load_buf<MODE>(src, k < m ? c : r, xb[uu]);
       }
+      // keep the U loads ahead of every use of their results
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
-        float xc[8], v[8];
+        float xc[8];
         unpack_buf<MODE>(xb[uu], xc);
-        float p = 0.f;
-#pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2) {
-          v[k2] = ok[uu] ? xd[k2] - xc[k2] : 0.f;
-          p = fmaf(v[k2], v[k2], p);
-        }
-        const float s = group_sum(p, G);
-        const float d1 = -2.0f / (1.0f + s);
-#pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2) acc[k2] = fmaf(lr, clampf(v[k2] * d1, bound), acc[k2]);
+        interact<true>(xd, xc, G, lr, bound, acc);
       }
     }
   }
@@ -338,16 +381,42 @@ __device__ __forceinline__ void gather_edges(const StepDesc &sd, const float (&x
 // when gathers read BF16 (no extra traffic), so a row meeting itself as a
 // negative or through a self-loop feels exactly zero force, as in the
 // reference; the update still goes to the FP32 master.
+// The row's own slice for the distances: the master row, rounded as the
+// shadow holds it when gathers read BF16 (no extra traffic), so a row meeting
+// itself as a negative or through a self-loop feels exactly zero force, as
+// in the reference; the update still goes to the FP32 master.
 template <int MODE>
-__device__ __forceinline__ void load_own(const StepDesc &sd, int r, int q, float (&xr)[8],
-                                         float (&xd)[8]) {
-  load8_f32(sd.X + (size_t)r * sd.D + 8 * q, xr);
+__device__ __forceinline__ void own_slice(const StepDesc &sd, const float (&xr)[8],
+                                          float (&xd)[8]) {
 #pragma unroll
   for (int k = 0; k < 8; ++k)
     xd[k] = use_bf16<MODE>(sd) ? __uint_as_float(bf16_bits(xr[k]) << 16) : xr[k];
 }
 
+template <int MODE>
+__device__ __forceinline__ void load_own(const StepDesc &sd, int r, int q, float (&xr)[8],
+                                         float (&xd)[8]) {
+  load8_f32(sd.X + (size_t)r * sd.D + 8 * q, xr);
+  own_slice<MODE>(sd, xr, xd);
+}
+
+// Row t of the batch into S (new row or delta) and, when the next launch
+// gathers BF16, its rounded copy into Sb.
+template <int MODE>
+__device__ __forceinline__ void write_row(const StepDesc &sd, int t, int q, const float (&o)[8]) {
+  store8_f32(sd.S + (size_t)t * sd.D + 8 * q, o);
+  if (MODE != 0 && use_bf16<MODE>(sd) && sd.Sb != nullptr) {
+    u4v w;
+    w.x = bf16_bits(o[0]) | (bf16_bits(o[1]) << 16);
+    w.y = bf16_bits(o[2]) | (bf16_bits(o[3]) << 16);
+    w.z = bf16_bits(o[4]) | (bf16_bits(o[5]) << 16);
+    w.w = bf16_bits(o[6]) | (bf16_bits(o[7]) << 16);
+    *(__attribute__((address_space(1))) u4v *)(sd.Sb + (size_t)t * sd.D + 8 * q) = w;
+  }
+}
+
 // Repulsion (first `rep` rows), group combine, new row (or delta) to S.
+template <int MODE>
 __device__ __forceinline__ void finish_row(const StepDesc &sd, const float *lds_neg, bool repel,
                                            int t, const float (&xr)[8], const float (&xd)[8],
                                            float (&acc)[8], int G, int NG, int g, int q) {
@@ -357,87 +426,73 @@ __device__ __forceinline__ void finish_row(const StepDesc &sd, const float *lds_
     float o[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
-    store8_f32(sd.S + (size_t)t * sd.D + 8 * q, o);
+    write_row<MODE>(sd, t, q, o);
   }
 }
 
-// Light rows, one per lane group: group g takes entry pack*NG + g of the
-// batch's light rows (sorted by degree), gathers its edges in CSR order (U
-// in flight), repels against every staged negative and writes its row. The
-// loops run to the pack's largest degree, so control flow stays uniform.
+// Light rows, one per lane group: the group takes light entry ia (row r,
+// edges [e0, e0+deg)), gathers its edges in CSR order (U in flight), repels
+// against every staged negative and writes its row. The loops run to the
+// pack's largest degree kmax, so control flow stays uniform; slots past a
+// row's degree read the row itself (no force).
 template <int U, int MODE>
-__device__ __forceinline__ void light_pack(const StepDesc &sd, const int32_t *lrows, int n_light,
-                                           int pack, const float *lds_neg, bool has_neg, int rep,
-                                           int NG, int g, int q, int lane) {
-  const int G = sd.G;
-  const int idx = pack * NG + g;
-  const bool valid = idx < n_light;
-  const u4v ia = gld_u4(lrows + (size_t)kItemInts * (valid ? idx : pack * NG));
+__device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src, u4v ia,
+                                           int kmax, bool valid, const float (&xr)[8],
+                                           const float *lds_neg, bool has_neg, int rep, int G,
+                                           int g, int q) {
   const int r = (int)ia.x, e0 = (int)ia.y, deg = valid ? (int)ia.z : 0;
-  const int kmax = __builtin_amdgcn_readfirstlane((int)gld_u4(lrows + (size_t)kItemInts * pack * NG + 4).x);
+  kmax = __builtin_amdgcn_readfirstlane(kmax);
   const float lr = sd.lr, bound = sd.bound;
   const int c_limit = sd.c_limit;
-  float xr[8], xd[8], acc[8];
-  load_own<MODE>(sd, r, q, xr, xd);
+  float xd[8], acc[8];
+  own_slice<MODE>(sd, xr, xd);
 #pragma unroll
   for (int k = 0; k < 8; ++k) acc[k] = 0.f;
   for (int kb = 0; kb < kmax; kb += G) {
-    const int cl = (kb + q < deg) ? gld_int(sd.col + e0 + kb + q) : 0;
+    int cl = (kb + q < deg) ? gld_int(sd.col + e0 + kb + q) : r;
+    cl = cl < c_limit ? cl : r;
     const int kend = min(kmax, kb + G);
     for (int k0 = kb; k0 < kend; k0 += U) {
       RowBuf<MODE> xb[U];
-      bool ok[U];
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
         const int k = k0 + uu;
         const int c = __shfl(cl, g * G + ((k - kb) & (G - 1)));
-        ok[uu] = (k < kend) && (k < deg) && (c < c_limit);
-        if (ok[uu]) load_buf<MODE>(sd, c, q, xb[uu]);
-        else zero_buf<MODE>(xb[uu]);
+        load_buf<MODE>(src, k < kend ? c : r, xb[uu]);
       }
+      __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
-        float xc[8], v[8];
+        float xc[8];
         unpack_buf<MODE>(xb[uu], xc);
-        float p = 0.f;
-#pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2) {
-          v[k2] = ok[uu] ? xd[k2] - xc[k2] : 0.f;
-          p = fmaf(v[k2], v[k2], p);
-        }
-        const float s = group_sum(p, G);
-        const float d1 = -2.0f / (1.0f + s);
-#pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2) acc[k2] = fmaf(lr, clampf(v[k2] * d1, bound), acc[k2]);
+        interact<true>(xd, xc, G, lr, bound, acc);
       }
     }
   }
   const int t = r - sd.row_begin;
   if (has_neg) {
+    // rows past the repulsive count (or an empty slot) meet themselves: v = 0
     const bool repel = valid && t < rep;
     for (int n = 0; n < sd.ns; ++n) {
-      const float *pn = lds_neg + (size_t)n * sd.D + 8 * q;
-      const float4 a = *reinterpret_cast<const float4 *>(pn);
-      const float4 b = *reinterpret_cast<const float4 *>(pn + 4);
-      const float xn[8] = {a.x, a.y, a.z, a.w, b.x, b.y, b.z, b.w};
-      float v[8];
-      float p = 0.f;
+      float xn[8];
+      if (repel) {
+        const float *pn = lds_neg + (size_t)n * sd.D + 8 * q;
+        const float4 a = *reinterpret_cast<const float4 *>(pn);
+        const float4 b = *reinterpret_cast<const float4 *>(pn + 4);
+        xn[0] = a.x; xn[1] = a.y; xn[2] = a.z; xn[3] = a.w;
+        xn[4] = b.x; xn[5] = b.y; xn[6] = b.z; xn[7] = b.w;
+      } else {
 #pragma unroll
-      for (int k2 = 0; k2 < 8; ++k2) {
-        v[k2] = repel ? xd[k2] - xn[k2] : 0.f;
-        p = fmaf(v[k2], v[k2], p);
+        for (int k2 = 0; k2 < 8; ++k2) xn[k2] = xd[k2];
       }
-      const float s = group_sum(p, G);
-      const float d1 = 2.0f / ((s + 1e-6f) * (1.0f + s));
-#pragma unroll
-      for (int k2 = 0; k2 < 8; ++k2) acc[k2] = fmaf(lr, clampf(v[k2] * d1, bound), acc[k2]);
+      interact<false>(xd, xn, G, lr, bound, acc);
     }
   }
   if (valid) {
     float o[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
-    store8_f32(sd.S + (size_t)t * sd.D + 8 * q, o);
+    write_row<MODE>(sd, t, q, o);
   }
 }
 
@@ -447,10 +502,12 @@ __device__ __forceinline__ void light_pack(const StepDesc &sd, const int32_t *lr
 // and then add in sub-range order through LDS. A chunk publishes its
 // partial (device-coherent stores) and bumps the row's counter; the chunk
 // that counts last adds all partials of the row in index order. The
-// remaining blocks' waves walk the light rows with a grid stride, one row
-// per wave at a time. Every block then copies its share of the previous
-// batch's rows from staging into X.
-template <int U, int MODE>
+// remaining blocks' waves walk the light-row packs and then the medium rows
+// with a grid stride. Before any of that, every block copies its share of
+// the previous batch's rows from staging into X, while its first work item
+// and the negative rows are being loaded. GT > 0: every step of the launch
+// has G = GT lanes per row (a compile-time constant); 0: read per step.
+template <int U, int MODE, int GT>
 __global__ __launch_bounds__(512) void force_kernel(
     const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
     int n_steps, int ipw) {
@@ -460,14 +517,8 @@ __global__ __launch_bounds__(512) void force_kernel(
   const StepDesc &sd = steps[si];
   const int lb = b - blk_prefix[si];
   const int nb = blk_prefix[si + 1] - blk_prefix[si];
-  const int epoch = *sd.epoch;
-  const int rep = epoch == 0 ? sd.rep0 : sd.rep1;
-  const int list = epoch + sd.batch;
-  const bool has_neg = sd.ns > 0 && list < sd.n_lists;
-  if (has_neg) stage_negatives<MODE>(sd, list, lds);
-  __syncthreads();
-
-  const int D = sd.D, G = sd.G;
+  const int D = sd.D;
+  const int G = GT > 0 ? GT : sd.G;
   const int NG = 64 / G;
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -475,85 +526,36 @@ __global__ __launch_bounds__(512) void force_kernel(
   const int g = lane / G;
   const int q = lane & (G - 1);
 
-  if (lb < sd.n_block_items) {
-    const int32_t *item = sd.items + (size_t)kItemInts * lb;
-    const u4v ia = gld_u4(item);
-    const int r = (int)ia.x, e0 = (int)ia.y, cnt = (int)ia.z, pslot = (int)ia.w;
-    const int t = r - sd.row_begin;
-    float xr[8], xd[8], acc[8];
-    load_own<MODE>(sd, r, q, xr, xd);
-    float *lds_sub = lds + (has_neg ? sd.ns * D : 0);
-    for (int s = wave; s < kSub; s += wpb) {
-      const int lo = (int)((int64_t)cnt * s / kSub), hi = (int)((int64_t)cnt * (s + 1) / kSub);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
-      gather_edges<U, MODE>(sd, xd, e0 + lo, hi - lo, NG, g, q, lane, acc);
-      combine_groups(acc, G);
-      if (g == 0) store8_lds(lds_sub + (size_t)s * D + 8 * q, acc);
-    }
-    __syncthreads();
-    if (wave == 0) {
-      // group 0 holds the sum of the sub-ranges, in sub-range order
-#pragma unroll
-      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
-      if (g == 0) {
-        for (int s = 0; s < kSub; ++s) {
-          const float *p = lds_sub + (size_t)s * D + 8 * q;
-#pragma unroll
-          for (int k = 0; k < 8; ++k) acc[k] += p[k];
-        }
-      }
-      bool finish = pslot < 0;
-      if (!finish) {
-        const u4v ib = gld_u4(item + 4);
-        const int sid = (int)ib.x, p0 = (int)ib.y, kparts = (int)ib.z;
-        if (g == 0) store8_coherent(sd.P + (size_t)pslot * D + 8 * q, acc);
-        wait_stores();
-        int old = 0;
-        if (lane == 0)
-          old = __hip_atomic_fetch_add((gint *)(sd.counters + sid), 1, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-        old = __shfl(old, 0);
-        if (old == kparts - 1) {
-#pragma unroll
-          for (int k = 0; k < 8; ++k) acc[k] = 0.f;
-          sum_partials<U>(sd, p0, kparts, NG, g, q, acc);
-          if (lane == 0)
-            __hip_atomic_store((gint *)(sd.counters + sid), 0, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
-          finish = true;
-        }
-      }
-      if (finish) finish_row(sd, lds, has_neg && t < rep, t, xr, xd, acc, G, NG, g, q);
-    }
-  } else {
-    // wave work: packs of light rows (one row per lane group), then the
-    // medium rows (one row per wave), walked with a grid stride
-    const int n_light = sd.n_light;
-    const int n_wave_items = sd.n_items - sd.n_block_items - n_light;
-    const int32_t *witems = sd.items + (size_t)kItemInts * sd.n_block_items;
-    const int32_t *lrows = witems + (size_t)kItemInts * n_wave_items;
-    const int n_packs = (n_light + NG - 1) / NG;
-    const int W = (nb - sd.n_block_items) * wpb;
-    int u = (lb - sd.n_block_items) * wpb + wave;
-    for (; u < n_packs; u += W)
-      light_pack<U, MODE>(sd, lrows, n_light, u, lds, has_neg, rep, NG, g, q, lane);
-    for (int it = u - n_packs; it < n_wave_items; it += W) {
-      const u4v ia = gld_u4(witems + (size_t)kItemInts * it);
-      const int r = (int)ia.x, e0 = (int)ia.y, cnt = (int)ia.z;
-      const int t = r - sd.row_begin;
-      float xr[8], xd[8], acc[8];
-      load_own<MODE>(sd, r, q, xr, xd);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
-      gather_edges<U, MODE>(sd, xd, e0, cnt, NG, g, q, lane, acc);
-      finish_row(sd, lds, has_neg && t < rep, t, xr, xd, acc, G, NG, g, q);
-    }
+  // this wave's first work item, in flight during the copy and the staging
+  const int nbi = sd.n_block_items;
+  const int n_light = sd.n_light;
+  const int n_wave_items = sd.n_items - nbi - n_light;
+  const int32_t *witems = sd.items + (size_t)kItemInts * nbi;
+  const int32_t *lrows = witems + (size_t)kItemInts * n_wave_items;
+  const int n_packs = (n_light + NG - 1) / NG;
+  int u = (lb - nbi) * wpb + wave;
+  u4v pre0 = {0u, 0u, 0u, 0u}, pre1 = {0u, 0u, 0u, 0u};
+  if (lb < nbi) {
+    pre0 = gld_u4(sd.items + (size_t)kItemInts * lb);
+    pre1 = gld_u4(sd.items + (size_t)kItemInts * lb + 4);
+  } else if (u < n_packs) {
+    const int idx = u * NG + g;
+    pre0 = gld_u4(lrows + (size_t)kItemInts * (idx < n_light ? idx : u * NG));
+    pre1 = gld_u4(lrows + (size_t)kItemInts * u * NG + 4);
+  } else if (u - n_packs < n_wave_items) {
+    pre0 = gld_u4(witems + (size_t)kItemInts * (u - n_packs));
   }
-  (void)ipw;
 
-  // copy phase: the previous batch's new rows into X (and the shadow); no
-  // thread of this launch reads those rows of X (load_row redirects them)
+  // this step's negative ids, one per staging piece (first blockDim pieces)
+  const int total8 = sd.ns * G;
+  const int pz = threadIdx.x;
+  int nid = 0;
+  if (pz < total8) nid = gld_int(sd.neg_ids + pz / G);
+  const int epoch = *sd.epoch;
+
+  // copy phase: the previous batch's new rows into X (and the shadow). No
+  // thread of this launch reads those rows of X: gathers of them are
+  // redirected to the staging slot.
   if (sd.prev_rows > 0) {
     const int64_t total4 = (int64_t)sd.prev_rows * D / 4;
     const int64_t per = (total4 + nb - 1) / nb;
@@ -573,6 +575,102 @@ __global__ __launch_bounds__(512) void force_kernel(
       }
     }
   }
+
+  // the own row of this wave's first unit, in flight with the negative rows
+  const bool unit0 = lb < nbi || u < n_packs + n_wave_items;
+  float xr0[8];
+  if (unit0) load8_f32(sd.X + (size_t)pre0.x * D + 8 * q, xr0);
+  const int rep = epoch == 0 ? sd.rep0 : sd.rep1;
+  const bool has_neg = sd.ns > 0 && epoch + sd.batch < sd.n_lists;
+  if (has_neg) {
+    if (pz < total8) stage_piece<MODE>(sd, pz, nid, G, lds);
+    for (int p = pz + blockDim.x; p < total8; p += blockDim.x)
+      stage_piece<MODE>(sd, p, gld_int(sd.neg_ids + p / G), G, lds);
+  }
+  __syncthreads();
+  const RowSrc src = make_src<MODE>(sd, q);
+
+  if (lb < nbi) {
+    const int r = (int)pre0.x, e0 = (int)pre0.y, cnt = (int)pre0.z, pslot = (int)pre0.w;
+    const int t = r - sd.row_begin;
+    float xd[8], acc[8];
+    own_slice<MODE>(sd, xr0, xd);
+    float *lds_sub = lds + (has_neg ? sd.ns * D : 0);
+    for (int s = wave; s < kSub; s += wpb) {
+      const int lo = (int)((int64_t)cnt * s / kSub), hi = (int)((int64_t)cnt * (s + 1) / kSub);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+      gather_edges<U, MODE>(sd, src, xd, r, e0 + lo, hi - lo, G, NG, g, lane, acc);
+      combine_groups(acc, G);
+      if (g == 0) store8_lds(lds_sub + (size_t)s * D + 8 * q, acc);
+    }
+    __syncthreads();
+    if (wave == 0) {
+      // group 0 holds the sum of the sub-ranges, in sub-range order
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+      if (g == 0) {
+        for (int s = 0; s < kSub; ++s) {
+          const float *p = lds_sub + (size_t)s * D + 8 * q;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) acc[k] += p[k];
+        }
+      }
+      bool finish = pslot < 0;
+      if (!finish) {
+        const int sid = (int)pre1.x, p0 = (int)pre1.y, kparts = (int)pre1.z;
+        if (g == 0) store8_coherent(sd.P + (size_t)pslot * D + 8 * q, acc);
+        wait_stores();
+        int old = 0;
+        if (lane == 0)
+          old = __hip_atomic_fetch_add((gint *)(sd.counters + sid), 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        old = __shfl(old, 0);
+        if (old == kparts - 1) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+          sum_partials<U>(sd, p0, kparts, NG, g, q, acc);
+          if (lane == 0)
+            __hip_atomic_store((gint *)(sd.counters + sid), 0, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+          finish = true;
+        }
+      }
+      if (finish) finish_row<MODE>(sd, lds, has_neg && t < rep, t, xr0, xd, acc, G, NG, g, q);
+    }
+  } else {
+    // wave work: packs of light rows (one row per lane group), then the
+    // medium rows (one row per wave), walked with a grid stride
+    const int W = (nb - nbi) * wpb;
+    bool have = true; // pre0/pre1 and xr0 hold this wave's current unit
+    for (; u < n_packs; u += W) {
+      const int idx = u * NG + g;
+      if (!have) {
+        pre0 = gld_u4(lrows + (size_t)kItemInts * (idx < n_light ? idx : u * NG));
+        pre1 = gld_u4(lrows + (size_t)kItemInts * u * NG + 4);
+        load8_f32(sd.X + (size_t)pre0.x * D + 8 * q, xr0);
+      }
+      have = false;
+      light_pack<U, MODE>(sd, src, pre0, (int)pre1.x, idx < n_light, xr0, lds, has_neg, rep, G,
+                          g, q);
+    }
+    for (int it = u - n_packs; it < n_wave_items; it += W) {
+      if (!have) {
+        pre0 = gld_u4(witems + (size_t)kItemInts * it);
+        load8_f32(sd.X + (size_t)pre0.x * D + 8 * q, xr0);
+      }
+      have = false;
+      const int r = (int)pre0.x, e0 = (int)pre0.y, cnt = (int)pre0.z;
+      const int t = r - sd.row_begin;
+      float xd[8], acc[8];
+      own_slice<MODE>(sd, xr0, xd);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+      gather_edges<U, MODE>(sd, src, xd, r, e0, cnt, G, NG, g, lane, acc);
+      finish_row<MODE>(sd, lds, has_neg && t < rep, t, xr0, xd, acc, G, NG, g, q);
+    }
+  }
+  (void)ipw;
 }
 
 // X[batch] = S (cpp/algo/algo.hpp:651-666 with the add already folded into
@@ -602,13 +700,23 @@ __global__ __launch_bounds__(256) void apply_kernel(
   }
 }
 
-__global__ void set_int_kernel(int *p, int v) {
-  if (blockIdx.x == 0 && threadIdx.x == 0) *p = v;
-}
-
-// Advances epoch counters (one per job) at the start of an epoch.
-__global__ void tick_kernel(int *const *ctrs, int n) {
-  for (int i = threadIdx.x; i < n; i += blockDim.x) *ctrs[i] = *ctrs[i] + 1;
+// Epoch counters of `n` jobs (one block each): set to `value` or advanced by
+// one, then each job's current negative ids negcur[b*ns + k] = list
+// (epoch + b), id k (0 where no such list exists) are refreshed, so the
+// force launches read them without waiting for the counter.
+__global__ void epoch_kernel(const EpochDesc *__restrict__ jobs, int set, int value) {
+  const EpochDesc ed = jobs[blockIdx.x];
+  const int old = *ed.epoch;
+  __syncthreads();
+  const int e = set ? value : old + 1;
+  if (threadIdx.x == 0) *ed.epoch = e;
+  const int total = ed.batches * ed.ns;
+  for (int i = threadIdx.x; i < total; i += blockDim.x) {
+    const int b = i / ed.ns;
+    const int k = i - b * ed.ns;
+    const int list = e + b;
+    ed.negcur[i] = (list >= 0 && list < ed.n_lists) ? ed.negs[(size_t)list * ed.ns + k] : 0;
+  }
 }
 
 __global__ void to_bf16_kernel(const float *__restrict__ x, uint16_t *__restrict__ xs,
@@ -745,41 +853,48 @@ __global__ __launch_bounds__(1024) void reduce_kernel(const double *__restrict__
 // ---- launch wrappers -------------------------------------------------------
 
 // lds_bytes: staged negative rows and sub-range sums (force_lds_bytes);
-// mode: 0 all steps gather FP32, 1 all gather BF16, 2 mixed.
+// mode: 0 all steps gather FP32, 1 all gather BF16, 2 mixed; g_const: the
+// lanes per row G shared by every step of the launch, or 0 if they differ.
+template <int U, int GT>
+static void launch_force_g(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
+                           const StepDesc *steps, const int *blk_prefix, int n_steps, int ipw) {
+  if (mode == 0) hipLaunchKernelGGL((force_kernel<U, 0, GT>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
+  else if (mode == 1) hipLaunchKernelGGL((force_kernel<U, 1, GT>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
+  else hipLaunchKernelGGL((force_kernel<U, 2, GT>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
+}
+
 template <int U>
 static void launch_force_u(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
-                           const StepDesc *steps, const int *blk_prefix, int n_steps, int ipw) {
-  if (mode == 0) hipLaunchKernelGGL((force_kernel<U, 0>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
-  else if (mode == 1) hipLaunchKernelGGL((force_kernel<U, 1>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
-  else hipLaunchKernelGGL((force_kernel<U, 2>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
+                           int g_const, const StepDesc *steps, const int *blk_prefix,
+                           int n_steps, int ipw) {
+  // compile-time G for the common widths d = 32, 64, 128 (FP32 or BF16)
+  if (mode != 2 && g_const == 16) launch_force_g<U, 16>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw);
+  else if (mode != 2 && g_const == 8) launch_force_g<U, 8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw);
+  else if (mode != 2 && g_const == 4) launch_force_g<U, 4>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw);
+  else launch_force_g<U, 0>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw);
 }
 
 hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_steps,
                         int n_blocks, int wpb, int ipw, int unroll, size_t lds_bytes, int mode,
-                        hipStream_t st) {
+                        int g_const, hipStream_t st) {
   dim3 grid(n_blocks), block(64 * wpb);
   switch (unroll) {
-  case 1: launch_force_u<1>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw); break;
-  case 2: launch_force_u<2>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw); break;
-  case 8: launch_force_u<8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw); break;
-  default: launch_force_u<4>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw); break;
+  case 1: launch_force_u<1>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, n_steps, ipw); break;
+  case 2: launch_force_u<2>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, n_steps, ipw); break;
+  case 8: launch_force_u<8>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, n_steps, ipw); break;
+  default: launch_force_u<4>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, n_steps, ipw); break;
   }
   return hipGetLastError();
 }
 
-hipError_t launch_tick(int *const *ctrs, int n, hipStream_t st) {
-  hipLaunchKernelGGL(tick_kernel, dim3(1), dim3(256), 0, st, ctrs, n);
+hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, hipStream_t st) {
+  hipLaunchKernelGGL(epoch_kernel, dim3(n), dim3(256), 0, st, jobs, set, value);
   return hipGetLastError();
 }
 
 hipError_t launch_apply(const StepDesc *steps, const int *blk_prefix, int n_steps,
                         int n_blocks, hipStream_t st) {
   hipLaunchKernelGGL(apply_kernel, dim3(n_blocks), dim3(256), 0, st, steps, blk_prefix, n_steps);
-  return hipGetLastError();
-}
-
-hipError_t launch_set_int(int *p, int v, hipStream_t st) {
-  hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(64), 0, st, p, v);
   return hipGetLastError();
 }
 
