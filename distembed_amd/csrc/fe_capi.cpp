@@ -254,47 +254,41 @@ extern "C" fe_status fe_step_delta(fe_ctx *c, int epoch, int batch, float *delta
   return hip_ok(c, hipStreamSynchronize(c->stream), "fe_step_delta sync");
 }
 
-// One epoch of launches on the context stream: the epoch tick, then either
-// the fused chain (each batch hands its rows to the next launch) or, with a
-// single batch, force + apply.
-static fe_status enqueue_epoch(fe_ctx *c, bool first) {
+// One epoch of launches on the context stream: the epoch tick, then one
+// fused force launch per batch (each writes its new rows into the free slot
+// of the layout, where the next launches read them).
+static fe_status enqueue_epoch(fe_ctx *c) {
   Job &j = c->job;
   fe_status s = from_job(c, job_tick(j, c->stream));
   if (s != FE_OK) return s;
-  if (!job_can_fuse(j)) {
-    if ((s = from_job(c, job_launch_force(j, 0, DESC_SINGLE, c->stream, next_events(c)))) != FE_OK)
+  for (int b = 0; b < j.plan.batches; ++b)
+    if ((s = from_job(c, job_launch_force(j, b, DESC_FUSED, c->stream, next_events(c)))) != FE_OK)
       return s;
-    return from_job(c, job_launch_apply(j, 0, DESC_SINGLE, c->stream));
-  }
-  for (int b = 0; b < j.plan.batches; ++b) {
-    const DescKind kind = (first && b == 0) ? DESC_FIRST : DESC_FUSED;
-    if ((s = from_job(c, job_launch_force(j, b, kind, c->stream, next_events(c)))) != FE_OK) return s;
-  }
   return FE_OK;
 }
 
+// Every epoch of a run is the same graph: the epoch state on the device
+// selects the negative lists, the repulsive row count and the slots.
 static fe_status capture_epochs(fe_ctx *c) {
   drop_graph(c);
-  for (int k = 0; k < 2; ++k) {
-    fe_status s = hip_ok(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
-                         "hipStreamBeginCapture");
-    if (s != FE_OK) return s;
-    const size_t ev0 = c->events_used;
-    s = enqueue_epoch(c, k == 0);
-    hipGraph_t g = nullptr;
-    hipError_t e = hipStreamEndCapture(c->stream, &g);
-    if (s == FE_OK) s = hip_ok(c, e, "hipStreamEndCapture");
-    if (s == FE_OK) s = hip_ok(c, hipGraphInstantiate(&c->graph_exec[k], g, nullptr, nullptr, 0),
-                               "hipGraphInstantiate");
-    c->graph[k] = g;
-    if (s != FE_OK) {
-      drop_graph(c);
-      c->events_used = 0;
-      return s;
-    }
-    if (c->timing && c->events_used - ev0 != (size_t)2 * (job_can_fuse(c->job) ? c->job.plan.batches : 1))
-      return fail(c, FE_ERR_STATE, "kernel timing: unexpected event count in the captured epoch");
+  fe_status s = hip_ok(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal),
+                       "hipStreamBeginCapture");
+  if (s != FE_OK) return s;
+  const size_t ev0 = c->events_used;
+  s = enqueue_epoch(c);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(c->stream, &g);
+  if (s == FE_OK) s = hip_ok(c, e, "hipStreamEndCapture");
+  if (s == FE_OK) s = hip_ok(c, hipGraphInstantiate(&c->graph_exec[0], g, nullptr, nullptr, 0),
+                             "hipGraphInstantiate");
+  c->graph[0] = g;
+  if (s != FE_OK) {
+    drop_graph(c);
+    c->events_used = 0;
+    return s;
   }
+  if (c->timing && c->events_used - ev0 != (size_t)2 * c->job.plan.batches)
+    return fail(c, FE_ERR_STATE, "kernel timing: unexpected event count in the captured epoch");
   c->graph_valid = true;
   c->graph_timed = c->timing;
   return FE_OK;
@@ -318,43 +312,43 @@ extern "C" fe_status fe_train(fe_ctx *c, int iterations, int batch_size, int ns,
   } else if (ns == 0 && j.ns != 0) {
     if ((s = fe_set_negatives(c, nullptr, 0, 0)) != FE_OK) return s;
   }
-  if ((s = from_job(c, job_prepare(j, c->stream))) != FE_OK) return s;
+  if ((s = from_job(c, job_prepare(j, c->stream, true))) != FE_OK) return s;
   if (iterations == 0) return FE_OK;
   if ((s = from_job(c, job_reset_counters(j, c->stream))) != FE_OK) return s;
   // the tick at the start of every epoch brings the counter to 0, 1, ...
   if ((s = from_job(c, job_set_epoch(j, -1, c->stream))) != FE_OK)
     return s;
+  int done = 0;
   if (use_graph) {
     if (!c->graph_valid || c->graph_timed != c->timing) {
       c->events_used = 0;
       if ((s = capture_epochs(c)) != FE_OK) return s;
     }
-    // events captured in the two graphs: [0, e1) first epoch, [e1, e2) later
-    const size_t per_epoch = c->timing ? (size_t)2 * (job_can_fuse(j) ? batches : 1) : 0;
+    const size_t per_epoch = c->timing ? (size_t)2 * batches : 0;
     for (int it = 0; it < iterations; ++it) {
-      const int k = it == 0 ? 0 : 1;
-      if ((s = hip_ok(c, hipGraphLaunch(c->graph_exec[k], c->stream), "hipGraphLaunch")) != FE_OK)
-        return s;
+      if ((s = hip_ok(c, hipGraphLaunch(c->graph_exec[0], c->stream), "hipGraphLaunch")) != FE_OK)
+        break;
+      ++done;
       if (c->timing) {
         // every replay re-stamps the captured events: read them now
-        if ((s = hip_ok(c, hipStreamSynchronize(c->stream), "sync")) != FE_OK) return s;
-        if ((s = harvest_range(c, k * per_epoch, (k + 1) * per_epoch)) != FE_OK) return s;
+        if ((s = hip_ok(c, hipStreamSynchronize(c->stream), "sync")) != FE_OK) break;
+        if ((s = harvest_range(c, 0, per_epoch)) != FE_OK) break;
       }
     }
   } else {
     c->events_used = 0;
     for (int it = 0; it < iterations; ++it) {
-      if ((s = enqueue_epoch(c, it == 0)) != FE_OK) return s;
+      if ((s = enqueue_epoch(c)) != FE_OK) break;
+      ++done;
       if (c->timing && c->events_used >= 4096) {
-        if ((s = hip_ok(c, hipStreamSynchronize(c->stream), "sync")) != FE_OK) return s;
-        if ((s = harvest_events(c)) != FE_OK) return s;
+        if ((s = hip_ok(c, hipStreamSynchronize(c->stream), "sync")) != FE_OK) break;
+        if ((s = harvest_events(c)) != FE_OK) break;
       }
     }
   }
-  if (job_can_fuse(j)) {
-    // the last batch's rows are still in its staging slot
-    if ((s = from_job(c, job_launch_apply(j, batches - 1, DESC_FUSED, c->stream))) != FE_OK) return s;
-  }
+  // every epoch moved each batch down one slot
+  job_advance_layout(j, done);
+  if (s != FE_OK) return s;
   if ((s = hip_ok(c, hipStreamSynchronize(c->stream), "fe_train sync")) != FE_OK) return s;
   if (c->timing && !use_graph) return harvest_events(c);
   return FE_OK;
@@ -410,6 +404,8 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
   std::vector<Tuning> cands = shapes(orig);
   size_t stage_end = cands.size();
   int stage = 1;
+  // trials start from rotation 0 and the rows are restored there after each
+  if ((s = from_job(c, job_normalize_layout(j, c->stream))) != FE_OK) return s;
   const size_t xbytes = sizeof(float) * (size_t)j.n * j.d;
   void *backup = nullptr;
   if ((s = hip_ok(c, hipMalloc(&backup, xbytes), "fe_autotune backup")) != FE_OK) return s;
@@ -446,6 +442,7 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
       best_tun = cands[i];
     }
     s = hip_ok(c, hipMemcpyAsync(j.X, backup, xbytes, hipMemcpyDeviceToDevice, c->stream), "restore");
+    j.rot = 0;
     if (s == FE_OK && j.dtype == FE_GATHER_BF16)
       s = from_job(c, job_check(j, launch_to_bf16(j.X, j.Xs, (int64_t)j.n * j.d, c->stream), "shadow"));
     if (i + 1 == stage_end && stage < stages) {

@@ -17,27 +17,40 @@ namespace fe {
 constexpr int kItemInts = 8;
 constexpr int kSub = 4;
 
+// Row layout of a job's embedding (X and the BF16 shadow Xs). The rows are
+// cut into batches of B; batch k occupies one slot of B rows, and there are
+// P = batches + 1 slots, one of them free. Between runs, batch k lives in
+// slot (k - rot) mod P (rot = 0: row c at position c). During epoch e of a
+// run, batch j writes its new rows into the free slot, so the slot of batch
+// k is (k - rot - e - [k < j]) mod P and every batch moves down one slot per
+// epoch: no staging copy and no second pass over the rows. The state
+// {epoch, rot, shift = (rot + epoch) mod P} sits in three device ints.
+struct Layout {
+  uint32_t div_m; // k = (c * div_m) >> div_sh = c / B for 0 <= c < 2^31
+  int div_sh;
+  int B;
+  int P;
+};
+
 // One minibatch step of one job (graph). Everything here is invariant across
 // epochs: the epoch-dependent parts (negative list index epoch+batch,
-// cpp/algo/algo.hpp:165, and the repulsive row count, :135,167-169) are
-// derived in the kernels from the job's device epoch counter.
+// cpp/algo/algo.hpp:165, the repulsive row count, :135,167-169, and the slot
+// rotation) are derived in the kernels from the job's device state ints.
 struct alignas(16) StepDesc {
-  const float *X;        // FP32 master copy, n x D row-major
-  float *Xw;             // same buffer, written by the copy phase / apply
+  const float *X;        // FP32 master copy, P*B x D slot layout
+  float *Xw;             // same buffer, written by the step or the apply
   const uint16_t *Xs;    // BF16 shadow (gather source when bf16 != 0)
   uint16_t *Xsw;         // same buffer, refreshed with the master rows
   const int32_t *col;    // CSR column ids (all of the job's graph)
   const int32_t *items;  // this batch's work items
-  float *S;              // rows x D: new rows of this batch (or deltas)
+  float *S;              // rows x D: new rows or deltas (unfused steps)
   float *P;              // partial sums of heavy-row chunks
   int *counters;         // per split row of the batch: chunks finished
   const int32_t *negs;   // negative lists, list l at negs + l*ns
   const int32_t *neg_ids; // this step's ns negative ids (list epoch + batch), read
                           // without waiting for the epoch counter
-  const int *epoch;      // device epoch counter of the job
-  const float *Sprev;    // new rows of the previous batch, not yet in X
-  uint16_t *Sb;          // BF16 gathers: this batch's new rows rounded, beside S
-  const uint16_t *Sbprev; // ... of the previous batch (read instead of Xs)
+  const int *state;      // {epoch, rot, shift} of the job (see Layout)
+  Layout lay;
   int n_items;           // block items + wave items + light rows
   int n_block_items;
   int n_light;           // light rows (one per lane group), last in items
@@ -53,23 +66,23 @@ struct alignas(16) StepDesc {
   int bf16;              // gathers read the BF16 shadow
   int c_limit;           // attractive neighbours must satisfy c < c_limit
   int delta_mode;        // S receives the delta instead of the new row
-  int prev_begin;        // rows of the previous batch held in Sprev
-  int prev_rows;         // 0: no pending previous batch
+  int fused;             // 1: rows go straight to the free slot (a run's
+                         //    epoch chain); 0: to S, then the apply kernel
   float lr;
   float bound;           // MAX_BOUND of cpp/algo/algo.hpp:65-72
 };
 
-// Epoch counter of one job and the per-batch copies of the negative ids of
+// Epoch state of one job and the per-batch copies of the negative ids of
 // the current epoch: negcur[b*ns + k] = negs[(epoch + b)*ns + k] (0 where the
 // list does not exist), refreshed whenever the counter changes.
 struct alignas(16) EpochDesc {
-  int *epoch;
+  int *state;           // {epoch, rot, shift}
   const int32_t *negs;
   int32_t *negcur;
   int batches;
   int ns;
   int n_lists;
-  int pad;
+  int P;                // slots of the layout
 };
 
 // Loss over one job's graph (FP64 reduction).
@@ -77,6 +90,8 @@ struct alignas(16) LossDesc {
   const float *X;
   const int32_t *col;
   const int32_t *items; // all batches' items
+  Layout lay;
+  int rot;              // rows between runs: batch k in slot (k - rot) mod P
   int n_items;
   int D;
   int G;

@@ -152,10 +152,13 @@ static fe_status ehip(fe_engine *e, hipError_t h, const char *what) {
 }
 
 // Round layout in the staging buffers for K steps:
-//   StepDesc[K] | int epochs[K] | int force_prefix[K+1] | int apply_prefix[K+1]
+//   StepDesc[K] | int state[4K] | int force_prefix[K+1] | int apply_prefix[K+1]
+// state of step k: {epoch, rot 0, shift 0, unused} (engine jobs keep the
+// layout at rest with rotation 0).
+constexpr int kStateInts = 4;
 static size_t round_bytes(size_t K) {
   size_t b = K * sizeof(StepDesc);
-  b += K * sizeof(int) + 2 * (K + 1) * sizeof(int);
+  b += K * kStateInts * sizeof(int) + 2 * (K + 1) * sizeof(int);
   return (b + 255) / 256 * 256;
 }
 
@@ -386,7 +389,7 @@ static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *
   K = (int)run.size();
   StepDesc *sd = reinterpret_cast<StepDesc *>(host);
   int *ep = reinterpret_cast<int *>(host + K * sizeof(StepDesc));
-  int *pf = ep + K;
+  int *pf = ep + kStateInts * K;
   int *pa = pf + K + 1;
   int *dep = reinterpret_cast<int *>(e->dev_round + K * sizeof(StepDesc));
   const int ipb = e->tun.waves_per_block * e->tun.items_per_wave;
@@ -397,8 +400,11 @@ static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *
     Job &j = ej->job;
     const int b = ej->batch;
     sd[k] = j.host_desc[(size_t)j.plan.batches * DESC_SINGLE + b];
-    sd[k].epoch = dep + k;
-    ep[k] = ej->epoch;
+    sd[k].state = dep + kStateInts * k;
+    ep[kStateInts * k] = ej->epoch;
+    ep[kStateInts * k + 1] = 0;
+    ep[kStateInts * k + 2] = 0;
+    ep[kStateInts * k + 3] = 0;
     // this round's negative ids, straight from the job's lists
     const int list = ej->epoch + b;
     sd[k].neg_ids = (j.ns > 0 && list < j.n_lists) ? j.negs + (size_t)list * j.ns : j.negs;
@@ -446,7 +452,7 @@ extern "C" fe_status fe_engine_run(fe_engine *e, int rounds, int *running) {
     size_t lds = 0;
     std::vector<EngineJob *> finished;
     build_round(e, run, stg.host, K, lds, finished);
-    const int *hpf = reinterpret_cast<const int *>(stg.host + K * sizeof(StepDesc)) + K;
+    const int *hpf = reinterpret_cast<const int *>(stg.host + K * sizeof(StepDesc)) + kStateInts * K;
     const int *hpa = hpf + K + 1;
     const int fblocks = hpf[K], ablocks = hpa[K];
     if ((s = ehip(e, hipMemcpyAsync(e->dev_round, stg.host, round_bytes(K), hipMemcpyHostToDevice,
@@ -455,7 +461,7 @@ extern "C" fe_status fe_engine_run(fe_engine *e, int rounds, int *running) {
       return s;
     if ((s = ehip(e, hipEventRecord(stg.copied, e->stream), "event")) != FE_OK) return s;
     const StepDesc *dsd = reinterpret_cast<const StepDesc *>(e->dev_round);
-    const int *dpf = reinterpret_cast<const int *>(e->dev_round + K * sizeof(StepDesc)) + K;
+    const int *dpf = reinterpret_cast<const int *>(e->dev_round + K * sizeof(StepDesc)) + kStateInts * K;
     const int *dpa = dpf + K + 1;
     if ((s = ehip(e, launch_force(dsd, dpf, K, fblocks, e->tun.waves_per_block,
                                   e->tun.items_per_wave, e->tun.unroll, lds, round_mode(run),
@@ -570,7 +576,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
     // descriptors point at this round's own epoch slots in the device schedule
     char *h = host.data() + total;
     build_round(e, run, h, K, lds, finished);
-    const int *hpf = reinterpret_cast<const int *>(h + K * sizeof(StepDesc)) + K;
+    const int *hpf = reinterpret_cast<const int *>(h + K * sizeof(StepDesc)) + kStateInts * K;
     rounds.push_back({total, K, hpf[K], hpf[2 * K + 1], lds, round_mode(run), round_g(run)});
     for (EngineJob *ej : finished) finish_at.push_back({(int)rounds.size() - 1, ej});
     total += rb;
@@ -582,7 +588,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
   for (const RoundInfo &ri : rounds) {
     StepDesc *sd = reinterpret_cast<StepDesc *>(host.data() + ri.offset);
     int *dep = reinterpret_cast<int *>(dsched + ri.offset + ri.K * sizeof(StepDesc));
-    for (int k = 0; k < ri.K; ++k) sd[k].epoch = dep + k;
+    for (int k = 0; k < ri.K; ++k) sd[k].state = dep + kStateInts * k;
   }
   s = ehip(e, hipMemcpy(dsched, host.data(), total, hipMemcpyHostToDevice), "schedule copy");
   // capture and instantiate every segment first, then replay them back to back
@@ -598,7 +604,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
     for (size_t r = r0; r < r1 && ls == FE_OK; ++r) {
       const RoundInfo &ri = rounds[r];
       const StepDesc *dsd = reinterpret_cast<const StepDesc *>(dsched + ri.offset);
-      const int *dpf = reinterpret_cast<const int *>(dsched + ri.offset + ri.K * sizeof(StepDesc)) + ri.K;
+      const int *dpf = reinterpret_cast<const int *>(dsched + ri.offset + ri.K * sizeof(StepDesc)) + kStateInts * ri.K;
       const int *dpa = dpf + ri.K + 1;
       ls = ehip(e, launch_force(dsd, dpf, ri.K, ri.fblocks, e->tun.waves_per_block,
                                 e->tun.items_per_wave, e->tun.unroll, ri.lds, ri.mode, ri.g_const,

@@ -55,6 +55,91 @@ static fe_status ensure_cap(Job &j, T *&ptr, size_t &cap, size_t count, const ch
   return FE_OK;
 }
 
+// Grows a buffer to `count` elements keeping its first `keep` elements.
+template <typename T>
+static fe_status grow_keep(Job &j, T *&ptr, size_t &cap, size_t count, size_t keep,
+                           const char *what, hipStream_t st) {
+  if (count <= cap && ptr) return FE_OK;
+  T *old = ptr;
+  const size_t old_cap = cap;
+  ptr = nullptr;
+  cap = 0;
+  fe_status s = ensure_cap(j, ptr, cap, count, what);
+  if (s != FE_OK) {
+    ptr = old;
+    cap = old_cap;
+    return s;
+  }
+  if (old && keep > 0)
+    FE_HIP(j, hipMemcpyAsync(ptr, old, sizeof(T) * std::min(keep, old_cap), hipMemcpyDeviceToDevice, st),
+           what);
+  if (old) {
+    FE_HIP(j, hipStreamSynchronize(st), what);
+    dev_free(j, old);
+    j.device_bytes -= (int64_t)(old_cap * sizeof(T));
+  }
+  return FE_OK;
+}
+
+// Slot layout for batch size B: P = batches + 1 slots and the constants of
+// k = floor(c / B) = (c * m) >> (31 + l), l = ceil(log2 B), m = ceil(2^(31+l)/B),
+// exact for every 0 <= c < 2^31 (Granlund and Montgomery, PLDI 1994, Thm 4.2).
+static Layout make_layout(int B, int batches) {
+  Layout L;
+  int l = 0;
+  while ((1LL << l) < B) ++l;
+  const uint64_t num = 1ULL << (31 + l);
+  L.div_m = (uint32_t)((num + (uint64_t)B - 1) / (uint64_t)B);
+  L.div_sh = 31 + l;
+  L.B = B;
+  L.P = batches + 1;
+  return L;
+}
+
+// Position of row c in the layout at rest with rotation rot.
+static int64_t rest_pos(const Job &j, const Layout &L, int rot, int64_t c) {
+  const int64_t k = c / L.B;
+  int64_t sl = (k - rot) % L.P;
+  if (sl < 0) sl += L.P;
+  return sl * L.B + (c - k * L.B);
+}
+
+// Brings X (and the shadow) into the layout at rest with rotation 0 (row c
+// at position c).
+fe_status job_normalize_layout(Job &j, hipStream_t st) {
+  if (j.rot == 0) return FE_OK;
+  const Layout old = make_layout(j.lay_B, (int)((j.n + j.lay_B - 1) / j.lay_B));
+  float *nx = nullptr;
+  size_t ncap = 0;
+  fe_status s = ensure_cap(j, nx, ncap, j.cap_X, "X (layout change)");
+  if (s != FE_OK) return s;
+  const int nbo = old.P - 1;
+  for (int k = 0; k < nbo; ++k) {
+    const int64_t r0 = (int64_t)k * old.B;
+    const int64_t rows = std::min<int64_t>(r0 + old.B, j.n) - r0;
+    const int64_t src = rest_pos(j, old, j.rot, r0);
+    FE_HIP(j, hipMemcpyAsync(nx + r0 * j.d, j.X + src * j.d, sizeof(float) * rows * j.d,
+                             hipMemcpyDeviceToDevice, st),
+           "layout change copy");
+  }
+  FE_HIP(j, hipStreamSynchronize(st), "layout change sync");
+  dev_free(j, j.X);
+  j.device_bytes -= (int64_t)(j.cap_X * sizeof(float));
+  j.X = nx;
+  j.cap_X = ncap;
+  if (j.dtype == FE_GATHER_BF16)
+    FE_HIP(j, launch_to_bf16(j.X, j.Xs, (int64_t)j.n * j.d, st), "layout change shadow");
+  j.rot = 0;
+  j.steps_dirty = true;
+  return FE_OK;
+}
+
+void job_advance_layout(Job &j, int epochs) {
+  if (j.lay.P <= 0 || epochs <= 0) return;
+  j.rot = (int)(((int64_t)j.rot + epochs) % j.lay.P);
+  j.lay_B = j.lay.B;
+}
+
 int force_blocks_for(const StepDesc &sd, int items_per_block) {
   const int pack = 64 / sd.G;
   const int units = (sd.n_items - sd.n_block_items - sd.n_light) + (sd.n_light + pack - 1) / pack;
@@ -108,11 +193,11 @@ fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks) {
     s = ensure_cap(j, j.Xs, j.cap_Xs, (size_t)n * d, "BF16 shadow");
     if (s != FE_OK) return s;
   }
-  return ensure_cap(j, j.epoch, j.cap_epoch, 1, "epoch counter");
+  return ensure_cap(j, j.epoch, j.cap_epoch, 4, "epoch state");
 }
 
 void job_free(Job &j) {
-  void *bufs[] = {j.X, j.Xs, j.col, j.items, j.S, j.Sb, j.P, j.counters, j.negs, j.epoch, j.desc,
+  void *bufs[] = {j.X, j.Xs, j.col, j.items, j.S, j.P, j.counters, j.negs, j.epoch, j.desc,
                   j.prefix, j.negcur, j.edesc};
   for (void *p : bufs) dev_free(j, p);
   j.X = nullptr;
@@ -120,10 +205,9 @@ void job_free(Job &j) {
   j.col = j.items = j.negs = j.negcur = nullptr;
   j.edesc = nullptr;
   j.S = j.P = nullptr;
-  j.Sb = nullptr;
   j.counters = j.epoch = j.prefix = nullptr;
   j.desc = nullptr;
-  j.cap_items = j.cap_S = j.cap_Sb = j.cap_P = j.cap_ctr = j.cap_negs = j.cap_desc = 0;
+  j.cap_items = j.cap_S = j.cap_P = j.cap_ctr = j.cap_negs = j.cap_desc = 0;
   j.cap_prefix = j.cap_negcur = j.cap_edesc = j.cap_col = j.cap_X = j.cap_Xs = j.cap_epoch = 0;
   j.device_bytes = 0;
 }
@@ -174,11 +258,14 @@ fe_status job_upload_x(Job &j, const float *x, hipStream_t st) {
     return FE_ERR_INVALID_ARG;
   }
   const size_t count = (size_t)j.n * j.d;
+  // the layout at rest with rotation 0: row c at position c
   FE_HIP(j, hipMemcpyAsync(j.X, x, sizeof(float) * count, hipMemcpyHostToDevice, st),
          "fe_upload_x: copy");
   if (j.dtype == FE_GATHER_BF16)
     FE_HIP(j, launch_to_bf16(j.X, j.Xs, (int64_t)count, st), "fe_upload_x: shadow refresh");
+  FE_HIP(j, hipMemsetAsync(j.epoch, 0, sizeof(int) * 4, st), "fe_upload_x: state");
   FE_HIP(j, hipStreamSynchronize(st), "fe_upload_x: sync");
+  j.rot = 0;
   j.x_ready = true;
   return FE_OK;
 }
@@ -192,8 +279,20 @@ fe_status job_download_x(Job &j, float *x, hipStream_t st) {
     j.err = "fe_download_x: X was never uploaded";
     return FE_ERR_STATE;
   }
-  FE_HIP(j, hipMemcpyAsync(x, j.X, sizeof(float) * (size_t)j.n * j.d, hipMemcpyDeviceToHost, st),
-         "fe_download_x: copy");
+  if (j.rot == 0) {
+    FE_HIP(j, hipMemcpyAsync(x, j.X, sizeof(float) * (size_t)j.n * j.d, hipMemcpyDeviceToHost, st),
+           "fe_download_x: copy");
+  } else {
+    // one copy per batch from its slot
+    const Layout L = make_layout(j.lay_B, (int)((j.n + j.lay_B - 1) / j.lay_B));
+    for (int k = 0; k < L.P - 1; ++k) {
+      const int64_t r0 = (int64_t)k * L.B;
+      const int64_t rows = std::min<int64_t>(r0 + L.B, j.n) - r0;
+      FE_HIP(j, hipMemcpyAsync(x + r0 * j.d, j.X + rest_pos(j, L, j.rot, r0) * j.d,
+                               sizeof(float) * rows * j.d, hipMemcpyDeviceToHost, st),
+             "fe_download_x: copy");
+    }
+  }
   FE_HIP(j, hipStreamSynchronize(st), "fe_download_x: sync");
   return FE_OK;
 }
@@ -254,32 +353,23 @@ fe_status job_set_tuning(Job &j, const Tuning &t) {
   return FE_OK;
 }
 
-bool job_can_fuse(const Job &j) { return j.plan.batches > 1; }
-
-// Staging slot of batch b when batches hand rows over: slots alternate, and
-// with an odd batch count the final batch takes a third slot so that batch 0
-// of the next epoch never writes the slot it reads.
-static int out_slot(const Job &j, int b) {
-  const int nb = j.plan.batches;
-  if ((nb & 1) && b == nb - 1) return 2;
-  return b & 1;
-}
 
 static StepDesc make_desc(const Job &j, int b, DescKind kind) {
   StepDesc sd;
   std::memset(&sd, 0, sizeof sd);
-  const size_t slot = (size_t)j.plan.B * j.d;
   sd.X = j.X;
   sd.Xw = j.X;
   sd.Xs = j.Xs;
   sd.Xsw = j.Xs;
   sd.col = j.col;
   sd.items = j.items + (size_t)kItemInts * j.plan.item_off[b];
+  sd.S = j.S;
   sd.P = j.P;
   sd.counters = j.counters;
   sd.negs = j.negs;
   sd.neg_ids = j.negcur ? j.negcur + (size_t)b * j.ns : nullptr;
-  sd.epoch = j.epoch;
+  sd.state = j.epoch;
+  sd.lay = j.lay;
   sd.n_items = (int)(j.plan.item_off[b + 1] - j.plan.item_off[b]);
   sd.n_block_items = j.plan.block_items[b];
   sd.n_light = j.plan.light_rows[b];
@@ -296,19 +386,7 @@ static StepDesc make_desc(const Job &j, int b, DescKind kind) {
   sd.lr = j.lr;
   sd.bound = j.bound;
   sd.delta_mode = kind == DESC_DELTA ? 1 : 0;
-  if (kind == DESC_FUSED || kind == DESC_FIRST) {
-    sd.S = j.S + slot * out_slot(j, b);
-    if (j.Sb) sd.Sb = j.Sb + slot * out_slot(j, b);
-    if (kind == DESC_FUSED) {
-      const int pb = b > 0 ? b - 1 : j.plan.batches - 1;
-      sd.Sprev = j.S + slot * out_slot(j, pb);
-      if (j.Sb) sd.Sbprev = j.Sb + slot * out_slot(j, pb);
-      sd.prev_begin = (int)((int64_t)pb * j.plan.B);
-      sd.prev_rows = rows_of_batch(j, pb);
-    }
-  } else {
-    sd.S = j.S;
-  }
+  sd.fused = kind == DESC_FUSED ? 1 : 0;
   return sd;
 }
 
@@ -317,8 +395,7 @@ const StepDesc *job_desc(const Job &j, int b, DescKind kind) {
   switch (kind) {
   case DESC_FUSED: return j.desc + b;
   case DESC_SINGLE: return j.desc + nb + b;
-  case DESC_DELTA: return j.desc + 2 * nb + b;
-  default: return j.desc + 3 * nb;
+  default: return j.desc + 2 * nb + b;
   }
 }
 
@@ -328,7 +405,7 @@ fe_status job_reset_counters(Job &j, hipStream_t st) {
   return FE_OK;
 }
 
-fe_status job_prepare(Job &j, hipStream_t st) {
+fe_status job_prepare(Job &j, hipStream_t st, bool slots) {
   if (j.nnz < 0) {
     j.err = "CSR not uploaded";
     return FE_ERR_STATE;
@@ -359,13 +436,13 @@ fe_status job_prepare(Job &j, hipStream_t st) {
       p.edges_eff_total += eff;
     }
     j.plan = std::move(p);
+    // a rotation left by runs with another batch size is undone first
+    if (j.rot != 0 && j.lay_B != j.plan.B && (s = job_normalize_layout(j, st)) != FE_OK) return s;
+    j.lay = make_layout(j.plan.B, j.plan.batches);
+    if (j.rot == 0) j.lay_B = j.plan.B;
     if ((s = ensure_cap(j, j.items, j.cap_items, j.plan.items.size(), "work items")) != FE_OK)
       return s;
-    if ((s = ensure_cap(j, j.S, j.cap_S, (size_t)3 * j.B * j.d, "staging rows")) != FE_OK)
-      return s;
-    if (j.dtype == FE_GATHER_BF16 &&
-        (s = ensure_cap(j, j.Sb, j.cap_Sb, (size_t)3 * j.B * j.d, "BF16 staging rows")) != FE_OK)
-      return s;
+    if ((s = ensure_cap(j, j.S, j.cap_S, (size_t)j.B * j.d, "step rows")) != FE_OK) return s;
     if ((s = ensure_cap(j, j.P, j.cap_P, (size_t)std::max(j.plan.max_pslots, 1) * j.d,
                         "partials")) != FE_OK)
       return s;
@@ -380,10 +457,24 @@ fe_status job_prepare(Job &j, hipStream_t st) {
     j.plan_dirty = false;
     j.steps_dirty = true;
   }
+  if (slots) {
+    // every slot of the layout: P * B rows (the rows at rest, < P * B, kept)
+    const size_t need = (size_t)j.lay.P * j.lay.B * j.d;
+    const size_t keep = j.rot == 0 ? (size_t)j.n * j.d : j.cap_X;
+    float *x0 = j.X;
+    uint16_t *xs0 = j.Xs;
+    fe_status s = grow_keep(j, j.X, j.cap_X, need, keep, "X slots", st);
+    if (s != FE_OK) return s;
+    if (j.dtype == FE_GATHER_BF16 &&
+        (s = grow_keep(j, j.Xs, j.cap_Xs, need, j.rot == 0 ? (size_t)j.n * j.d : j.cap_Xs,
+                       "shadow slots", st)) != FE_OK)
+      return s;
+    if (j.X != x0 || j.Xs != xs0) j.steps_dirty = true;
+  }
   if (j.steps_dirty) {
     const int nb = j.plan.batches;
     fe_status s;
-    if ((s = ensure_cap(j, j.desc, j.cap_desc, (size_t)3 * nb + 1, "step descriptors")) != FE_OK)
+    if ((s = ensure_cap(j, j.desc, j.cap_desc, (size_t)3 * nb, "step descriptors")) != FE_OK)
       return s;
     if ((s = ensure_cap(j, j.prefix, j.cap_prefix, (size_t)nb * 4, "block prefixes")) != FE_OK)
       return s;
@@ -392,7 +483,8 @@ fe_status job_prepare(Job &j, hipStream_t st) {
       return s;
     if ((s = ensure_cap(j, j.edesc, j.cap_edesc, 1, "epoch state")) != FE_OK) return s;
     EpochDesc ed{};
-    ed.epoch = j.epoch;
+    ed.state = j.epoch;
+    ed.P = j.lay.P;
     ed.negs = j.negs;
     ed.negcur = j.negcur;
     ed.batches = nb;
@@ -400,7 +492,7 @@ fe_status job_prepare(Job &j, hipStream_t st) {
     ed.n_lists = j.n_lists;
     FE_HIP(j, hipMemcpyAsync(j.edesc, &ed, sizeof ed, hipMemcpyHostToDevice, st),
            "epoch state copy");
-    std::vector<StepDesc> hd((size_t)3 * nb + 1);
+    std::vector<StepDesc> hd((size_t)3 * nb);
     std::vector<int> pf((size_t)nb * 4, 0);
     j.force_blocks.assign(nb, 0);
     j.apply_blocks.assign(nb, 0);
@@ -415,7 +507,6 @@ fe_status job_prepare(Job &j, hipStream_t st) {
       pf[4 * b + 1] = j.force_blocks[b];
       pf[4 * b + 3] = j.apply_blocks[b];
     }
-    hd[3 * nb] = make_desc(j, 0, DESC_FIRST);
     FE_HIP(j, hipMemcpyAsync(j.desc, hd.data(), sizeof(StepDesc) * hd.size(),
                              hipMemcpyHostToDevice, st),
            "descriptor copy");
@@ -434,7 +525,7 @@ fe_status job_set_epoch(Job &j, int value, hipStream_t st) {
     j.err = "epoch state not prepared";
     return FE_ERR_STATE;
   }
-  FE_HIP(j, launch_epoch(j.edesc, 1, 1, value, st), "epoch set");
+  FE_HIP(j, launch_epoch(j.edesc, 1, 1, value, j.rot, st), "epoch set");
   return FE_OK;
 }
 
@@ -443,7 +534,7 @@ fe_status job_tick(Job &j, hipStream_t st) {
     j.err = "epoch state not prepared";
     return FE_ERR_STATE;
   }
-  FE_HIP(j, launch_epoch(j.edesc, 1, 0, 0, st), "epoch tick");
+  FE_HIP(j, launch_epoch(j.edesc, 1, 0, 0, 0, st), "epoch tick");
   return FE_OK;
 }
 
@@ -475,6 +566,8 @@ fe_status job_loss(Job &j, uint64_t seed, int eval_ns, double *attr, double *rep
   ld.X = j.X;
   ld.col = j.col;
   ld.items = j.items;
+  ld.lay = j.rot == 0 ? j.lay : make_layout(j.lay_B, (int)((j.n + j.lay_B - 1) / j.lay_B));
+  ld.rot = j.rot;
   ld.n_items = (int)(j.plan.items.size() / kItemInts);
   ld.D = j.d;
   ld.G = j.G;
@@ -524,11 +617,11 @@ int64_t job_bytes_estimate(const Job &j) {
   int64_t b = nd * 4 + (j.dtype == FE_GATHER_BF16 ? nd * 2 : 0);
   b += std::max<int64_t>(j.nnz, 1) * 4;
   b += (int64_t)j.plan.items.size() * 4;
-  b += 3 * (int64_t)j.plan.B * j.d * (j.dtype == FE_GATHER_BF16 ? 6 : 4);
+  b += (int64_t)j.plan.B * j.d * 4;
   b += (int64_t)std::max(j.plan.max_pslots, 1) * j.d * 4;
   b += (int64_t)std::max(j.plan.max_splits, 1) * 4;
   b += (int64_t)j.n_lists * j.ns * 4;
-  b += (int64_t)(3 * j.plan.batches + 1) * (int64_t)sizeof(StepDesc) + 16LL * j.plan.batches + 4;
+  b += (int64_t)(3 * j.plan.batches) * (int64_t)sizeof(StepDesc) + 16LL * j.plan.batches + 16;
   b += (int64_t)j.plan.batches * std::max(j.ns, 1) * 4 + (int64_t)sizeof(EpochDesc);
   return b;
 }

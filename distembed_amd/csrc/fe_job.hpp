@@ -21,7 +21,8 @@ hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_step
 hipError_t launch_apply(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
                         hipStream_t st);
 // set != 0: every job's epoch counter := value; else += 1. Refreshes negcur.
-hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, hipStream_t st);
+hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, int rot,
+                        hipStream_t st);
 hipError_t launch_to_bf16(const float *x, uint16_t *xs, int64_t count, hipStream_t st);
 hipError_t launch_loss_edges(const LossDesc &ld, double *block_out, int n_blocks, int ipw,
                              hipStream_t st);
@@ -40,12 +41,12 @@ struct Tuning {
 };
 
 // Descriptor variants of one batch.
-//   FUSED: the previous batch's rows are pending in its staging slot; this
-//          launch reads them from there and copies them into X.
-//   FIRST: batch 0 of a run, nothing pending.
-//   SINGLE: nothing pending; an apply launch follows (fe_step).
-//   DELTA: nothing pending; S receives the delta, no apply (fe_step_delta).
-enum DescKind { DESC_FUSED = 0, DESC_SINGLE = 1, DESC_DELTA = 2, DESC_FIRST = 3 };
+//   FUSED: a step of a run's epoch chain: new rows go straight into the free
+//          slot of the layout (see Layout in fe_device.hpp).
+//   SINGLE: the layout at rest is read, new rows go to S and an apply launch
+//           writes them back in place (fe_step, engine rounds).
+//   DELTA: as SINGLE, but S receives the delta and nothing is applied.
+enum DescKind { DESC_FUSED = 0, DESC_SINGLE = 1, DESC_DELTA = 2 };
 
 // Device memory source of a job: hipMalloc by default, or an engine arena.
 struct DeviceAllocator {
@@ -83,18 +84,20 @@ struct Job {
   uint16_t *Xs = nullptr;
   int32_t *col = nullptr;
   int32_t *items = nullptr;
-  float *S = nullptr;        // 3 staging slots of B x d
-  uint16_t *Sb = nullptr;    // BF16 gathers: the same slots rounded to BF16
+  float *S = nullptr;        // B x d: new rows or deltas of unfused steps
   float *P = nullptr;
   int *counters = nullptr;
   int32_t *negs = nullptr;
   int32_t *negcur = nullptr;   // batches x ns: ids of the current epoch's lists
   EpochDesc *edesc = nullptr;  // one entry: this job's epoch state
-  int *epoch = nullptr;
+  int *epoch = nullptr;       // device state {epoch, rot, shift} (see Layout)
+  Layout lay{};               // slot layout for the planned batch size
+  int rot = 0;                // layout at rest: batch k in slot (k - rot) mod P
+  int lay_B = 0;              // batch size rot refers to
   StepDesc *desc = nullptr;  // 3 * batches + 1: FUSED[b], SINGLE[b], DELTA[b], FIRST
   int *prefix = nullptr;     // per batch: {0, force blocks}, {0, apply blocks}
   std::vector<StepDesc> host_desc;
-  size_t cap_items = 0, cap_S = 0, cap_Sb = 0, cap_P = 0, cap_ctr = 0, cap_negs = 0, cap_desc = 0;
+  size_t cap_items = 0, cap_S = 0, cap_P = 0, cap_ctr = 0, cap_negs = 0, cap_desc = 0;
   size_t cap_prefix = 0, cap_negcur = 0, cap_edesc = 0, cap_col = 0, cap_X = 0, cap_Xs = 0, cap_epoch = 0;
   bool plan_dirty = true;  // plan must be rebuilt (batch size, chunking, CSR)
   bool steps_dirty = true; // descriptors must be rebuilt
@@ -112,8 +115,14 @@ fe_status job_download_x(Job &j, float *x, hipStream_t st);
 fe_status job_set_negatives(Job &j, const int32_t *lists, int n_lists, int ns, hipStream_t st);
 fe_status job_configure(Job &j, int B, float lr, float bound);
 fe_status job_set_tuning(Job &j, const Tuning &t);
-// Builds the plan / descriptors if dirty and uploads them.
-fe_status job_prepare(Job &j, hipStream_t st);
+// Builds the plan / descriptors if dirty and uploads them. slots: X (and the
+// shadow) must hold every slot of the layout (fused runs); otherwise only
+// the n rows of the layout at rest are needed.
+fe_status job_prepare(Job &j, hipStream_t st, bool slots = false);
+// Layout at rest after a fused run of `epochs` epochs.
+void job_advance_layout(Job &j, int epochs);
+// Moves the rows back to rotation 0 (row c at position c).
+fe_status job_normalize_layout(Job &j, hipStream_t st);
 // Epoch counter := value (and the current negative ids), on stream st.
 fe_status job_set_epoch(Job &j, int value, hipStream_t st);
 // Epoch counter += 1 (start of an epoch; may be captured in a graph).
@@ -125,8 +134,6 @@ const StepDesc *job_desc(const Job &j, int b, DescKind kind);
 fe_status job_launch_force(Job &j, int b, DescKind kind, hipStream_t st, hipEvent_t *ev);
 // Apply launch (S -> X) of batch `b` with descriptor `kind`.
 fe_status job_launch_apply(Job &j, int b, DescKind kind, hipStream_t st);
-// Whether consecutive batches can hand rows over through staging slots.
-bool job_can_fuse(const Job &j);
 fe_status job_loss(Job &j, uint64_t seed, int eval_ns, double *attr, double *rep, hipStream_t st);
 // Device bytes a prepared job holds.
 int64_t job_bytes_estimate(const Job &j);

@@ -138,30 +138,42 @@ __device__ __forceinline__ int find_step(const int *__restrict__ prefix,
   return lo;
 }
 
-// Row c as 8 FP32 components [8q, 8q+8). Rows of the previous batch whose
-// new values still sit in Sprev are read from there; with BF16 gathers they
-// are rounded exactly as the shadow refresh will round them.
 // MODE 0: FP32 gathers, 1: BF16 gathers, 2: per step at run time (a launch
 // mixing jobs of both kinds).
 template <int MODE> __device__ __forceinline__ bool use_bf16(const StepDesc &sd) {
   return MODE == 2 ? sd.bf16 != 0 : MODE == 1;
 }
 
+// Slot positions of the rows one step reads (see Layout): batch k of row c
+// is in slot (k - shift - [k < jcut]) mod P.
+struct SlotMap {
+  uint32_t div_m;
+  int div_sh;
+  int B;
+  int P;
+  int shift;
+  int jcut;
+};
+
+__device__ __forceinline__ SlotMap make_map(const Layout &lay, int shift, int jcut) {
+  return SlotMap{lay.div_m, lay.div_sh, lay.B, lay.P, shift, jcut};
+}
+
+// Position (row index in the slot layout) of row c.
+__device__ __forceinline__ uint32_t row_pos(const SlotMap &m, uint32_t c) {
+  const int k = (int)(uint32_t)(((uint64_t)c * m.div_m) >> m.div_sh);
+  int sl = k - m.shift - (k < m.jcut ? 1 : 0);
+  sl += sl < 0 ? m.P : 0;
+  return c + (uint32_t)(sl - k) * (uint32_t)m.B;
+}
+
+// Row c as 8 FP32 components [8q, 8q+8) (BF16 gathers: the shadow's values).
 template <int MODE>
-__device__ __forceinline__ void load_row(const StepDesc &sd, int c, int q, float (&o)[8]) {
-  const int D = sd.D;
-  const unsigned off = (unsigned)(c - sd.prev_begin);
-  if (off < (unsigned)sd.prev_rows) {
-    load8_f32(sd.Sprev + (size_t)off * D + 8 * q, o);
-    if (use_bf16<MODE>(sd)) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) o[k] = __uint_as_float(bf16_bits(o[k]) << 16);
-    }
-  } else if (use_bf16<MODE>(sd)) {
-    load8_bf16(sd.Xs + (size_t)c * D + 8 * q, o);
-  } else {
-    load8_f32(sd.X + (size_t)c * D + 8 * q, o);
-  }
+__device__ __forceinline__ void load_row(const StepDesc &sd, const SlotMap &m, int c, int q,
+                                         float (&o)[8]) {
+  const size_t pos = row_pos(m, (uint32_t)c);
+  if (use_bf16<MODE>(sd)) load8_bf16(sd.Xs + pos * sd.D + 8 * q, o);
+  else load8_f32(sd.X + pos * sd.D + 8 * q, o);
 }
 
 // A gathered row slice held while its load is in flight: 8 FP32 registers,
@@ -175,44 +187,33 @@ template <> struct RowBuf<1> {
 };
 
 // Where a lane reads gathered rows from: its 8-component slice of X (or of
-// the BF16 shadow) and of the previous batch's staging slot. A row of the
-// previous batch is read from the slot (its new values are not in X yet);
-// the source is selected per lane without a branch and the address is one
-// 32 x 32 -> 64-bit multiply-add, so the U loads of a round issue back to
-// back.
+// the BF16 shadow) at the rows' slot positions. The address is arithmetic
+// only (no branch), so the U loads of a round issue back to back.
 struct RowSrc {
   const char *xq;      // X or Xs, plus this lane's slice offset
-  const char *sq;      // Sprev or Sbprev, plus the slice offset
   uint32_t row_bytes;  // bytes per gathered row
-  uint32_t prev_begin;
-  uint32_t prev_rows;
+  SlotMap m;
   bool bf;             // gathers read BF16
 };
 
 template <int MODE>
-__device__ __forceinline__ RowSrc make_src(const StepDesc &sd, int q) {
+__device__ __forceinline__ RowSrc make_src(const StepDesc &sd, const SlotMap &m, int q) {
   RowSrc r;
   r.bf = use_bf16<MODE>(sd);
   if (r.bf) {
     r.xq = reinterpret_cast<const char *>(sd.Xs + 8 * q);
-    r.sq = reinterpret_cast<const char *>(sd.Sbprev + 8 * q);
     r.row_bytes = (uint32_t)sd.D * 2u;
   } else {
     r.xq = reinterpret_cast<const char *>(sd.X + 8 * q);
-    r.sq = reinterpret_cast<const char *>(sd.Sprev + 8 * q);
     r.row_bytes = (uint32_t)sd.D * 4u;
   }
-  r.prev_begin = (uint32_t)sd.prev_begin;
-  r.prev_rows = (uint32_t)sd.prev_rows;
+  r.m = m;
   return r;
 }
 
 template <int MODE>
 __device__ __forceinline__ void load_buf(const RowSrc &src, int c, RowBuf<MODE> &b) {
-  const uint32_t off = (uint32_t)c - src.prev_begin;
-  const bool prev = off < src.prev_rows;
-  const uint32_t idx = prev ? off : (uint32_t)c;
-  const char *p = (prev ? src.sq : src.xq) + (uint64_t)idx * src.row_bytes;
+  const char *p = src.xq + (uint64_t)row_pos(src.m, (uint32_t)c) * src.row_bytes;
   if constexpr (MODE == 1) {
     b.w = *(gcu4 *)p;
   } else if constexpr (MODE == 0) {
@@ -275,11 +276,12 @@ __device__ __forceinline__ void interact(const float (&xd)[8], const float (&xc)
 // Staging piece p (negative p / G, slice p % G) of the step's negative rows,
 // as FP32 in LDS; id is that negative's row.
 template <int MODE>
-__device__ __forceinline__ void stage_piece(const StepDesc &sd, int p, int id, int G, float *lds) {
+__device__ __forceinline__ void stage_piece(const StepDesc &sd, const SlotMap &m, int p, int id,
+                                            int G, float *lds) {
   const int k = p / G;
   const int q = p - k * G;
   float v[8];
-  load_row<MODE>(sd, id, q, v);
+  load_row<MODE>(sd, m, id, q, v);
   store8_lds(lds + (size_t)k * sd.D + 8 * q, v);
 }
 
@@ -393,40 +395,41 @@ __device__ __forceinline__ void own_slice(const StepDesc &sd, const float (&xr)[
     xd[k] = use_bf16<MODE>(sd) ? __uint_as_float(bf16_bits(xr[k]) << 16) : xr[k];
 }
 
+// Row t of the batch: a fused step writes the new row straight into the
+// free slot (position out + t) of X and, with BF16 gathers, of the shadow;
+// an unfused step writes the new row or the delta to S.
 template <int MODE>
-__device__ __forceinline__ void load_own(const StepDesc &sd, int r, int q, float (&xr)[8],
-                                         float (&xd)[8]) {
-  load8_f32(sd.X + (size_t)r * sd.D + 8 * q, xr);
-  own_slice<MODE>(sd, xr, xd);
-}
-
-// Row t of the batch into S (new row or delta) and, when the next launch
-// gathers BF16, its rounded copy into Sb.
-template <int MODE>
-__device__ __forceinline__ void write_row(const StepDesc &sd, int t, int q, const float (&o)[8]) {
-  store8_f32(sd.S + (size_t)t * sd.D + 8 * q, o);
-  if (MODE != 0 && use_bf16<MODE>(sd) && sd.Sb != nullptr) {
+__device__ __forceinline__ void write_row(const StepDesc &sd, uint32_t out, int t, int q,
+                                          const float (&o)[8]) {
+  if (!sd.fused) {
+    store8_f32(sd.S + (size_t)t * sd.D + 8 * q, o);
+    return;
+  }
+  const size_t pos = (size_t)(out + (uint32_t)t);
+  store8_f32(sd.Xw + pos * sd.D + 8 * q, o);
+  if (use_bf16<MODE>(sd)) {
     u4v w;
     w.x = bf16_bits(o[0]) | (bf16_bits(o[1]) << 16);
     w.y = bf16_bits(o[2]) | (bf16_bits(o[3]) << 16);
     w.z = bf16_bits(o[4]) | (bf16_bits(o[5]) << 16);
     w.w = bf16_bits(o[6]) | (bf16_bits(o[7]) << 16);
-    *(__attribute__((address_space(1))) u4v *)(sd.Sb + (size_t)t * sd.D + 8 * q) = w;
+    *(__attribute__((address_space(1))) u4v *)(sd.Xsw + pos * sd.D + 8 * q) = w;
   }
 }
 
 // Repulsion (first `rep` rows), group combine, new row (or delta) to S.
 template <int MODE>
-__device__ __forceinline__ void finish_row(const StepDesc &sd, const float *lds_neg, bool repel,
-                                           int t, const float (&xr)[8], const float (&xd)[8],
-                                           float (&acc)[8], int G, int NG, int g, int q) {
+__device__ __forceinline__ void finish_row(const StepDesc &sd, uint32_t out, const float *lds_neg,
+                                           bool repel, int t, const float (&xr)[8],
+                                           const float (&xd)[8], float (&acc)[8], int G, int NG,
+                                           int g, int q) {
   if (repel) repulse(sd, lds_neg, xd, acc, G, NG, g, q);
   combine_groups(acc, G);
   if (g == 0) {
     float o[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
-    write_row<MODE>(sd, t, q, o);
+    write_row<MODE>(sd, out, t, q, o);
   }
 }
 
@@ -436,8 +439,8 @@ __device__ __forceinline__ void finish_row(const StepDesc &sd, const float *lds_
 // pack's largest degree kmax, so control flow stays uniform; slots past a
 // row's degree read the row itself (no force).
 template <int U, int MODE>
-__device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src, u4v ia,
-                                           int kmax, bool valid, const float (&xr)[8],
+__device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src, uint32_t out,
+                                           u4v ia, int kmax, bool valid, const float (&xr)[8],
                                            const float *lds_neg, bool has_neg, int rep, int G,
                                            int g, int q) {
   const int r = (int)ia.x, e0 = (int)ia.y, deg = valid ? (int)ia.z : 0;
@@ -492,7 +495,7 @@ __device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src
     float o[8];
 #pragma unroll
     for (int k = 0; k < 8; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
-    write_row<MODE>(sd, t, q, o);
+    write_row<MODE>(sd, out, t, q, o);
   }
 }
 
@@ -503,10 +506,9 @@ __device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src
 // partial (device-coherent stores) and bumps the row's counter; the chunk
 // that counts last adds all partials of the row in index order. The
 // remaining blocks' waves walk the light-row packs and then the medium rows
-// with a grid stride. Before any of that, every block copies its share of
-// the previous batch's rows from staging into X, while its first work item
-// and the negative rows are being loaded. GT > 0: every step of the launch
-// has G = GT lanes per row (a compile-time constant); 0: read per step.
+// with a grid stride. A fused step writes its new rows into the layout's
+// free slot, which no thread of the launch reads. GT > 0: every step of the
+// launch has G = GT lanes per row (a compile-time constant); 0: per step.
 template <int U, int MODE, int GT>
 __global__ __launch_bounds__(512) void force_kernel(
     const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
@@ -526,7 +528,7 @@ __global__ __launch_bounds__(512) void force_kernel(
   const int g = lane / G;
   const int q = lane & (G - 1);
 
-  // this wave's first work item, in flight during the copy and the staging
+  // this wave's first work item, in flight during the staging
   const int nbi = sd.n_block_items;
   const int n_light = sd.n_light;
   const int n_wave_items = sd.n_items - nbi - n_light;
@@ -545,50 +547,37 @@ __global__ __launch_bounds__(512) void force_kernel(
   } else if (u - n_packs < n_wave_items) {
     pre0 = gld_u4(witems + (size_t)kItemInts * (u - n_packs));
   }
-
   // this step's negative ids, one per staging piece (first blockDim pieces)
   const int total8 = sd.ns * G;
   const int pz = threadIdx.x;
   int nid = 0;
   if (pz < total8) nid = gld_int(sd.neg_ids + pz / G);
-  const int epoch = *sd.epoch;
 
-  // copy phase: the previous batch's new rows into X (and the shadow). No
-  // thread of this launch reads those rows of X: gathers of them are
-  // redirected to the staging slot.
-  if (sd.prev_rows > 0) {
-    const int64_t total4 = (int64_t)sd.prev_rows * D / 4;
-    const int64_t per = (total4 + nb - 1) / nb;
-    const int64_t lo = (int64_t)lb * per;
-    const int64_t hi = min(total4, lo + per);
-    const int64_t base4 = (int64_t)sd.prev_begin * D / 4;
-    const float4 *S4 = reinterpret_cast<const float4 *>(sd.Sprev);
-    float4 *X4 = reinterpret_cast<float4 *>(sd.Xw);
-    for (int64_t idx = lo + threadIdx.x; idx < hi; idx += blockDim.x) {
-      const float4 v = S4[idx];
-      X4[base4 + idx] = v;
-      if (use_bf16<MODE>(sd)) {
-        uint2 w;
-        w.x = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
-        w.y = bf16_bits(v.z) | (bf16_bits(v.w) << 16);
-        reinterpret_cast<uint2 *>(sd.Xsw)[base4 + idx] = w;
-      }
-    }
-  }
+  // slot positions: a fused step reads with the run's rotation and writes
+  // the free slot; an unfused step reads the layout at rest
+  const int epoch = sd.state[0];
+  const int shift = sd.fused ? sd.state[2] : sd.state[1];
+  const SlotMap m = make_map(sd.lay, shift, sd.fused ? sd.batch : 0);
+  int own_sl = sd.batch - shift;
+  own_sl += own_sl < 0 ? sd.lay.P : 0;
+  const uint32_t own_delta = (uint32_t)(own_sl - sd.batch) * (uint32_t)sd.lay.B;
+  int out_sl = own_sl - 1;
+  out_sl += out_sl < 0 ? sd.lay.P : 0;
+  const uint32_t out = (uint32_t)out_sl * (uint32_t)sd.lay.B;
 
   // the own row of this wave's first unit, in flight with the negative rows
   const bool unit0 = lb < nbi || u < n_packs + n_wave_items;
   float xr0[8];
-  if (unit0) load8_f32(sd.X + (size_t)pre0.x * D + 8 * q, xr0);
+  if (unit0) load8_f32(sd.X + (size_t)(pre0.x + own_delta) * D + 8 * q, xr0);
   const int rep = epoch == 0 ? sd.rep0 : sd.rep1;
   const bool has_neg = sd.ns > 0 && epoch + sd.batch < sd.n_lists;
   if (has_neg) {
-    if (pz < total8) stage_piece<MODE>(sd, pz, nid, G, lds);
+    if (pz < total8) stage_piece<MODE>(sd, m, pz, nid, G, lds);
     for (int p = pz + blockDim.x; p < total8; p += blockDim.x)
-      stage_piece<MODE>(sd, p, gld_int(sd.neg_ids + p / G), G, lds);
+      stage_piece<MODE>(sd, m, p, gld_int(sd.neg_ids + p / G), G, lds);
   }
   __syncthreads();
-  const RowSrc src = make_src<MODE>(sd, q);
+  const RowSrc src = make_src<MODE>(sd, m, q);
 
   if (lb < nbi) {
     const int r = (int)pre0.x, e0 = (int)pre0.y, cnt = (int)pre0.z, pslot = (int)pre0.w;
@@ -636,7 +625,8 @@ __global__ __launch_bounds__(512) void force_kernel(
           finish = true;
         }
       }
-      if (finish) finish_row<MODE>(sd, lds, has_neg && t < rep, t, xr0, xd, acc, G, NG, g, q);
+      if (finish)
+        finish_row<MODE>(sd, out, lds, has_neg && t < rep, t, xr0, xd, acc, G, NG, g, q);
     }
   } else {
     // wave work: packs of light rows (one row per lane group), then the
@@ -648,16 +638,16 @@ __global__ __launch_bounds__(512) void force_kernel(
       if (!have) {
         pre0 = gld_u4(lrows + (size_t)kItemInts * (idx < n_light ? idx : u * NG));
         pre1 = gld_u4(lrows + (size_t)kItemInts * u * NG + 4);
-        load8_f32(sd.X + (size_t)pre0.x * D + 8 * q, xr0);
+        load8_f32(sd.X + (size_t)(pre0.x + own_delta) * D + 8 * q, xr0);
       }
       have = false;
-      light_pack<U, MODE>(sd, src, pre0, (int)pre1.x, idx < n_light, xr0, lds, has_neg, rep, G,
-                          g, q);
+      light_pack<U, MODE>(sd, src, out, pre0, (int)pre1.x, idx < n_light, xr0, lds, has_neg,
+                          rep, G, g, q);
     }
     for (int it = u - n_packs; it < n_wave_items; it += W) {
       if (!have) {
         pre0 = gld_u4(witems + (size_t)kItemInts * it);
-        load8_f32(sd.X + (size_t)pre0.x * D + 8 * q, xr0);
+        load8_f32(sd.X + (size_t)(pre0.x + own_delta) * D + 8 * q, xr0);
       }
       have = false;
       const int r = (int)pre0.x, e0 = (int)pre0.y, cnt = (int)pre0.z;
@@ -667,14 +657,15 @@ __global__ __launch_bounds__(512) void force_kernel(
 #pragma unroll
       for (int k = 0; k < 8; ++k) acc[k] = 0.f;
       gather_edges<U, MODE>(sd, src, xd, r, e0, cnt, G, NG, g, lane, acc);
-      finish_row<MODE>(sd, lds, has_neg && t < rep, t, xr0, xd, acc, G, NG, g, q);
+      finish_row<MODE>(sd, out, lds, has_neg && t < rep, t, xr0, xd, acc, G, NG, g, q);
     }
   }
   (void)ipw;
 }
 
 // X[batch] = S (cpp/algo/algo.hpp:651-666 with the add already folded into
-// S) and the BF16 shadow refresh of the same rows.
+// S) and the BF16 shadow refresh of the same rows, at the batch's slot in
+// the layout at rest.
 __global__ __launch_bounds__(256) void apply_kernel(
     const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
     int n_steps) {
@@ -683,8 +674,10 @@ __global__ __launch_bounds__(256) void apply_kernel(
   const StepDesc &sd = steps[si];
   const int lb = b - blk_prefix[si];
   const int nb = blk_prefix[si + 1] - blk_prefix[si];
+  int sl = sd.batch - sd.state[1];
+  sl += sl < 0 ? sd.lay.P : 0;
   const int64_t total4 = (int64_t)sd.rows * sd.D / 4;
-  const int64_t base4 = (int64_t)sd.row_begin * sd.D / 4;
+  const int64_t base4 = (int64_t)sl * sd.lay.B * sd.D / 4;
   const float4 *S4 = reinterpret_cast<const float4 *>(sd.S);
   float4 *X4 = reinterpret_cast<float4 *>(sd.Xw);
   for (int64_t idx = (int64_t)lb * blockDim.x + threadIdx.x; idx < total4;
@@ -700,16 +693,31 @@ __global__ __launch_bounds__(256) void apply_kernel(
   }
 }
 
-// Epoch counters of `n` jobs (one block each): set to `value` or advanced by
-// one, then each job's current negative ids negcur[b*ns + k] = list
-// (epoch + b), id k (0 where no such list exists) are refreshed, so the
-// force launches read them without waiting for the counter.
-__global__ void epoch_kernel(const EpochDesc *__restrict__ jobs, int set, int value) {
+// Epoch state of `n` jobs (one block each): set (epoch := value, rot :=
+// rot_value) or advance (epoch += 1), keeping shift = (rot + epoch) mod P;
+// then each job's current negative ids negcur[b*ns + k] = list (epoch + b),
+// id k (0 where no such list exists) are refreshed, so the force launches
+// read them without waiting for the counter.
+__global__ void epoch_kernel(const EpochDesc *__restrict__ jobs, int set, int value,
+                             int rot_value) {
   const EpochDesc ed = jobs[blockIdx.x];
-  const int old = *ed.epoch;
+  const int old_epoch = ed.state[0];
+  const int old_shift = ed.state[2];
   __syncthreads();
-  const int e = set ? value : old + 1;
-  if (threadIdx.x == 0) *ed.epoch = e;
+  int e, sh;
+  if (set) {
+    e = value;
+    sh = (rot_value + value) % ed.P;
+    sh += sh < 0 ? ed.P : 0;
+  } else {
+    e = old_epoch + 1;
+    sh = old_shift + 1 == ed.P ? 0 : old_shift + 1;
+  }
+  if (threadIdx.x == 0) {
+    ed.state[0] = e;
+    if (set) ed.state[1] = rot_value;
+    ed.state[2] = sh;
+  }
   const int total = ed.batches * ed.ns;
   for (int i = threadIdx.x; i < total; i += blockDim.x) {
     const int b = i / ed.ns;
@@ -768,6 +776,7 @@ __global__ __launch_bounds__(256) void loss_edges_kernel(LossDesc ld, double *bl
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int wpb = blockDim.x >> 6;
   const int g = lane / G, q = lane & (G - 1);
+  const SlotMap mp = make_map(ld.lay, ld.rot, 0);
   double tot = 0.0;
   for (int i = 0; i < ipw; ++i) {
     const int it = (blockIdx.x * ipw + i) * wpb + wave;
@@ -776,7 +785,7 @@ __global__ __launch_bounds__(256) void loss_edges_kernel(LossDesc ld, double *bl
     const int e0 = ld.items[kItemInts * it + 1];
     const int cnt = ld.items[kItemInts * it + 2];
     float xr[8];
-    load8_f32(ld.X + (size_t)r * D + 8 * q, xr);
+    load8_f32(ld.X + (size_t)row_pos(mp, r) * D + 8 * q, xr);
     for (int eb = 0; eb < cnt; eb += 64) {
       const int m = min(64, cnt - eb);
       const int cl = lane < m ? ld.col[e0 + eb + lane] : 0;
@@ -786,7 +795,7 @@ __global__ __launch_bounds__(256) void loss_edges_kernel(LossDesc ld, double *bl
         const int c = __shfl(cl, k & 63);
         const bool ok = (k < m) && (c < ld.c_limit);
         float xc[8];
-        load8_f32(ld.X + (size_t)(ok ? c : r) * D + 8 * q, xc);
+        load8_f32(ld.X + (size_t)row_pos(mp, ok ? c : r) * D + 8 * q, xc);
         float p = 0.f;
 #pragma unroll
         for (int k2 = 0; k2 < 8; ++k2) {
@@ -809,6 +818,7 @@ __global__ __launch_bounds__(256) void loss_negs_kernel(LossDesc ld, double *blo
   const int g = lane / G, q = lane & (G - 1);
   const int64_t group_id = ((int64_t)blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) * NG + g;
   const int64_t total = ld.n * (int64_t)ld.eval_ns;
+  const SlotMap mp = make_map(ld.lay, ld.rot, 0);
   double tot = 0.0;
   const double eps = 1e-6;
   for (int i = 0; i < pairs_per_group; ++i) {
@@ -818,8 +828,8 @@ __global__ __launch_bounds__(256) void loss_negs_kernel(LossDesc ld, double *blo
     const int k = ok ? (int)(p - r * ld.eval_ns) : 0;
     const int64_t m = ok ? eval_negative(ld.eval_seed, r, k, ld.n) : 0;
     float xr[8], xm[8];
-    load8_f32(ld.X + (size_t)r * D + 8 * q, xr);
-    load8_f32(ld.X + (size_t)m * D + 8 * q, xm);
+    load8_f32(ld.X + (size_t)row_pos(mp, (uint32_t)r) * D + 8 * q, xr);
+    load8_f32(ld.X + (size_t)row_pos(mp, (uint32_t)m) * D + 8 * q, xm);
     float s2 = 0.f;
 #pragma unroll
     for (int k2 = 0; k2 < 8; ++k2) {
@@ -887,8 +897,9 @@ hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_step
   return hipGetLastError();
 }
 
-hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, hipStream_t st) {
-  hipLaunchKernelGGL(epoch_kernel, dim3(n), dim3(256), 0, st, jobs, set, value);
+hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, int rot,
+                        hipStream_t st) {
+  hipLaunchKernelGGL(epoch_kernel, dim3(n), dim3(256), 0, st, jobs, set, value, rot);
   return hipGetLastError();
 }
 
