@@ -87,12 +87,16 @@ def run_form(api, g, X0, lists, gather, args, tuning=None):
         ctx.set_negatives(lists)
         ctx.configure(B, lr, 5.0)
         if tuning:
-            ctx.set_tuning(**tuning)
+            ctx.set_tuning(**tuning.get("tuning", {}))
+            if tuning.get("per_batch"):
+                ctx.set_batch_tunings(tuning["per_batch"])
         elif not args.no_autotune:
             best, best_ms, table = ctx.autotune(epochs=2, search_chunking=True)
             log(f"autotune ({'bf16' if gather else 'fp32'}): {best} {best_ms:.3f} ms/epoch "
                 f"over {len(table)} trials")
         tun = ctx.get_tuning()
+        per_batch = [[t[k] for k in ("unroll", "waves_per_block", "items_per_wave", "chunk",
+                                     "max_chunks", "light")] for t in ctx.batch_tunings()]
         edges, n0, n1 = ctx.epoch_units(ns)
         use_graph = not args.no_graph
         if args.warmup:
@@ -121,7 +125,8 @@ def run_form(api, g, X0, lists, gather, args, tuning=None):
         ctx.close()
     units = args.steps * edges + (n0 + (args.steps - 1) * n1 if args.steps else 0)
     return {"dt": dt, "dt_events": dt_events, "units": units, "edges": edges, "kt": kt,
-            "tuning": tun, "loss": attr + rep, "loss_parts": (attr, rep), "use_graph": use_graph}
+            "tuning": {"tuning": tun, "per_batch": per_batch}, "loss": attr + rep,
+            "loss_parts": (attr, rep), "use_graph": use_graph}
 
 
 def main() -> int:
@@ -140,7 +145,10 @@ def main() -> int:
     ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
-    ap.add_argument("--tuning", type=str, default="", help="json dict of fe_tuning fields")
+    ap.add_argument("--tuning", type=str, default="",
+                    help='json {"tuning": {fe_tuning fields}, "per_batch": [[unroll, '
+                         'waves_per_block, items_per_wave, chunk, max_chunks, light] per '
+                         'batch]} (as reported in config.tuning); skips the autotuner')
     ap.add_argument("--form", choices=["plain", "precision", "batching", "serving"],
                     default="plain")
     ap.add_argument("--epochs", type=int, default=50, help="epochs of the batching/serving jobs")

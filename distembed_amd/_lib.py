@@ -90,6 +90,8 @@ SIGNATURES = {
     "fe_kernel_times": (C.c_int, [_P, _DP, _DP, _DP, _I64P]),
     "fe_autotune": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(fe_tuning), _DP, C.c_int,
                               C.POINTER(C.c_int), C.POINTER(fe_tuning), _DP]),
+    "fe_batch_tuning": (C.c_int, [_P, C.c_int, C.POINTER(fe_tuning)]),
+    "fe_set_batch_tuning": (C.c_int, [_P, C.c_int, C.POINTER(fe_tuning)]),
     "fe_reference_negatives": (C.c_int, [C.c_int64, C.c_int, C.c_int, C.c_int, _I32P, _I64P]),
     "fe_eval_negative": (C.c_int64, [C.c_uint64, C.c_int64, C.c_int, C.c_int64]),
     "fe_gen_rmat": (C.c_int, [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double,

@@ -231,6 +231,26 @@ class ForceEmbed:
         table = [(as_dict(res[i]), ms[i]) for i in range(min(n.value, cap))]
         return as_dict(best), best_ms.value, table
 
+    def batch_tunings(self) -> list:
+        """The tuning of every batch's force launch (per-batch shapes after
+        autotune)."""
+        out = []
+        b = 0
+        while True:
+            t = fe_tuning()
+            if self._lib.fe_batch_tuning(self._ctx, b, C.byref(t)) != _lib.FE_OK:
+                break
+            out.append({k: getattr(t, k) for k, _ in fe_tuning._fields_})
+            b += 1
+        return out
+
+    def set_batch_tunings(self, per_batch) -> None:
+        """Per-batch tunings: one (unroll, waves_per_block, items_per_wave,
+        chunk, max_chunks, light) per batch."""
+        for b, (u, w, i, ch, mc, li) in enumerate(per_batch):
+            t = fe_tuning(int(ch), int(mc), int(u), int(w), int(i), -1 if int(li) == 0 else int(li))
+            self._check(self._lib.fe_set_batch_tuning(self._ctx, b, C.byref(t)))
+
     def set_timing(self, enable: bool) -> None:
         self._check(self._lib.fe_set_timing(self._ctx, 1 if enable else 0))
 

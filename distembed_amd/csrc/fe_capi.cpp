@@ -23,6 +23,9 @@ struct fe_ctx {
   size_t events_used = 0;
   double t_force = 0;
   int64_t n_timed = 0;
+  // per-batch force times of replayed epochs (event pair p = batch p)
+  std::vector<double> t_batch;
+  std::vector<int64_t> n_batch;
   std::string err;
 };
 
@@ -183,6 +186,11 @@ static fe_status harvest_range(fe_ctx *c, size_t begin, size_t end) {
     }
     c->t_force += a;
     c->n_timed += 1;
+    const size_t p = (i - begin) / 2;
+    if (p < c->t_batch.size()) {
+      c->t_batch[p] += a;
+      c->n_batch[p] += 1;
+    }
   }
   return FE_OK;
 }
@@ -356,11 +364,53 @@ extern "C" fe_status fe_train(fe_ctx *c, int iterations, int batch_size, int ns,
 
 // ---- autotuner ------------------------------------------------------------
 
-// Times `epochs` epochs of the captured chain for every candidate tiling and
-// keeps the fastest. X (and the shadow) are restored after every trial, so
-// the call leaves the embedding as it found it. Unroll, waves per block and
-// items per wave never change results; chunk and max_chunks change the
-// summation order of heavy rows and are only searched when allowed.
+// Times one tiling: a warm-up epoch (captures the graph), then `epochs`
+// replays. per_batch: every force launch is timed with an event pair and
+// tb[b] gets batch b's mean launch time; otherwise only the whole epochs are
+// timed. Returns ms per epoch (per_batch: the sum of the batches' means).
+static fe_status time_tiling(fe_ctx *c, int epochs, bool per_batch, double *ms_epoch,
+                             std::vector<double> *tb, hipEvent_t e0, hipEvent_t e1) {
+  Job &j = c->job;
+  drop_graph(c);
+  c->timing = per_batch;
+  fe_status s = fe_train(c, 1, j.B, j.ns, j.lr, 1);
+  if (s != FE_OK) return s;
+  const int nb = j.plan.batches;
+  c->t_batch.assign(per_batch ? nb : 0, 0.0);
+  c->n_batch.assign(per_batch ? nb : 0, 0);
+  if ((s = hip_ok(c, hipEventRecord(e0, c->stream), "event")) != FE_OK) return s;
+  if ((s = fe_train(c, epochs, j.B, j.ns, j.lr, 1)) != FE_OK) return s;
+  if ((s = hip_ok(c, hipEventRecord(e1, c->stream), "event")) != FE_OK) return s;
+  if ((s = hip_ok(c, hipEventSynchronize(e1), "event sync")) != FE_OK) return s;
+  if (per_batch) {
+    double sum = 0;
+    tb->assign(nb, 0.0);
+    for (int b = 0; b < nb; ++b) {
+      (*tb)[b] = c->n_batch[b] ? c->t_batch[b] / c->n_batch[b] : 0.0;
+      sum += (*tb)[b];
+    }
+    *ms_epoch = sum;
+  } else {
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, e0, e1);
+    *ms_epoch = ms / epochs;
+  }
+  c->t_batch.clear();
+  c->n_batch.clear();
+  return FE_OK;
+}
+
+// Searches, per batch, the launch shape and (optionally) the cutting of rows:
+//   1. every shape (unroll x waves per block x items per wave) on every
+//      batch at once, each force launch timed: each batch keeps its fastest;
+//   2. with those shapes, every cutting (chunk x max_chunks x light), each
+//      force launch timed: each batch keeps its fastest;
+//   3. step 1 again with those cuttings.
+// A batch's launch time depends only on its own shape and cutting, so one
+// trial serves every batch. X is restored after every trial (from rotation
+// 0), so the call leaves the embedding as it found it. Shapes never change
+// results; the cutting changes the summation order of heavy rows and is
+// only searched when allowed.
 extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_tuning *results,
                                  double *results_ms, int max_results, int *n_results,
                                  fe_tuning *best, double *best_ms) {
@@ -371,41 +421,19 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
   if (s != FE_OK) return s;
   if (j.ns > 0 && j.n_lists < epochs + j.plan.batches - 1)
     return fail(c, FE_ERR_STATE, "fe_autotune: set negative lists covering the trial epochs");
-  const Tuning orig = j.tun;
-  // staged search: launch shape, then (optionally) the cutting of rows at
-  // the best shape, then the launch shape again at the best cutting
-  auto shapes = [](const Tuning &base) {
-    std::vector<Tuning> v;
-    for (int u : {1, 2, 4})
-      for (int w : {2, 4, 8})
-        for (int ipw : {1, 2, 4}) {
-          Tuning t = base;
-          t.unroll = u;
-          t.waves_per_block = w;
-          t.items_per_wave = ipw;
-          v.push_back(t);
-        }
-    return v;
-  };
-  auto cuttings = [](const Tuning &base) {
-    std::vector<Tuning> v;
-    for (int ch : {32, 64})
-      for (int mc : {64, 256, 1024})
-        for (int li : {0, 8, 16}) {
-          Tuning t = base;
-          t.chunk = ch;
-          t.max_chunks = mc;
-          t.light = li;
-          v.push_back(t);
-        }
-    return v;
-  };
-  const int stages = search_chunking ? 3 : 1;
-  std::vector<Tuning> cands = shapes(orig);
-  size_t stage_end = cands.size();
-  int stage = 1;
-  // trials start from rotation 0 and the rows are restored there after each
   if ((s = from_job(c, job_normalize_layout(j, c->stream))) != FE_OK) return s;
+  const Tuning orig = j.tun;
+  const std::vector<Shape> orig_shapes = j.bshape;
+  const std::vector<Cut> orig_cuts = j.bcut;
+  const int nb = j.plan.batches;
+  std::vector<Shape> shapes;
+  for (int u : {1, 2, 4})
+    for (int w : {2, 4, 8})
+      for (int ipw : {1, 2, 4}) shapes.push_back(Shape{u, w, ipw});
+  std::vector<Cut> cuttings;
+  for (int ch : {16, 32, 64})
+    for (int mc : {64, 256, 1024})
+      for (int li : {0, 8, 16}) cuttings.push_back(Cut{ch, mc, li});
   const size_t xbytes = sizeof(float) * (size_t)j.n * j.d;
   void *backup = nullptr;
   if ((s = hip_ok(c, hipMalloc(&backup, xbytes), "fe_autotune backup")) != FE_OK) return s;
@@ -414,57 +442,174 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
   hipEventCreate(&e1);
   s = hip_ok(c, hipMemcpyAsync(backup, j.X, xbytes, hipMemcpyDeviceToDevice, c->stream), "backup");
   const bool was_timing = c->timing;
-  c->timing = false;
-  double best_t = 1e300;
-  Tuning best_tun = orig;
   int n = 0;
-  for (size_t i = 0; s == FE_OK && i < cands.size(); ++i) {
-    drop_graph(c);
-    if ((s = from_job(c, job_set_tuning(j, cands[i]))) != FE_OK) break;
-    if ((s = from_job(c, job_prepare(j, c->stream))) != FE_OK) break;
-    // one untimed epoch captures the graphs and warms the caches
-    if ((s = fe_train(c, 1, j.B, j.ns, j.lr, 1)) != FE_OK) break;
-    if ((s = hip_ok(c, hipEventRecord(e0, c->stream), "event")) != FE_OK) break;
-    if ((s = fe_train(c, epochs, j.B, j.ns, j.lr, 1)) != FE_OK) break;
-    if ((s = hip_ok(c, hipEventRecord(e1, c->stream), "event")) != FE_OK) break;
-    if ((s = hip_ok(c, hipEventSynchronize(e1), "event sync")) != FE_OK) break;
-    float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
-    const double per_epoch = ms / epochs;
+  auto restore = [&]() -> fe_status {
+    fe_status r = hip_ok(c, hipMemcpyAsync(j.X, backup, xbytes, hipMemcpyDeviceToDevice, c->stream),
+                         "restore");
+    j.rot = 0;
+    if (r == FE_OK && j.dtype == FE_GATHER_BF16)
+      r = from_job(c, job_check(j, launch_to_bf16(j.X, j.Xs, (int64_t)j.n * j.d, c->stream),
+                                "shadow"));
+    return r;
+  };
+  auto record = [&](const Shape &sh, const Cut &ct, double ms) {
     if (results && n < max_results) {
-      results[n] = fe_tuning{cands[i].chunk, cands[i].max_chunks, cands[i].unroll,
-                             cands[i].waves_per_block, cands[i].items_per_wave, cands[i].light};
-      if (results_ms) results_ms[n] = per_epoch;
+      results[n] = fe_tuning{ct.chunk, ct.max_chunks, sh.unroll, sh.waves_per_block,
+                             sh.items_per_wave, ct.light};
+      if (results_ms) results_ms[n] = ms;
     }
     ++n;
-    if (per_epoch < best_t) {
-      best_t = per_epoch;
-      best_tun = cands[i];
+  };
+  // the value most batches chose becomes the tuning's own field
+  auto most_common_shape = [](const std::vector<Shape> &v, Shape dflt) {
+    Shape m = dflt;
+    int bestc = 0;
+    for (const Shape &a : v) {
+      int k = 0;
+      for (const Shape &b : v)
+        k += a.unroll == b.unroll && a.waves_per_block == b.waves_per_block &&
+             a.items_per_wave == b.items_per_wave;
+      if (k > bestc) {
+        bestc = k;
+        m = a;
+      }
     }
-    s = hip_ok(c, hipMemcpyAsync(j.X, backup, xbytes, hipMemcpyDeviceToDevice, c->stream), "restore");
-    j.rot = 0;
-    if (s == FE_OK && j.dtype == FE_GATHER_BF16)
-      s = from_job(c, job_check(j, launch_to_bf16(j.X, j.Xs, (int64_t)j.n * j.d, c->stream), "shadow"));
-    if (i + 1 == stage_end && stage < stages) {
-      const std::vector<Tuning> next = (stage == 1) ? cuttings(best_tun) : shapes(best_tun);
-      cands.insert(cands.end(), next.begin(), next.end());
-      stage_end = cands.size();
-      ++stage;
+    return m;
+  };
+  auto most_common_cut = [](const std::vector<Cut> &v, Cut dflt) {
+    Cut m = dflt;
+    int bestc = 0;
+    for (const Cut &a : v) {
+      int k = 0;
+      for (const Cut &b : v)
+        k += a.chunk == b.chunk && a.max_chunks == b.max_chunks && a.light == b.light;
+      if (k > bestc) {
+        bestc = k;
+        m = a;
+      }
     }
+    return m;
+  };
+  // applies per-batch shapes and cuttings (the tuning holds the common ones)
+  auto apply = [&](const std::vector<Shape> &vs, const std::vector<Cut> &vc) -> fe_status {
+    const Shape ms = most_common_shape(vs, Shape{orig.unroll, orig.waves_per_block,
+                                                 orig.items_per_wave});
+    const Cut mc = most_common_cut(vc, Cut{orig.chunk, orig.max_chunks, orig.light});
+    Tuning t = orig;
+    t.unroll = ms.unroll;
+    t.waves_per_block = ms.waves_per_block;
+    t.items_per_wave = ms.items_per_wave;
+    t.chunk = mc.chunk;
+    t.max_chunks = mc.max_chunks;
+    t.light = mc.light;
+    fe_status r = from_job(c, job_set_tuning(j, t));
+    if (r == FE_OK) r = from_job(c, job_set_shapes(j, vs));
+    if (r == FE_OK) r = from_job(c, job_set_cuts(j, vc));
+    return r;
+  };
+  std::vector<Shape> bs(nb, Shape{orig.unroll, orig.waves_per_block, orig.items_per_wave});
+  std::vector<Cut> bc(nb);
+  for (int b = 0; b < nb; ++b) bc[b] = job_cut(j, b);
+  std::vector<double> bt(nb, 1e300), tb;
+  // one trial: every batch gets (shape or cut); batches that run faster keep it
+  auto trial = [&](const std::vector<Shape> &vs, const std::vector<Cut> &vc, bool take_shape,
+                   const Shape &sh, const Cut &ct) -> fe_status {
+    double ms = 0;
+    fe_status r = apply(vs, vc);
+    if (r == FE_OK) r = time_tiling(c, epochs, true, &ms, &tb, e0, e1);
+    if (r == FE_OK) r = restore();
+    if (r != FE_OK) return r;
+    record(sh, ct, ms);
+    for (int b = 0; b < nb && b < (int)tb.size(); ++b) {
+      if (tb[b] < bt[b]) {
+        bt[b] = tb[b];
+        if (take_shape) bs[b] = sh;
+        else bc[b] = ct;
+      }
+    }
+    return FE_OK;
+  };
+  auto shape_stage = [&]() -> fe_status {
+    std::fill(bt.begin(), bt.end(), 1e300);
+    const std::vector<Cut> vc = bc;
+    for (const Shape &sh : shapes) {
+      fe_status r = trial(std::vector<Shape>(nb, sh), vc, true, sh, most_common_cut(vc, bc[0]));
+      if (r != FE_OK) return r;
+    }
+    return FE_OK;
+  };
+  auto cut_stage = [&]() -> fe_status {
+    std::fill(bt.begin(), bt.end(), 1e300);
+    const std::vector<Shape> vs = bs;
+    for (const Cut &ct : cuttings) {
+      fe_status r = trial(vs, std::vector<Cut>(nb, ct), false, most_common_shape(vs, bs[0]), ct);
+      if (r != FE_OK) return r;
+    }
+    return FE_OK;
+  };
+  double best_t = 0;
+  if (s == FE_OK) s = shape_stage();
+  if (s == FE_OK && search_chunking) {
+    s = cut_stage();
+    if (s == FE_OK) s = shape_stage();
   }
+  if (s == FE_OK) s = apply(bs, bc);
+  if (s == FE_OK) s = time_tiling(c, epochs, false, &best_t, nullptr, e0, e1);
+  if (s == FE_OK) s = restore();
   c->timing = was_timing;
   drop_graph(c);
-  fe_status s2 = from_job(c, job_set_tuning(j, s == FE_OK ? best_tun : orig));
-  if (s == FE_OK) s = s2;
+  if (s != FE_OK) {
+    job_set_tuning(j, orig);
+    job_set_shapes(j, orig_shapes);
+    job_set_cuts(j, orig_cuts);
+  }
   hipStreamSynchronize(c->stream);
   hipFree(backup);
   hipEventDestroy(e0);
   hipEventDestroy(e1);
   if (s != FE_OK) return s;
   if (n_results) *n_results = n;
-  if (best) *best = fe_tuning{best_tun.chunk, best_tun.max_chunks, best_tun.unroll,
-                              best_tun.waves_per_block, best_tun.items_per_wave, best_tun.light};
+  if (best) *best = fe_tuning{j.tun.chunk, j.tun.max_chunks, j.tun.unroll,
+                              j.tun.waves_per_block, j.tun.items_per_wave, j.tun.light};
   if (best_ms) *best_ms = best_t;
+  return FE_OK;
+}
+
+extern "C" fe_status fe_set_batch_tuning(fe_ctx *c, int batch, const fe_tuning *t) {
+  CTX_GUARD(c);
+  if (!t) return fail(c, FE_ERR_INVALID_ARG, "fe_set_batch_tuning: NULL tuning");
+  Job &j = c->job;
+  fe_status s = from_job(c, job_prepare(j, c->stream));
+  if (s != FE_OK) return s;
+  const int nb = j.plan.batches;
+  if (batch < 0 || batch >= nb)
+    return fail(c, FE_ERR_INVALID_ARG, "fe_set_batch_tuning: batch out of range");
+  std::vector<Shape> v(nb);
+  std::vector<Cut> w(nb);
+  for (int b = 0; b < nb; ++b) {
+    v[b] = job_shape(j, b);
+    w[b] = job_cut(j, b);
+  }
+  v[batch] = Shape{t->unroll ? t->unroll : v[batch].unroll,
+                   t->waves_per_block ? t->waves_per_block : v[batch].waves_per_block,
+                   t->items_per_wave ? t->items_per_wave : v[batch].items_per_wave};
+  w[batch] = Cut{t->chunk ? t->chunk : w[batch].chunk,
+                 t->max_chunks ? t->max_chunks : w[batch].max_chunks,
+                 t->light ? (t->light < 0 ? 0 : t->light) : w[batch].light};
+  drop_graph(c);
+  if ((s = from_job(c, job_set_shapes(j, v))) != FE_OK) return s;
+  return from_job(c, job_set_cuts(j, w));
+}
+
+extern "C" fe_status fe_batch_tuning(const fe_ctx *c, int batch, fe_tuning *out) {
+  if (!c || !out) return FE_ERR_INVALID_ARG;
+  const Job &j = c->job;
+  if (j.plan.batches == 0) return FE_ERR_STATE;
+  if (batch < 0 || batch >= j.plan.batches) return FE_ERR_INVALID_ARG;
+  const Shape sh = job_shape(j, batch);
+  const Cut ct = job_cut(j, batch);
+  *out = fe_tuning{ct.chunk, ct.max_chunks, sh.unroll, sh.waves_per_block, sh.items_per_wave,
+                   ct.light};
   return FE_OK;
 }
 
@@ -513,6 +658,8 @@ extern "C" fe_status fe_set_timing(fe_ctx *c, int enable) {
   c->timing = enable != 0;
   c->t_force = 0;
   c->n_timed = 0;
+  std::fill(c->t_batch.begin(), c->t_batch.end(), 0.0);
+  std::fill(c->n_batch.begin(), c->n_batch.end(), 0);
   c->events_used = 0;
   return FE_OK;
 }

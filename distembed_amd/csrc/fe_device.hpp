@@ -30,6 +30,8 @@ struct Layout {
   int div_sh;
   int B;
   int P;
+  int b_shift;    // log2(B) when B is a power of two (k = c >> b_shift), else -1
+  int pad;
 };
 
 // One minibatch step of one job (graph). Everything here is invariant across

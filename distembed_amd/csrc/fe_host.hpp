@@ -46,8 +46,17 @@ struct Plan {
   int64_t edges_eff_total = 0;
 };
 
+// Cutting of one batch's rows (chunk, max_chunks, light as above).
+struct Cut {
+  int chunk;
+  int max_chunks;
+  int light;
+};
+
+// cuts: one per batch, or null for chunk / max_chunks / light everywhere.
 fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int B, int chunk,
-                     int max_chunks, int light, int pack, int64_t c_limit, Plan *out);
+                     int max_chunks, int light, int pack, int64_t c_limit, Plan *out,
+                     const Cut *cuts = nullptr);
 
 // Repulsive row counts of batch j in epoch 0 and in later epochs
 // (cpp/algo/algo.hpp:135,167-169,594).

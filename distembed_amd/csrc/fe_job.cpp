@@ -93,6 +93,8 @@ static Layout make_layout(int B, int batches) {
   L.div_sh = 31 + l;
   L.B = B;
   L.P = batches + 1;
+  L.b_shift = (B & (B - 1)) == 0 ? l : -1;
+  L.pad = 0;
   return L;
 }
 
@@ -349,6 +351,46 @@ fe_status job_set_tuning(Job &j, const Tuning &t) {
   if (t.chunk != j.tun.chunk || t.max_chunks != j.tun.max_chunks || t.light != j.tun.light)
     j.plan_dirty = true;
   j.tun = t;
+  j.bshape.clear();
+  if (!j.bcut.empty()) j.plan_dirty = true;
+  j.bcut.clear();
+  j.steps_dirty = true;
+  return FE_OK;
+}
+
+Cut job_cut(const Job &j, int b) {
+  if (b >= 0 && b < (int)j.bcut.size()) return j.bcut[b];
+  return Cut{j.tun.chunk, j.tun.max_chunks, j.tun.light};
+}
+
+fe_status job_set_cuts(Job &j, const std::vector<Cut> &cuts) {
+  for (const Cut &c : cuts) {
+    if (c.chunk < 1 || c.chunk > 64 || c.max_chunks < 1 || c.light < 0 || c.light > 64) {
+      j.err = "batch cutting out of range";
+      return FE_ERR_INVALID_ARG;
+    }
+  }
+  j.bcut = cuts;
+  j.plan_dirty = true;
+  j.steps_dirty = true;
+  return FE_OK;
+}
+
+Shape job_shape(const Job &j, int b) {
+  if (b >= 0 && b < (int)j.bshape.size()) return j.bshape[b];
+  return Shape{j.tun.unroll, j.tun.waves_per_block, j.tun.items_per_wave};
+}
+
+fe_status job_set_shapes(Job &j, const std::vector<Shape> &shapes) {
+  auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
+  for (const Shape &sh : shapes) {
+    if (!pow2(sh.unroll) || sh.unroll > 8 || !pow2(sh.waves_per_block) ||
+        sh.waves_per_block > 8 || sh.items_per_wave < 1 || sh.items_per_wave > 64) {
+      j.err = "batch launch shape out of range";
+      return FE_ERR_INVALID_ARG;
+    }
+  }
+  j.bshape = shapes;
   j.steps_dirty = true;
   return FE_OK;
 }
@@ -420,8 +462,11 @@ fe_status job_prepare(Job &j, hipStream_t st, bool slots) {
   }
   if (j.plan_dirty) {
     Plan p;
+    const int nbp = (int)((j.n + j.B - 1) / j.B);
+    const bool per_batch = (int)j.bcut.size() == nbp;
     fe_status s = build_plan(j.n, j.row_ptr.data(), nullptr, j.B, j.tun.chunk,
-                             j.tun.max_chunks, j.tun.light, 64 / j.G, j.c_limit, &p);
+                             j.tun.max_chunks, j.tun.light, 64 / j.G, j.c_limit, &p,
+                             per_batch ? j.bcut.data() : nullptr);
     if (s != FE_OK) {
       j.err = host_error();
       return s;
@@ -496,8 +541,9 @@ fe_status job_prepare(Job &j, hipStream_t st, bool slots) {
     std::vector<int> pf((size_t)nb * 4, 0);
     j.force_blocks.assign(nb, 0);
     j.apply_blocks.assign(nb, 0);
-    const int ipb = j.tun.waves_per_block * j.tun.items_per_wave;
     for (int b = 0; b < nb; ++b) {
+      const Shape sh = job_shape(j, b);
+      const int ipb = sh.waves_per_block * sh.items_per_wave;
       hd[b] = make_desc(j, b, DESC_FUSED);
       hd[nb + b] = make_desc(j, b, DESC_SINGLE);
       hd[2 * nb + b] = make_desc(j, b, DESC_DELTA);
@@ -541,8 +587,9 @@ fe_status job_tick(Job &j, hipStream_t st) {
 fe_status job_launch_force(Job &j, int b, DescKind kind, hipStream_t st, hipEvent_t *ev) {
   const size_t lds = force_lds_bytes(j.ns, j.d);
   if (ev) FE_HIP(j, record_event(ev[0], st), "event");
+  const Shape sh = job_shape(j, b);
   FE_HIP(j, launch_force(job_desc(j, b, kind), j.prefix + 4 * b, 1, j.force_blocks[b],
-                         j.tun.waves_per_block, j.tun.items_per_wave, j.tun.unroll, lds,
+                         sh.waves_per_block, sh.items_per_wave, sh.unroll, lds,
                          j.dtype == FE_GATHER_BF16 ? 1 : 0, j.G, st),
          "force kernel launch");
   if (ev) FE_HIP(j, record_event(ev[1], st), "event");

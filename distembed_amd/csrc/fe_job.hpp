@@ -40,6 +40,13 @@ struct Tuning {
   int light = 16;
 };
 
+// Launch shape of one batch's force launch (does not change results).
+struct Shape {
+  int unroll;
+  int waves_per_block;
+  int items_per_wave;
+};
+
 // Descriptor variants of one batch.
 //   FUSED: a step of a run's epoch chain: new rows go straight into the free
 //          slot of the layout (see Layout in fe_device.hpp).
@@ -76,6 +83,8 @@ struct Job {
   int ns = 0;
   int n_lists = 0;
   Tuning tun;
+  std::vector<Shape> bshape; // per batch (from the autotuner); empty: tun's shape
+  std::vector<Cut> bcut;     // per batch cutting; empty: tun's chunk / max_chunks / light
   Plan plan;
   std::vector<int> force_blocks; // per batch
   std::vector<int> apply_blocks; // per batch
@@ -115,6 +124,12 @@ fe_status job_download_x(Job &j, float *x, hipStream_t st);
 fe_status job_set_negatives(Job &j, const int32_t *lists, int n_lists, int ns, hipStream_t st);
 fe_status job_configure(Job &j, int B, float lr, float bound);
 fe_status job_set_tuning(Job &j, const Tuning &t);
+// Launch shape and cutting of batch b; per-batch shapes and cuttings
+// (both cleared by job_set_tuning).
+Shape job_shape(const Job &j, int b);
+Cut job_cut(const Job &j, int b);
+fe_status job_set_shapes(Job &j, const std::vector<Shape> &shapes);
+fe_status job_set_cuts(Job &j, const std::vector<Cut> &cuts);
 // Builds the plan / descriptors if dirty and uploads them. slots: X (and the
 // shadow) must hold every slot of the layout (fused runs); otherwise only
 // the n rows of the layout at rest are needed.

@@ -145,26 +145,41 @@ template <int MODE> __device__ __forceinline__ bool use_bf16(const StepDesc &sd)
 }
 
 // Slot positions of the rows one step reads (see Layout): batch k of row c
-// is in slot (k - shift - [k < jcut]) mod P.
+// is in slot (k - shift - [k < jcut]) mod P = t or t - P, where
+// t = k + (P - shift - 1) + [k >= jcut] lies in [0, 2P).
 struct SlotMap {
   uint32_t div_m;
   int div_sh;
   int B;
   int P;
-  int shift;
+  int b_shift;
+  int kadd;    // P - shift - 1
   int jcut;
 };
 
 __device__ __forceinline__ SlotMap make_map(const Layout &lay, int shift, int jcut) {
-  return SlotMap{lay.div_m, lay.div_sh, lay.B, lay.P, shift, jcut};
+  return SlotMap{lay.div_m, lay.div_sh, lay.B, lay.P, lay.b_shift, lay.P - shift - 1, jcut};
 }
 
-// Position (row index in the slot layout) of row c.
+// Position (row index in the slot layout) of row c. P2: the batch size is a
+// power of two (shifts only); otherwise one multiply-high division.
+template <bool P2>
+__device__ __forceinline__ uint32_t row_pos_t(const SlotMap &m, uint32_t c) {
+  if constexpr (P2) {
+    const int k = (int)(c >> m.b_shift);
+    int t = k + m.kadd + (k >= m.jcut ? 1 : 0);
+    t -= t >= m.P ? m.P : 0;
+    return c + ((uint32_t)(t - k) << m.b_shift);
+  } else {
+    const int k = (int)(uint32_t)(((uint64_t)c * m.div_m) >> m.div_sh);
+    int t = k + m.kadd + (k >= m.jcut ? 1 : 0);
+    t -= t >= m.P ? m.P : 0;
+    return c + (uint32_t)(t - k) * (uint32_t)m.B;
+  }
+}
+
 __device__ __forceinline__ uint32_t row_pos(const SlotMap &m, uint32_t c) {
-  const int k = (int)(uint32_t)(((uint64_t)c * m.div_m) >> m.div_sh);
-  int sl = k - m.shift - (k < m.jcut ? 1 : 0);
-  sl += sl < 0 ? m.P : 0;
-  return c + (uint32_t)(sl - k) * (uint32_t)m.B;
+  return m.b_shift >= 0 ? row_pos_t<true>(m, c) : row_pos_t<false>(m, c);
 }
 
 // Row c as 8 FP32 components [8q, 8q+8) (BF16 gathers: the shadow's values).
@@ -191,7 +206,7 @@ template <> struct RowBuf<1> {
 // only (no branch), so the U loads of a round issue back to back.
 struct RowSrc {
   const char *xq;      // X or Xs, plus this lane's slice offset
-  uint32_t row_bytes;  // bytes per gathered row
+  int row_shift;       // log2 of the bytes per gathered row (D is 8 x 2^i)
   SlotMap m;
   bool bf;             // gathers read BF16
 };
@@ -200,20 +215,21 @@ template <int MODE>
 __device__ __forceinline__ RowSrc make_src(const StepDesc &sd, const SlotMap &m, int q) {
   RowSrc r;
   r.bf = use_bf16<MODE>(sd);
+  const int dshift = 31 - __builtin_clz((unsigned)sd.D);
   if (r.bf) {
     r.xq = reinterpret_cast<const char *>(sd.Xs + 8 * q);
-    r.row_bytes = (uint32_t)sd.D * 2u;
+    r.row_shift = dshift + 1;
   } else {
     r.xq = reinterpret_cast<const char *>(sd.X + 8 * q);
-    r.row_bytes = (uint32_t)sd.D * 4u;
+    r.row_shift = dshift + 2;
   }
   r.m = m;
   return r;
 }
 
-template <int MODE>
+template <int MODE, bool P2>
 __device__ __forceinline__ void load_buf(const RowSrc &src, int c, RowBuf<MODE> &b) {
-  const char *p = src.xq + (uint64_t)row_pos(src.m, (uint32_t)c) * src.row_bytes;
+  const char *p = src.xq + ((uint64_t)row_pos_t<P2>(src.m, (uint32_t)c) << src.row_shift);
   if constexpr (MODE == 1) {
     b.w = *(gcu4 *)p;
   } else if constexpr (MODE == 0) {
@@ -271,6 +287,37 @@ __device__ __forceinline__ void interact(const float (&xd)[8], const float (&xc)
 #pragma unroll
   for (int k = 1; k < 8; ++k) p = fmaf(v[k], v[k], p);
   add_force<ATTR>(v, group_sum(p, G), lr, bound, acc);
+}
+
+// Attractive forces of U gathered slices (cpp/algo/algo.hpp:559-579) on
+// the row's accumulators, one slot after the other. While the clamp bound
+// is >= 2 it never binds: |v_k d1| = 2|v_k| / (1 + s) <= 1 because
+// s >= v_k^2, so scale() is the identity and each component takes one fused
+// multiply-add with lr*d1 (d1 from the hardware reciprocal). A non-finite
+// distance anywhere in the wave, or a bound below 2, takes the exact
+// expressions of add_force.
+template <int U, int MODE>
+__device__ __forceinline__ void attract_round(const float (&xd)[8], const RowBuf<MODE> (&xb)[U],
+                                              int G, float lr, float bound, float (&acc)[8]) {
+  const bool small_bound = bound < 2.0f;
+#pragma unroll
+  for (int uu = 0; uu < U; ++uu) {
+    float xc[8], v[8];
+    unpack_buf<MODE>(xb[uu], xc);
+#pragma unroll
+    for (int k = 0; k < 8; ++k) v[k] = xd[k] - xc[k];
+    float p = v[0] * v[0];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) p = fmaf(v[k], v[k], p);
+    const float sq = group_sum(p, G);
+    if (__builtin_amdgcn_ballot_w64(small_bound || !__builtin_isfinite(sq)) == 0) {
+      const float c = lr * (-2.0f * __builtin_amdgcn_rcpf(1.0f + sq));
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = fmaf(c, v[k], acc[k]);
+    } else {
+      add_force<true>(v, sq, lr, bound, acc);
+    }
+  }
 }
 
 // Staging piece p (negative p / G, slice p % G) of the step's negative rows,
@@ -348,8 +395,8 @@ __device__ __forceinline__ void sum_partials(const StepDesc &sd, int p0, int k, 
 // accumulators: group g takes edges g, g+NG, ... in order, U rows in flight.
 // A slot past the end, or holding a column the quirk skips (c >= c_limit),
 // reads row r itself, whose distance to itself is exactly zero: no force.
-template <int U, int MODE>
-__device__ __forceinline__ void gather_edges(const StepDesc &sd, const RowSrc &src,
+template <int U, int MODE, bool P2>
+__device__ __forceinline__ void gather_edges_t(const StepDesc &sd, const RowSrc &src,
                                              const float (&xd)[8], int r, int e0, int cnt,
                                              int G, int NG, int g, int lane, float (&acc)[8]) {
   const float lr = sd.lr, bound = sd.bound;
@@ -365,16 +412,11 @@ __device__ __forceinline__ void gather_edges(const StepDesc &sd, const RowSrc &s
       for (int uu = 0; uu < U; ++uu) {
         const int k = (u0 + uu) * NG + g;
         const int c = __shfl(cl, k & 63);
-        load_buf<MODE>(src, k < m ? c : r, xb[uu]);
+        load_buf<MODE, P2>(src, k < m ? c : r, xb[uu]);
       }
       // keep the U loads ahead of every use of their results
       __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int uu = 0; uu < U; ++uu) {
-        float xc[8];
-        unpack_buf<MODE>(xb[uu], xc);
-        interact<true>(xd, xc, G, lr, bound, acc);
-      }
+      attract_round<U, MODE>(xd, xb, G, lr, bound, acc);
     }
   }
 }
@@ -438,8 +480,8 @@ __device__ __forceinline__ void finish_row(const StepDesc &sd, uint32_t out, con
 // against every staged negative and writes its row. The loops run to the
 // pack's largest degree kmax, so control flow stays uniform; slots past a
 // row's degree read the row itself (no force).
-template <int U, int MODE>
-__device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src, uint32_t out,
+template <int U, int MODE, bool P2>
+__device__ __forceinline__ void light_pack_t(const StepDesc &sd, const RowSrc &src, uint32_t out,
                                            u4v ia, int kmax, bool valid, const float (&xr)[8],
                                            const float *lds_neg, bool has_neg, int rep, int G,
                                            int g, int q) {
@@ -461,15 +503,10 @@ __device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src
       for (int uu = 0; uu < U; ++uu) {
         const int k = k0 + uu;
         const int c = __shfl(cl, g * G + ((k - kb) & (G - 1)));
-        load_buf<MODE>(src, k < kend ? c : r, xb[uu]);
+        load_buf<MODE, P2>(src, k < kend ? c : r, xb[uu]);
       }
       __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int uu = 0; uu < U; ++uu) {
-        float xc[8];
-        unpack_buf<MODE>(xb[uu], xc);
-        interact<true>(xd, xc, G, lr, bound, acc);
-      }
+      attract_round<U, MODE>(xd, xb, G, lr, bound, acc);
     }
   }
   const int t = r - sd.row_begin;
@@ -497,6 +534,26 @@ __device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src
     for (int k = 0; k < 8; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
     write_row<MODE>(sd, out, t, q, o);
   }
+}
+
+// One uniform branch per work unit picks the power-of-two addressing.
+template <int U, int MODE>
+__device__ __forceinline__ void gather_edges(const StepDesc &sd, const RowSrc &src,
+                                             const float (&xd)[8], int r, int e0, int cnt,
+                                             int G, int NG, int g, int lane, float (&acc)[8]) {
+  if (src.m.b_shift >= 0) gather_edges_t<U, MODE, true>(sd, src, xd, r, e0, cnt, G, NG, g, lane, acc);
+  else gather_edges_t<U, MODE, false>(sd, src, xd, r, e0, cnt, G, NG, g, lane, acc);
+}
+
+template <int U, int MODE>
+__device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src, uint32_t out,
+                                           u4v ia, int kmax, bool valid, const float (&xr)[8],
+                                           const float *lds_neg, bool has_neg, int rep, int G,
+                                           int g, int q) {
+  if (src.m.b_shift >= 0)
+    light_pack_t<U, MODE, true>(sd, src, out, ia, kmax, valid, xr, lds_neg, has_neg, rep, G, g, q);
+  else
+    light_pack_t<U, MODE, false>(sd, src, out, ia, kmax, valid, xr, lds_neg, has_neg, rep, G, g, q);
 }
 
 // One minibatch of one or more jobs. Blocks [0, n_block_items) of a step

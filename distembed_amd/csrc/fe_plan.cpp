@@ -7,7 +7,8 @@
 namespace fe {
 
 fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int B, int chunk,
-                     int max_chunks, int light, int pack, int64_t c_limit, Plan *out) {
+                     int max_chunks, int light, int pack, int64_t c_limit, Plan *out,
+                     const Cut *cuts) {
   if (n <= 0 || !row_ptr || B <= 0 || chunk <= 0 || max_chunks <= 0 || light < 0 || pack < 1 ||
       !out) {
     set_host_error("build_plan: invalid argument");
@@ -34,10 +35,17 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
   p.pslots.assign(p.batches, 0);
   p.edges_eff.assign(p.batches, 0);
   p.items.reserve((size_t)kItemInts * (size_t)(n + (row_ptr[n] / chunk) + 16));
-  const int64_t block_edges = (int64_t)kSub * chunk;
   std::vector<int32_t> waves;
   std::vector<std::pair<int32_t, int32_t>> lights; // (degree, row)
   for (int j = 0; j < p.batches; ++j) {
+    const int chunk_j = cuts ? cuts[j].chunk : chunk;
+    const int max_chunks_j = cuts ? cuts[j].max_chunks : max_chunks;
+    const int light_j = std::min(cuts ? cuts[j].light : light, chunk_j);
+    if (chunk_j <= 0 || max_chunks_j <= 0 || light_j < 0) {
+      set_host_error("build_plan: invalid cutting");
+      return FE_ERR_INVALID_ARG;
+    }
+    const int64_t block_edges = (int64_t)kSub * chunk_j;
     const int64_t r0 = (int64_t)j * B;
     const int64_t r1 = std::min<int64_t>(r0 + B, n);
     int32_t pcount = 0, scount = 0, nblk = 0;
@@ -50,11 +58,11 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
       if (col) {
         for (int64_t e = e0; e < e0 + deg; ++e) eff += (col[e] < c_limit) ? 1 : 0;
       }
-      if (deg <= p.light) {
+      if (deg <= light_j) {
         lights.push_back({(int32_t)deg, (int32_t)r});
         continue;
       }
-      if (deg <= chunk) {
+      if (deg <= chunk_j) {
         waves.insert(waves.end(), {(int32_t)r, (int32_t)e0, (int32_t)deg, -1, 0, 0, 0, 0});
         continue;
       }
@@ -63,7 +71,7 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
         ++nblk;
         continue;
       }
-      const int64_t per = std::max<int64_t>(block_edges, (deg + max_chunks - 1) / max_chunks);
+      const int64_t per = std::max<int64_t>(block_edges, (deg + max_chunks_j - 1) / max_chunks_j);
       const int64_t k = (deg + per - 1) / per;
       for (int64_t i = 0; i < k; ++i) {
         const int64_t lo = deg * i / k, hi = deg * (i + 1) / k;

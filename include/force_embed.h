@@ -154,17 +154,29 @@ fe_status fe_epoch_units(fe_ctx *ctx, int ns, int64_t *edges,
  * Timing is enabled with fe_set_timing(ctx, 1).                            */
 fe_status fe_set_timing(fe_ctx *ctx, int enable);
 
-/* Autotuner (no counterpart in the reference): times `epochs` epochs of the
- * captured chain for each candidate tiling -- unroll {2,4,8} x waves per
- * block {2,4,8} x items per wave {1,2,4}, and with search_chunking also
- * chunk {16,32,64} x max_chunks {64,256,1024} -- restores X after every
- * trial and keeps the fastest. The launch-shape fields never change results;
- * the chunking fields change the summation order of heavy rows. Up to
- * max_results trials are reported in results / results_ms (ms per epoch).
+/* Autotuner (no counterpart in the reference). Trials time `epochs` epochs of
+ * the captured chain after one warm-up epoch, each force launch timed with
+ * HIP events; X is restored after every trial. A batch's launch time
+ * depends only on its own tiling, so each trial serves every batch:
+ * (1) every launch shape -- unroll {1,2,4} x waves per block {2,4,8} x
+ * items per wave {1,2,4} -- on all batches: each batch keeps its fastest;
+ * (2) with search_chunking, every cutting -- chunk {16,32,64} x max_chunks
+ * {64,256,1024} x light {0,8,16} -- the same way, then (1) again. The
+ * context keeps the per-batch result (fe_batch_tuning); *best holds the
+ * values most batches chose and *best_ms the resulting ms per epoch. Shapes
+ * never change results; the cutting changes the summation order of heavy
+ * rows. Up to max_results trials are reported in results / results_ms (the
+ * sum of the batches' mean launch times, ms).
  * Needs negative lists covering epochs + batches - 1 lists.                */
 fe_status fe_autotune(fe_ctx *ctx, int epochs, int search_chunking,
                       fe_tuning *results, double *results_ms, int max_results,
                       int *n_results, fe_tuning *best, double *best_ms);
+/* The tuning one batch's force launch uses (its own after fe_autotune or
+ * fe_set_batch_tuning; fe_set_tuning gives every batch the same).          */
+fe_status fe_batch_tuning(const fe_ctx *ctx, int batch, fe_tuning *out);
+/* Sets one batch's tuning (zero fields keep the current value; light -1
+ * sets 0). Needs the plan (CSR, X and batch size set).                     */
+fe_status fe_set_batch_tuning(fe_ctx *ctx, int batch, const fe_tuning *t);
 fe_status fe_kernel_times(fe_ctx *ctx, double *force_ms, double *fixup_ms,
                           double *apply_ms, int64_t *launches);
 

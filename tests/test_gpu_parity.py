@@ -141,6 +141,50 @@ def test_tuning_does_not_change_results():
         np.testing.assert_array_equal(o, outs[0])
 
 
+def test_per_batch_tunings():
+    # per-batch launch shapes give the same bits as one shape everywhere; a
+    # per-batch cutting equals the run whose cutting is uniform at each batch
+    # (a batch's result depends only on its own plan); autotune leaves X as
+    # it found it
+    g = api.Graph.rmat(12, 16, seed=5)
+    X = api.init_uniform(g.n, 64, seed=3)
+    B = 512
+    nb = -(-g.n // B)
+    lists = api.reference_negatives(g.n, 20, 5)[0]
+    rng = np.random.default_rng(0)
+    shapes = [(int(rng.choice([1, 2, 4, 8])), int(rng.choice([1, 2, 4, 8])),
+               int(rng.choice([1, 2, 3]))) for _ in range(nb)]
+    ctx = _ctx(g, 64, X, B, lists)
+    ctx.set_batch_tunings([s + (64, 1024, 16) for s in shapes])
+    assert [tuple(t[k] for k in ("unroll", "waves_per_block", "items_per_wave"))
+            for t in ctx.batch_tunings()] == shapes
+    ctx.train(3, B, 5, 0.02)
+    a = ctx.download_x()
+    ctx.close()
+    ctx = _ctx(g, 64, X, B, lists, tuning={"chunk": 64, "max_chunks": 1024, "light": 16})
+    ctx.train(3, B, 5, 0.02)
+    np.testing.assert_array_equal(a, ctx.download_x())
+    ctx.close()
+    # cut per batch: batch 0 with chunk 16 / max_chunks 64, the rest default
+    cut = [(2, 4, 1, 16, 64, 8)] + [(2, 4, 1, 64, 1024, 16)] * (nb - 1)
+    ctx = _ctx(g, 64, X, B, lists)
+    ctx.set_batch_tunings(cut)
+    d_mixed = ctx.step_delta(0, 0, B)
+    d_mixed1 = ctx.step_delta(0, 1, B)
+    ctx.close()
+    ctx = _ctx(g, 64, X, B, lists, tuning={"chunk": 16, "max_chunks": 64, "light": 8})
+    np.testing.assert_array_equal(d_mixed, ctx.step_delta(0, 0, B))
+    ctx.close()
+    ctx = _ctx(g, 64, X, B, lists, tuning={"chunk": 64, "max_chunks": 1024, "light": 16})
+    np.testing.assert_array_equal(d_mixed1, ctx.step_delta(0, 1, B))
+    ctx.train(2, B, 5, 0.02)
+    before = ctx.download_x()
+    ctx.autotune(epochs=1, search_chunking=True)
+    np.testing.assert_array_equal(before, ctx.download_x())
+    assert len(ctx.batch_tunings()) == nb
+    ctx.close()
+
+
 def test_graph_replay_equals_stream_launches_and_steps():
     g = api.Graph.sbm(1500, 6, 0.02, 0.001, seed=4)
     X = api.init_uniform(g.n, 64, seed=8)
