@@ -432,8 +432,8 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
       for (int ipw : {1, 2, 4}) shapes.push_back(Shape{u, w, ipw});
   std::vector<Cut> cuttings;
   for (int ch : {16, 32, 64})
-    for (int mc : {64, 256, 1024})
-      for (int li : {0, 8, 16}) cuttings.push_back(Cut{ch, mc, li});
+    for (int mc : {64, 1024})
+      for (int li : {0, 4, 8, 16}) cuttings.push_back(Cut{ch, mc, li});
   const size_t xbytes = sizeof(float) * (size_t)j.n * j.d;
   void *backup = nullptr;
   if ((s = hip_ok(c, hipMalloc(&backup, xbytes), "fe_autotune backup")) != FE_OK) return s;

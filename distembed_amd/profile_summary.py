@@ -38,10 +38,16 @@ def trace(stats_csv: str, trace_csv: str, out: str, command: str) -> None:
         durs[_short(r["Kernel_Name"])].append(
             (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
     lines.append("")
+    allv = []
     for k, v in sorted(durs.items()):
         if "force_kernel" in k:
+            allv += v
             lines.append(f"force launches {k}: n={len(v)} avg_us={statistics.mean(v):.2f} "
                          f"median_us={statistics.median(v):.2f} max_us={max(v):.2f}")
+    if allv:
+        lines.append(f"force launches, all instantiations: n={len(allv)} "
+                     f"avg_us={statistics.mean(allv):.2f} median_us={statistics.median(allv):.2f} "
+                     f"max_us={max(allv):.2f}")
     open(out, "w").write("\n".join(lines) + "\n")
 
 
@@ -67,6 +73,8 @@ def pmc(fetch_csv: str, write_csv: str, out: str, command: str, cal_fetch_csv: s
              "copyprobe (a cold 1 GiB device-to-device copy)", ""]
     lines.append(f"calibration factors: FETCH_SIZE reads {cal_f:.4f} x the copied KiB, "
                  f"WRITE_SIZE reads {cal_w:.4f} x")
+    tot_f = tot_w = 0.0
+    tot_n = 0
     for k in sorted(fetch):
         if "force_kernel" not in k:
             continue
@@ -74,10 +82,18 @@ def pmc(fetch_csv: str, write_csv: str, out: str, command: str, cal_fetch_csv: s
         w = statistics.mean(write.get(k, {0: 0.0}).values())
         fb = f * 1024 / (cal_f or 1.0)
         wb = w * 1024 / (cal_w or 1.0)
-        lines.append(f"{k}: dispatches={len(fetch[k])} fetch_KiB={f:.1f} write_KiB={w:.1f}")
-        lines.append(f"hbm_bytes_per_launch = {fb + wb:.6e}")
-        lines.append(f"hbm_read_bytes_per_launch = {fb:.6e}")
-        lines.append(f"hbm_write_bytes_per_launch = {wb:.6e}")
+        n = len(fetch[k])
+        tot_f += fb * n
+        tot_w += wb * n
+        tot_n += n
+        lines.append(f"{k}: dispatches={n} fetch_KiB={f:.1f} write_KiB={w:.1f} "
+                     f"read_bytes={fb:.4e} write_bytes={wb:.4e}")
+    if tot_n:
+        # every instantiation of the force kernel (batches use different ones)
+        lines.append(f"all force launches: dispatches={tot_n}")
+        lines.append(f"hbm_bytes_per_launch = {(tot_f + tot_w) / tot_n:.6e}")
+        lines.append(f"hbm_read_bytes_per_launch = {tot_f / tot_n:.6e}")
+        lines.append(f"hbm_write_bytes_per_launch = {tot_w / tot_n:.6e}")
     open(out, "w").write("\n".join(lines) + "\n")
 
 

@@ -161,7 +161,7 @@ fe_status fe_set_timing(fe_ctx *ctx, int enable);
  * (1) every launch shape -- unroll {1,2,4} x waves per block {2,4,8} x
  * items per wave {1,2,4} -- on all batches: each batch keeps its fastest;
  * (2) with search_chunking, every cutting -- chunk {16,32,64} x max_chunks
- * {64,256,1024} x light {0,8,16} -- the same way, then (1) again. The
+ * {64,1024} x light {0,4,8,16} -- the same way, then (1) again. The
  * context keeps the per-batch result (fe_batch_tuning); *best holds the
  * values most batches chose and *best_ms the resulting ms per epoch. Shapes
  * never change results; the cutting changes the summation order of heavy
