@@ -98,6 +98,17 @@ def init_uniform(n: int, d: int, lo: float = -0.5, hi: float = 0.5, seed: int = 
     return x
 
 
+def round_bf16(x: np.ndarray, device: int = 0) -> np.ndarray:
+    """x rounded to BF16 on the device as the BF16 shadow holds it
+    (fe_round_bf16), returned as float32."""
+    lib = _lib_or_raise()
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    bits = np.empty(x.shape, dtype=np.uint16)
+    _lib.check(lib.fe_round_bf16(device, _ptr(x, C.c_float), _ptr(bits, C.c_uint16), x.size),
+               _host_err)
+    return (bits.astype(np.uint32) << 16).view(np.float32)
+
+
 def reference_negatives(n: int, n_lists: int, ns: int, policy: int = _lib.FE_NEG_REDRAW):
     """Lists l = 0..n_lists-1 of generate_random_numbers(0, n, l, ns)
     (cpp/core/common.cpp:37-53); returns (int32 [n_lists, ns], draws equal to n)."""

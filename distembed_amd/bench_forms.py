@@ -169,7 +169,7 @@ def rmat_job_pool(count: int, seed: int = 1):
 
 def serving(qps_levels=(4.0, 8.0, 16.0), jobs_per_level: int = 48, epochs: int = 50,
             B: int = 16384, ns: int = 5, budget_gib: float = 32.0,
-            rounds_per_call: int = 4) -> dict:
+            rounds_per_call: int = 16) -> dict:
     pool = rmat_job_pool(jobs_per_level)
     X0s = [api.init_uniform(g.n, d, -0.5, 0.5, seed=i) for i, (g, d) in enumerate(pool)]
     units = [_units(g, min(B, g.n), ns, epochs) for g, _ in pool]

@@ -664,6 +664,38 @@ extern "C" fe_status fe_set_timing(fe_ctx *c, int enable) {
   return FE_OK;
 }
 
+extern "C" fe_status fe_round_bf16(int device, const float *x, uint16_t *out, int64_t n) {
+  if (n < 0 || (n > 0 && (!x || !out))) {
+    set_host_error("fe_round_bf16: NULL array or negative count");
+    return FE_ERR_INVALID_ARG;
+  }
+  if (n == 0) return FE_OK;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
+    (void)hipGetLastError();
+    set_host_error("fe_round_bf16: no such HIP device");
+    return FE_ERR_NO_DEVICE;
+  }
+  hipError_t e = hipSetDevice(device);
+  // the kernel converts pairs: pad an odd count with one zero
+  const int64_t n2 = (n + 1) / 2 * 2;
+  float *dx = nullptr;
+  uint16_t *ds = nullptr;
+  if (e == hipSuccess) e = hipMalloc((void **)&dx, sizeof(float) * n2);
+  if (e == hipSuccess) e = hipMalloc((void **)&ds, sizeof(uint16_t) * n2);
+  if (e == hipSuccess) e = hipMemset(dx, 0, sizeof(float) * n2);
+  if (e == hipSuccess) e = hipMemcpy(dx, x, sizeof(float) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = launch_to_bf16(dx, ds, n2, nullptr);
+  if (e == hipSuccess) e = hipMemcpy(out, ds, sizeof(uint16_t) * n, hipMemcpyDeviceToHost);
+  if (dx) hipFree(dx);
+  if (ds) hipFree(ds);
+  if (e != hipSuccess) {
+    set_host_error(std::string("fe_round_bf16: ") + hipGetErrorString(e));
+    return e == hipErrorOutOfMemory ? FE_ERR_OOM : FE_ERR_HIP;
+  }
+  return FE_OK;
+}
+
 extern "C" fe_status fe_kernel_times(fe_ctx *c, double *force_ms, double *fixup_ms,
                                      double *apply_ms, int64_t *launches) {
   CTX_GUARD(c);

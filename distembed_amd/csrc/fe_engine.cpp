@@ -115,6 +115,10 @@ struct EngineJob {
 
 } // namespace
 
+// Pinned round buffers in flight: the host may build this many rounds ahead
+// of the device.
+constexpr int kStages = 8;
+
 struct fe_engine {
   int device = 0;
   hipStream_t stream = nullptr;    // rounds
@@ -122,11 +126,12 @@ struct fe_engine {
   std::unique_ptr<Arena> arena;
   std::vector<std::unique_ptr<EngineJob>> jobs;
   Tuning tun;
-  // round staging: pinned host ring of 2 and one device buffer
+  // round staging: pinned host ring of kStages; one device buffer, reused in
+  // stream order
   struct Stage {
     char *host = nullptr;
     hipEvent_t copied = nullptr;
-  } stage[2];
+  } stage[kStages];
   int stage_next = 0;
   size_t stage_cap = 0; // steps
   char *dev_round = nullptr;
@@ -328,6 +333,8 @@ extern "C" fe_status fe_engine_submit(fe_engine *e, const fe_job_desc *d, int *j
   j.allocator = e->arena.get();
   j.tun = e->tun;
   s = job_init(j, d->n, d->d, d->gather_dtype, d->quirks);
+  // X at its full slot size from the start: no second copy while growing
+  if (s == FE_OK) s = job_reserve_slots(j, d->batch_size);
   if (s == FE_OK) s = job_upload_csr(j, d->row_ptr, d->col_idx, d->nnz, e->upload);
   if (s == FE_OK) s = job_upload_x(j, d->x0, e->upload);
   if (s == FE_OK) s = job_configure(j, d->batch_size, d->lr, d->max_bound);
@@ -346,7 +353,8 @@ extern "C" fe_status fe_engine_submit(fe_engine *e, const fe_job_desc *d, int *j
       }
     }
   }
-  if (s == FE_OK) s = job_prepare(j, e->upload);
+  // every slot of the layout: rounds write new rows into the free slot
+  if (s == FE_OK) s = job_prepare(j, e->upload, true);
   if (s == FE_OK) s = ehip(e, hipEventCreateWithFlags(&ej->uploaded, hipEventDisableTiming), "event");
   if (s == FE_OK) s = ehip(e, hipEventCreateWithFlags(&ej->done, hipEventDisableTiming), "event");
   if (s == FE_OK) s = ehip(e, hipEventRecord(ej->uploaded, e->upload), "event");
@@ -399,11 +407,13 @@ static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *
     EngineJob *ej = run[k];
     Job &j = ej->job;
     const int b = ej->batch;
-    sd[k] = j.host_desc[(size_t)j.plan.batches * DESC_SINGLE + b];
+    // a fused step: the batch's new rows go straight into the free slot of
+    // the job's layout (no apply launch), read in place by its next round
+    sd[k] = j.host_desc[(size_t)j.plan.batches * DESC_FUSED + b];
     sd[k].state = dep + kStateInts * k;
     ep[kStateInts * k] = ej->epoch;
-    ep[kStateInts * k + 1] = 0;
-    ep[kStateInts * k + 2] = 0;
+    ep[kStateInts * k + 1] = j.rot;
+    ep[kStateInts * k + 2] = (int)(((int64_t)j.rot + ej->epoch) % j.lay.P);
     ep[kStateInts * k + 3] = 0;
     // this round's negative ids, straight from the job's lists
     const int list = ej->epoch + b;
@@ -420,6 +430,8 @@ static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *
       if (++ej->epoch == ej->iterations) {
         ej->launched_all = true;
         finished.push_back(ej);
+        // after its last round every batch has moved down `iterations` slots
+        job_advance_layout(j, ej->iterations);
       }
     }
   }
@@ -446,15 +458,14 @@ extern "C" fe_status fe_engine_run(fe_engine *e, int rounds, int *running) {
       }
     }
     auto &stg = e->stage[e->stage_next];
-    e->stage_next ^= 1;
+    e->stage_next = (e->stage_next + 1) % kStages;
     if ((s = ehip(e, hipEventSynchronize(stg.copied), "staging reuse")) != FE_OK) return s;
     int K = 0;
     size_t lds = 0;
     std::vector<EngineJob *> finished;
     build_round(e, run, stg.host, K, lds, finished);
     const int *hpf = reinterpret_cast<const int *>(stg.host + K * sizeof(StepDesc)) + kStateInts * K;
-    const int *hpa = hpf + K + 1;
-    const int fblocks = hpf[K], ablocks = hpa[K];
+    const int fblocks = hpf[K];
     if ((s = ehip(e, hipMemcpyAsync(e->dev_round, stg.host, round_bytes(K), hipMemcpyHostToDevice,
                                     e->stream),
                   "round copy")) != FE_OK)
@@ -462,13 +473,10 @@ extern "C" fe_status fe_engine_run(fe_engine *e, int rounds, int *running) {
     if ((s = ehip(e, hipEventRecord(stg.copied, e->stream), "event")) != FE_OK) return s;
     const StepDesc *dsd = reinterpret_cast<const StepDesc *>(e->dev_round);
     const int *dpf = reinterpret_cast<const int *>(e->dev_round + K * sizeof(StepDesc)) + kStateInts * K;
-    const int *dpa = dpf + K + 1;
     if ((s = ehip(e, launch_force(dsd, dpf, K, fblocks, e->tun.waves_per_block,
                                   e->tun.items_per_wave, e->tun.unroll, lds, round_mode(run),
                                   round_g(run), e->stream),
                   "force launch")) != FE_OK)
-      return s;
-    if ((s = ehip(e, launch_apply(dsd, dpa, K, ablocks, e->stream), "apply launch")) != FE_OK)
       return s;
     for (EngineJob *ej : finished)
       if ((s = ehip(e, hipEventRecord(ej->done, e->stream), "event")) != FE_OK) return s;
@@ -605,12 +613,10 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
       const RoundInfo &ri = rounds[r];
       const StepDesc *dsd = reinterpret_cast<const StepDesc *>(dsched + ri.offset);
       const int *dpf = reinterpret_cast<const int *>(dsched + ri.offset + ri.K * sizeof(StepDesc)) + kStateInts * ri.K;
-      const int *dpa = dpf + ri.K + 1;
       ls = ehip(e, launch_force(dsd, dpf, ri.K, ri.fblocks, e->tun.waves_per_block,
                                 e->tun.items_per_wave, e->tun.unroll, ri.lds, ri.mode, ri.g_const,
                                 e->stream),
                 "force launch");
-      if (ls == FE_OK) ls = ehip(e, launch_apply(dsd, dpa, ri.K, ri.ablocks, e->stream), "apply launch");
     }
     hipGraph_t g = nullptr;
     hipError_t ce = hipStreamEndCapture(e->stream, &g);

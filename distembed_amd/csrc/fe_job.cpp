@@ -136,6 +136,19 @@ fe_status job_normalize_layout(Job &j, hipStream_t st) {
   return FE_OK;
 }
 
+fe_status job_reserve_slots(Job &j, int B) {
+  if (B <= 0) {
+    j.err = "batch size must be positive";
+    return FE_ERR_INVALID_ARG;
+  }
+  const int64_t batches = (j.n + B - 1) / B;
+  const size_t need = (size_t)(batches + 1) * B * j.d;
+  fe_status s = ensure_cap(j, j.X, j.cap_X, need, "X slots");
+  if (s == FE_OK && j.dtype == FE_GATHER_BF16) s = ensure_cap(j, j.Xs, j.cap_Xs, need, "shadow slots");
+  if (s == FE_OK) j.x_ready = false;
+  return s;
+}
+
 void job_advance_layout(Job &j, int epochs) {
   if (j.lay.P <= 0 || epochs <= 0) return;
   j.rot = (int)(((int64_t)j.rot + epochs) % j.lay.P);
@@ -660,7 +673,9 @@ fe_status job_loss(Job &j, uint64_t seed, int eval_ns, double *attr, double *rep
 }
 
 int64_t job_bytes_estimate(const Job &j) {
-  const int64_t nd = j.n * (int64_t)j.d;
+  // X (and the shadow) in the slot layout: batches + 1 slots of B rows
+  const int64_t rows = j.plan.B > 0 ? (int64_t)(j.plan.batches + 1) * j.plan.B : j.n;
+  const int64_t nd = rows * (int64_t)j.d;
   int64_t b = nd * 4 + (j.dtype == FE_GATHER_BF16 ? nd * 2 : 0);
   b += std::max<int64_t>(j.nnz, 1) * 4;
   b += (int64_t)j.plan.items.size() * 4;

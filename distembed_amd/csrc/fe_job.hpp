@@ -134,6 +134,9 @@ fe_status job_set_cuts(Job &j, const std::vector<Cut> &cuts);
 // shadow) must hold every slot of the layout (fused runs); otherwise only
 // the n rows of the layout at rest are needed.
 fe_status job_prepare(Job &j, hipStream_t st, bool slots = false);
+// Sizes X (and the shadow) for the slot layout of batch size B before any
+// rows are uploaded (their contents are not kept).
+fe_status job_reserve_slots(Job &j, int B);
 // Layout at rest after a fused run of `epochs` epochs.
 void job_advance_layout(Job &j, int epochs);
 // Moves the rows back to rotation 0 (row c at position c).

@@ -21,10 +21,10 @@
 // are summed in index order by whichever chunk finishes last (agent-scope
 // release/acquire around a per-row counter), so every result is
 // deterministic and independent of the launch shape. The batch's negative
-// rows are staged once per block in LDS. Batch j's launch also copies batch
-// j-1's new rows from staging into X, reading them from staging meanwhile,
-// so one launch per minibatch suffices. No MFMA: the step is a sparse
-// gather, not a dense contraction.
+// rows are staged once per block in LDS. X holds one slot more than there
+// are batches: a fused launch writes its batch's new rows into the free slot,
+// which none of its threads reads, so one launch per minibatch suffices.
+// No MFMA: the step is a sparse gather, not a dense contraction.
 #include <hip/hip_runtime.h>
 #include "fe_device.hpp"
 
@@ -115,12 +115,13 @@ __device__ __forceinline__ void load8_coherent(const float *p, float (&o)[8]) {
 // s_waitcnt vmcnt(0): this wave's stores have completed.
 __device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
-// FP32 -> BF16, round to nearest even; NaN stays a quiet NaN.
-__device__ __forceinline__ uint32_t bf16_bits(float f) {
-  uint32_t u = __float_as_uint(f);
-  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return ((u >> 16) | 0x40u) & 0xFFFFu;
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return u >> 16;
+// Two FP32 values to BF16, round to nearest even (one v_cvt_pk_bf16_f32;
+// subnormals are kept, a NaN stays a NaN); lo in bits 0-15.
+typedef __bf16 bf16x2v __attribute__((ext_vector_type(2)));
+typedef float f2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t bf16x2(float lo, float hi) {
+  const bf16x2v b = __builtin_convertvector((f2v){lo, hi}, bf16x2v);
+  return __builtin_bit_cast(uint32_t, b);
 }
 
 // scale() of cpp/algo/algo.hpp:65-72 (a NaN passes through unchanged).
@@ -212,10 +213,11 @@ struct RowSrc {
 };
 
 template <int MODE>
-__device__ __forceinline__ RowSrc make_src(const StepDesc &sd, const SlotMap &m, int q) {
+__device__ __forceinline__ RowSrc make_src(const StepDesc &sd, const SlotMap &m, int q, int G) {
   RowSrc r;
   r.bf = use_bf16<MODE>(sd);
-  const int dshift = 31 - __builtin_clz((unsigned)sd.D);
+  // D = 8G floats per row
+  const int dshift = 3 + (31 - __builtin_clz((unsigned)G));
   if (r.bf) {
     r.xq = reinterpret_cast<const char *>(sd.Xs + 8 * q);
     r.row_shift = dshift + 1;
@@ -227,9 +229,10 @@ __device__ __forceinline__ RowSrc make_src(const StepDesc &sd, const SlotMap &m,
   return r;
 }
 
-template <int MODE, bool P2>
-__device__ __forceinline__ void load_buf(const RowSrc &src, int c, RowBuf<MODE> &b) {
-  const char *p = src.xq + ((uint64_t)row_pos_t<P2>(src.m, (uint32_t)c) << src.row_shift);
+// The lane's slice of the row at slot position pos.
+template <int MODE>
+__device__ __forceinline__ void load_at(const RowSrc &src, uint32_t pos, RowBuf<MODE> &b) {
+  const char *p = src.xq + ((uint64_t)pos << src.row_shift);
   if constexpr (MODE == 1) {
     b.w = *(gcu4 *)p;
   } else if constexpr (MODE == 0) {
@@ -260,12 +263,21 @@ __device__ __forceinline__ void unpack_buf(const RowBuf<MODE> &b, float (&o)[8])
 // hardware reciprocal (1 ulp) and the clamp is one med3, which for finite
 // values equals scale(). A non-finite s anywhere in the wave takes the exact
 // expressions instead, whose clamp passes a NaN through as scale() does.
+// Repulsive forces whose clamp binds nowhere in the wave (s d1^2 <= bound^2,
+// so |v_k d1| <= sqrt(s) d1 <= bound, the usual case away from the row
+// itself) take one fused multiply-add per component with lr*d1.
 template <bool ATTR>
 __device__ __forceinline__ void add_force(const float (&v)[8], float s, float lr, float bound,
                                           float (&acc)[8]) {
   if (__builtin_amdgcn_ballot_w64(!__builtin_isfinite(s)) == 0) {
     const float d1 = ATTR ? -2.0f * __builtin_amdgcn_rcpf(1.0f + s)
                           : 2.0f * __builtin_amdgcn_rcpf((s + 1e-6f) * (1.0f + s));
+    if (!ATTR && __builtin_amdgcn_ballot_w64(!(s * d1 * d1 <= bound * bound)) == 0) {
+      const float c = lr * d1;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = fmaf(c, v[k], acc[k]);
+      return;
+    }
 #pragma unroll
     for (int k = 0; k < 8; ++k)
       acc[k] = fmaf(lr, __builtin_amdgcn_fmed3f(v[k] * d1, -bound, bound), acc[k]);
@@ -293,29 +305,41 @@ __device__ __forceinline__ void interact(const float (&xd)[8], const float (&xc)
 // the row's accumulators, one slot after the other. While the clamp bound
 // is >= 2 it never binds: |v_k d1| = 2|v_k| / (1 + s) <= 1 because
 // s >= v_k^2, so scale() is the identity and each component takes one fused
-// multiply-add with lr*d1 (d1 from the hardware reciprocal). A non-finite
-// distance anywhere in the wave, or a bound below 2, takes the exact
+// multiply-add with c = -2 lr rcp(1 + s) (the hardware reciprocal, 1 ulp).
+// The same form reproduces the reference's non-finite cases: s = +inf
+// gives c = -0, so finite components gain +-0 and infinite ones NaN
+// (inf * -0), as v*d1 with d1 = -2/inf does; s = NaN gives NaN throughout.
+// FAST is bound >= 2 (uniform per step); a smaller bound takes the exact
 // expressions of add_force.
-template <int U, int MODE>
+template <int U, int MODE, bool FAST>
 __device__ __forceinline__ void attract_round(const float (&xd)[8], const RowBuf<MODE> (&xb)[U],
                                               int G, float lr, float bound, float (&acc)[8]) {
-  const bool small_bound = bound < 2.0f;
+  if constexpr (FAST) {
+    const float c2 = -2.0f * lr;
 #pragma unroll
-  for (int uu = 0; uu < U; ++uu) {
-    float xc[8], v[8];
-    unpack_buf<MODE>(xb[uu], xc);
+    for (int uu = 0; uu < U; ++uu) {
+      float xc[8], v[8];
+      unpack_buf<MODE>(xb[uu], xc);
 #pragma unroll
-    for (int k = 0; k < 8; ++k) v[k] = xd[k] - xc[k];
-    float p = v[0] * v[0];
+      for (int k = 0; k < 8; ++k) v[k] = xd[k] - xc[k];
+      float p = v[0] * v[0];
 #pragma unroll
-    for (int k = 1; k < 8; ++k) p = fmaf(v[k], v[k], p);
-    const float sq = group_sum(p, G);
-    if (__builtin_amdgcn_ballot_w64(small_bound || !__builtin_isfinite(sq)) == 0) {
-      const float c = lr * (-2.0f * __builtin_amdgcn_rcpf(1.0f + sq));
+      for (int k = 1; k < 8; ++k) p = fmaf(v[k], v[k], p);
+      const float c = c2 * __builtin_amdgcn_rcpf(1.0f + group_sum(p, G));
 #pragma unroll
       for (int k = 0; k < 8; ++k) acc[k] = fmaf(c, v[k], acc[k]);
-    } else {
-      add_force<true>(v, sq, lr, bound, acc);
+    }
+  } else {
+#pragma unroll
+    for (int uu = 0; uu < U; ++uu) {
+      float xc[8], v[8];
+      unpack_buf<MODE>(xb[uu], xc);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) v[k] = xd[k] - xc[k];
+      float p = v[0] * v[0];
+#pragma unroll
+      for (int k = 1; k < 8; ++k) p = fmaf(v[k], v[k], p);
+      add_force<true>(v, group_sum(p, G), lr, bound, acc);
     }
   }
 }
@@ -395,36 +419,39 @@ __device__ __forceinline__ void sum_partials(const StepDesc &sd, int p0, int k, 
 // accumulators: group g takes edges g, g+NG, ... in order, U rows in flight.
 // A slot past the end, or holding a column the quirk skips (c >= c_limit),
 // reads row r itself, whose distance to itself is exactly zero: no force.
-template <int U, int MODE, bool P2>
+// Each lane turns its column of the 64-edge block into a slot position once;
+// a slot then fetches the position from lane k (ds_bpermute takes the lane
+// index modulo 64). Lanes k in [m, 64) hold row r; k >= 64 happens only for
+// U > G.
+template <int U, int MODE, bool FAST>
 __device__ __forceinline__ void gather_edges_t(const StepDesc &sd, const RowSrc &src,
                                              const float (&xd)[8], int r, int e0, int cnt,
                                              int G, int NG, int g, int lane, float (&acc)[8]) {
   const float lr = sd.lr, bound = sd.bound;
   const int c_limit = sd.c_limit;
+  const uint32_t pos_r = row_pos(src.m, (uint32_t)r);
   for (int eb = 0; eb < cnt; eb += 64) {
     const int m = min(64, cnt - eb);
     int cl = lane < m ? gld_int(sd.col + e0 + eb + lane) : r;
     cl = cl < c_limit ? cl : r;
+    const int pl = (int)row_pos(src.m, (uint32_t)cl);
     const int rounds = (m + NG - 1) / NG;
     for (int u0 = 0; u0 < rounds; u0 += U) {
       RowBuf<MODE> xb[U];
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
         const int k = (u0 + uu) * NG + g;
-        const int c = __shfl(cl, k & 63);
-        load_buf<MODE, P2>(src, k < m ? c : r, xb[uu]);
+        uint32_t pos = (uint32_t)__builtin_amdgcn_ds_bpermute(k << 2, pl);
+        if (U > G && k >= 64) pos = pos_r;
+        load_at<MODE>(src, pos, xb[uu]);
       }
       // keep the U loads ahead of every use of their results
       __builtin_amdgcn_sched_barrier(0);
-      attract_round<U, MODE>(xd, xb, G, lr, bound, acc);
+      attract_round<U, MODE, FAST>(xd, xb, G, lr, bound, acc);
     }
   }
 }
 
-// Row load for the distances: the master row, rounded as the shadow holds it
-// when gathers read BF16 (no extra traffic), so a row meeting itself as a
-// negative or through a self-loop feels exactly zero force, as in the
-// reference; the update still goes to the FP32 master.
 // The row's own slice for the distances: the master row, rounded as the
 // shadow holds it when gathers read BF16 (no extra traffic), so a row meeting
 // itself as a negative or through a self-loop feels exactly zero force, as
@@ -432,9 +459,17 @@ __device__ __forceinline__ void gather_edges_t(const StepDesc &sd, const RowSrc 
 template <int MODE>
 __device__ __forceinline__ void own_slice(const StepDesc &sd, const float (&xr)[8],
                                           float (&xd)[8]) {
+  if (use_bf16<MODE>(sd)) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k)
-    xd[k] = use_bf16<MODE>(sd) ? __uint_as_float(bf16_bits(xr[k]) << 16) : xr[k];
+    for (int k = 0; k < 8; k += 2) {
+      const uint32_t w = bf16x2(xr[k], xr[k + 1]);
+      xd[k] = __uint_as_float(w << 16);
+      xd[k + 1] = __uint_as_float(w & 0xFFFF0000u);
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) xd[k] = xr[k];
+  }
 }
 
 // Row t of the batch: a fused step writes the new row straight into the
@@ -451,10 +486,10 @@ __device__ __forceinline__ void write_row(const StepDesc &sd, uint32_t out, int 
   store8_f32(sd.Xw + pos * sd.D + 8 * q, o);
   if (use_bf16<MODE>(sd)) {
     u4v w;
-    w.x = bf16_bits(o[0]) | (bf16_bits(o[1]) << 16);
-    w.y = bf16_bits(o[2]) | (bf16_bits(o[3]) << 16);
-    w.z = bf16_bits(o[4]) | (bf16_bits(o[5]) << 16);
-    w.w = bf16_bits(o[6]) | (bf16_bits(o[7]) << 16);
+    w.x = bf16x2(o[0], o[1]);
+    w.y = bf16x2(o[2], o[3]);
+    w.z = bf16x2(o[4], o[5]);
+    w.w = bf16x2(o[6], o[7]);
     *(__attribute__((address_space(1))) u4v *)(sd.Xsw + pos * sd.D + 8 * q) = w;
   }
 }
@@ -480,11 +515,11 @@ __device__ __forceinline__ void finish_row(const StepDesc &sd, uint32_t out, con
 // against every staged negative and writes its row. The loops run to the
 // pack's largest degree kmax, so control flow stays uniform; slots past a
 // row's degree read the row itself (no force).
-template <int U, int MODE, bool P2>
+template <int U, int MODE, bool FAST>
 __device__ __forceinline__ void light_pack_t(const StepDesc &sd, const RowSrc &src, uint32_t out,
                                            u4v ia, int kmax, bool valid, const float (&xr)[8],
                                            const float *lds_neg, bool has_neg, int rep, int G,
-                                           int g, int q) {
+                                           int g, int q, bool &synced) {
   const int r = (int)ia.x, e0 = (int)ia.y, deg = valid ? (int)ia.z : 0;
   kmax = __builtin_amdgcn_readfirstlane(kmax);
   const float lr = sd.lr, bound = sd.bound;
@@ -493,21 +528,31 @@ __device__ __forceinline__ void light_pack_t(const StepDesc &sd, const RowSrc &s
   own_slice<MODE>(sd, xr, xd);
 #pragma unroll
   for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+  // slot positions as in gather_edges_t: lane q of the group holds the
+  // position of edge kb+q (row r past the degree); k - kb >= G only for U > G
+  const uint32_t pos_r = row_pos(src.m, (uint32_t)r);
   for (int kb = 0; kb < kmax; kb += G) {
     int cl = (kb + q < deg) ? gld_int(sd.col + e0 + kb + q) : r;
     cl = cl < c_limit ? cl : r;
+    const int pl = (int)row_pos(src.m, (uint32_t)cl);
     const int kend = min(kmax, kb + G);
     for (int k0 = kb; k0 < kend; k0 += U) {
       RowBuf<MODE> xb[U];
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
         const int k = k0 + uu;
-        const int c = __shfl(cl, g * G + ((k - kb) & (G - 1)));
-        load_buf<MODE, P2>(src, k < kend ? c : r, xb[uu]);
+        uint32_t pos = (uint32_t)__builtin_amdgcn_ds_bpermute((g * G + (k - kb)) << 2, pl);
+        if (U > G && k - kb >= G) pos = pos_r;
+        load_at<MODE>(src, pos, xb[uu]);
       }
       __builtin_amdgcn_sched_barrier(0);
-      attract_round<U, MODE>(xd, xb, G, lr, bound, acc);
+      attract_round<U, MODE, FAST>(xd, xb, G, lr, bound, acc);
     }
+  }
+  // the wave's first unit waits here for the block's staged negative rows
+  if (!synced) {
+    __syncthreads();
+    synced = true;
   }
   const int t = r - sd.row_begin;
   if (has_neg) {
@@ -536,12 +581,13 @@ __device__ __forceinline__ void light_pack_t(const StepDesc &sd, const RowSrc &s
   }
 }
 
-// One uniform branch per work unit picks the power-of-two addressing.
+// One uniform branch per work unit picks the single-FMA attractive form
+// (clamp bound >= 2).
 template <int U, int MODE>
 __device__ __forceinline__ void gather_edges(const StepDesc &sd, const RowSrc &src,
                                              const float (&xd)[8], int r, int e0, int cnt,
                                              int G, int NG, int g, int lane, float (&acc)[8]) {
-  if (src.m.b_shift >= 0) gather_edges_t<U, MODE, true>(sd, src, xd, r, e0, cnt, G, NG, g, lane, acc);
+  if (sd.bound >= 2.0f) gather_edges_t<U, MODE, true>(sd, src, xd, r, e0, cnt, G, NG, g, lane, acc);
   else gather_edges_t<U, MODE, false>(sd, src, xd, r, e0, cnt, G, NG, g, lane, acc);
 }
 
@@ -549,11 +595,13 @@ template <int U, int MODE>
 __device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src, uint32_t out,
                                            u4v ia, int kmax, bool valid, const float (&xr)[8],
                                            const float *lds_neg, bool has_neg, int rep, int G,
-                                           int g, int q) {
-  if (src.m.b_shift >= 0)
-    light_pack_t<U, MODE, true>(sd, src, out, ia, kmax, valid, xr, lds_neg, has_neg, rep, G, g, q);
+                                           int g, int q, bool &synced) {
+  if (sd.bound >= 2.0f)
+    light_pack_t<U, MODE, true>(sd, src, out, ia, kmax, valid, xr, lds_neg, has_neg, rep, G, g, q,
+                                synced);
   else
-    light_pack_t<U, MODE, false>(sd, src, out, ia, kmax, valid, xr, lds_neg, has_neg, rep, G, g, q);
+    light_pack_t<U, MODE, false>(sd, src, out, ia, kmax, valid, xr, lds_neg, has_neg, rep, G, g, q,
+                                 synced);
 }
 
 // One minibatch of one or more jobs. Blocks [0, n_block_items) of a step
@@ -562,10 +610,15 @@ __device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src
 // and then add in sub-range order through LDS. A chunk publishes its
 // partial (device-coherent stores) and bumps the row's counter; the chunk
 // that counts last adds all partials of the row in index order. The
-// remaining blocks' waves walk the light-row packs and then the medium rows
-// with a grid stride. A fused step writes its new rows into the layout's
-// free slot, which no thread of the launch reads. GT > 0: every step of the
-// launch has G = GT lanes per row (a compile-time constant); 0: per step.
+// remaining blocks' waves walk the medium rows (longest first) and then the
+// light-row packs with a grid stride, so the launch ends on short units. The
+// negative rows are staged in LDS without a barrier of their own: every wave
+// passes exactly one barrier before its first repulsion (a block item's
+// after its gathers, a wave's after its first unit's gathers, or at its end
+// if it has no unit), so staging overlaps the first gathers. A fused step
+// writes its new rows into the layout's free slot, which no thread of the
+// launch reads. GT > 0: every step of the launch has G = GT lanes per row (a
+// compile-time constant); 0: per step.
 template <int U, int MODE, int GT>
 __global__ __launch_bounds__(512) void force_kernel(
     const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
@@ -592,17 +645,19 @@ __global__ __launch_bounds__(512) void force_kernel(
   const int32_t *witems = sd.items + (size_t)kItemInts * nbi;
   const int32_t *lrows = witems + (size_t)kItemInts * n_wave_items;
   const int n_packs = (n_light + NG - 1) / NG;
+  // wave units: medium rows [0, n_wave_items), then light packs
   int u = (lb - nbi) * wpb + wave;
   u4v pre0 = {0u, 0u, 0u, 0u}, pre1 = {0u, 0u, 0u, 0u};
   if (lb < nbi) {
     pre0 = gld_u4(sd.items + (size_t)kItemInts * lb);
     pre1 = gld_u4(sd.items + (size_t)kItemInts * lb + 4);
-  } else if (u < n_packs) {
-    const int idx = u * NG + g;
-    pre0 = gld_u4(lrows + (size_t)kItemInts * (idx < n_light ? idx : u * NG));
-    pre1 = gld_u4(lrows + (size_t)kItemInts * u * NG + 4);
-  } else if (u - n_packs < n_wave_items) {
-    pre0 = gld_u4(witems + (size_t)kItemInts * (u - n_packs));
+  } else if (u < n_wave_items) {
+    pre0 = gld_u4(witems + (size_t)kItemInts * u);
+  } else if (u - n_wave_items < n_packs) {
+    const int pk = u - n_wave_items;
+    const int idx = pk * NG + g;
+    pre0 = gld_u4(lrows + (size_t)kItemInts * (idx < n_light ? idx : pk * NG));
+    pre1 = gld_u4(lrows + (size_t)kItemInts * pk * NG + 4);
   }
   // this step's negative ids, one per staging piece (first blockDim pieces)
   const int total8 = sd.ns * G;
@@ -623,7 +678,7 @@ __global__ __launch_bounds__(512) void force_kernel(
   const uint32_t out = (uint32_t)out_sl * (uint32_t)sd.lay.B;
 
   // the own row of this wave's first unit, in flight with the negative rows
-  const bool unit0 = lb < nbi || u < n_packs + n_wave_items;
+  const bool unit0 = lb < nbi || u < n_wave_items + n_packs;
   float xr0[8];
   if (unit0) load8_f32(sd.X + (size_t)(pre0.x + own_delta) * D + 8 * q, xr0);
   const int rep = epoch == 0 ? sd.rep0 : sd.rep1;
@@ -633,8 +688,7 @@ __global__ __launch_bounds__(512) void force_kernel(
     for (int p = pz + blockDim.x; p < total8; p += blockDim.x)
       stage_piece<MODE>(sd, m, p, gld_int(sd.neg_ids + p / G), G, lds);
   }
-  __syncthreads();
-  const RowSrc src = make_src<MODE>(sd, m, q);
+  const RowSrc src = make_src<MODE>(sd, m, q, G);
 
   if (lb < nbi) {
     const int r = (int)pre0.x, e0 = (int)pre0.y, cnt = (int)pre0.z, pslot = (int)pre0.w;
@@ -686,24 +740,14 @@ __global__ __launch_bounds__(512) void force_kernel(
         finish_row<MODE>(sd, out, lds, has_neg && t < rep, t, xr0, xd, acc, G, NG, g, q);
     }
   } else {
-    // wave work: packs of light rows (one row per lane group), then the
-    // medium rows (one row per wave), walked with a grid stride
+    // wave work: the medium rows (one row per wave), then packs of light
+    // rows (one row per lane group), walked with a grid stride
     const int W = (nb - nbi) * wpb;
-    bool have = true; // pre0/pre1 and xr0 hold this wave's current unit
-    for (; u < n_packs; u += W) {
-      const int idx = u * NG + g;
+    bool have = true;    // pre0/pre1 and xr0 hold this wave's current unit
+    bool synced = false; // this wave has passed the block's one barrier
+    for (; u < n_wave_items; u += W) {
       if (!have) {
-        pre0 = gld_u4(lrows + (size_t)kItemInts * (idx < n_light ? idx : u * NG));
-        pre1 = gld_u4(lrows + (size_t)kItemInts * u * NG + 4);
-        load8_f32(sd.X + (size_t)(pre0.x + own_delta) * D + 8 * q, xr0);
-      }
-      have = false;
-      light_pack<U, MODE>(sd, src, out, pre0, (int)pre1.x, idx < n_light, xr0, lds, has_neg,
-                          rep, G, g, q);
-    }
-    for (int it = u - n_packs; it < n_wave_items; it += W) {
-      if (!have) {
-        pre0 = gld_u4(witems + (size_t)kItemInts * it);
+        pre0 = gld_u4(witems + (size_t)kItemInts * u);
         load8_f32(sd.X + (size_t)(pre0.x + own_delta) * D + 8 * q, xr0);
       }
       have = false;
@@ -714,8 +758,24 @@ __global__ __launch_bounds__(512) void force_kernel(
 #pragma unroll
       for (int k = 0; k < 8; ++k) acc[k] = 0.f;
       gather_edges<U, MODE>(sd, src, xd, r, e0, cnt, G, NG, g, lane, acc);
+      if (!synced) {
+        __syncthreads();
+        synced = true;
+      }
       finish_row<MODE>(sd, out, lds, has_neg && t < rep, t, xr0, xd, acc, G, NG, g, q);
     }
+    for (int pk = u - n_wave_items; pk < n_packs; pk += W) {
+      const int idx = pk * NG + g;
+      if (!have) {
+        pre0 = gld_u4(lrows + (size_t)kItemInts * (idx < n_light ? idx : pk * NG));
+        pre1 = gld_u4(lrows + (size_t)kItemInts * pk * NG + 4);
+        load8_f32(sd.X + (size_t)(pre0.x + own_delta) * D + 8 * q, xr0);
+      }
+      have = false;
+      light_pack<U, MODE>(sd, src, out, pre0, (int)pre1.x, idx < n_light, xr0, lds, has_neg,
+                          rep, G, g, q, synced);
+    }
+    if (!synced) __syncthreads();
   }
   (void)ipw;
 }
@@ -743,8 +803,8 @@ __global__ __launch_bounds__(256) void apply_kernel(
     X4[base4 + idx] = v;
     if (sd.bf16) {
       uint2 w;
-      w.x = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
-      w.y = bf16_bits(v.z) | (bf16_bits(v.w) << 16);
+      w.x = bf16x2(v.x, v.y);
+      w.y = bf16x2(v.z, v.w);
       reinterpret_cast<uint2 *>(sd.Xsw)[base4 + idx] = w;
     }
   }
@@ -789,7 +849,7 @@ __global__ void to_bf16_kernel(const float *__restrict__ x, uint16_t *__restrict
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < count2;
        i += (int64_t)gridDim.x * blockDim.x) {
     const float2 v = reinterpret_cast<const float2 *>(x)[i];
-    reinterpret_cast<uint32_t *>(xs)[i] = bf16_bits(v.x) | (bf16_bits(v.y) << 16);
+    reinterpret_cast<uint32_t *>(xs)[i] = bf16x2(v.x, v.y);
   }
 }
 

@@ -35,7 +35,7 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
   p.pslots.assign(p.batches, 0);
   p.edges_eff.assign(p.batches, 0);
   p.items.reserve((size_t)kItemInts * (size_t)(n + (row_ptr[n] / chunk) + 16));
-  std::vector<int32_t> waves;
+  std::vector<std::pair<int32_t, int32_t>> waves;  // (degree, row)
   std::vector<std::pair<int32_t, int32_t>> lights; // (degree, row)
   for (int j = 0; j < p.batches; ++j) {
     const int chunk_j = cuts ? cuts[j].chunk : chunk;
@@ -63,7 +63,7 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
         continue;
       }
       if (deg <= chunk_j) {
-        waves.insert(waves.end(), {(int32_t)r, (int32_t)e0, (int32_t)deg, -1, 0, 0, 0, 0});
+        waves.push_back({(int32_t)deg, (int32_t)r});
         continue;
       }
       if (deg <= block_edges) {
@@ -83,13 +83,17 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
       pcount += (int32_t)k;
       ++scount;
     }
-    p.items.insert(p.items.end(), waves.begin(), waves.end());
-    // light rows by decreasing degree (row id breaks ties), so the rows of a
-    // pack have similar degrees; each entry carries its pack's largest degree
-    std::sort(lights.begin(), lights.end(),
-              [](const std::pair<int32_t, int32_t> &a, const std::pair<int32_t, int32_t> &b) {
-                return a.first != b.first ? a.first > b.first : a.second < b.second;
-              });
+    // medium and light rows by decreasing degree (row id breaks ties): the
+    // grid-stride walk takes the longest units first, and the rows of a
+    // light pack have similar degrees (each entry carries its pack's largest)
+    auto longer = [](const std::pair<int32_t, int32_t> &a, const std::pair<int32_t, int32_t> &b) {
+      return a.first != b.first ? a.first > b.first : a.second < b.second;
+    };
+    std::sort(waves.begin(), waves.end(), longer);
+    for (const auto &w : waves)
+      p.items.insert(p.items.end(),
+                     {w.second, (int32_t)row_ptr[w.second], w.first, -1, 0, 0, 0, 0});
+    std::sort(lights.begin(), lights.end(), longer);
     for (size_t i = 0; i < lights.size(); ++i) {
       const int32_t r = lights[i].second;
       const int32_t pack_max = lights[(i / pack) * pack].first;

@@ -179,6 +179,11 @@ fe_status fe_batch_tuning(const fe_ctx *ctx, int batch, fe_tuning *out);
 fe_status fe_set_batch_tuning(fe_ctx *ctx, int batch, const fe_tuning *t);
 fe_status fe_kernel_times(fe_ctx *ctx, double *force_ms, double *fixup_ms,
                           double *apply_ms, int64_t *launches);
+/* Rounds n floats to BF16 on device `device` exactly as the BF16 shadow
+ * holds them (round to nearest even, subnormals kept, a NaN stays a NaN);
+ * out receives the 16-bit patterns. No counterpart in the reference: it
+ * exposes the shadow's rounding to callers and tests.                      */
+fe_status fe_round_bf16(int device, const float *x, uint16_t *out, int64_t n);
 
 /* ---- host helpers (no device needed) ---------------------------------- */
 

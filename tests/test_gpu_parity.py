@@ -397,6 +397,33 @@ def _bf16_batch_parity(g, d, B, steps, tuning=None):
         ctx.close()
 
 
+def test_device_bf16_rounding_matches_oracle():
+    """The shadow's rounding on the device (fe_round_bf16, the conversion
+    every BF16 launch applies) equals the oracle's round-to-nearest-even bit
+    for bit: random bit patterns, exact ties with an even and an odd kept
+    bit, neighbours of ties, subnormals, overflow to infinity, signed zeros
+    and infinities. A NaN stays a NaN."""
+    rng = np.random.default_rng(5)
+    hi = rng.integers(0, 2 ** 16, 4096, dtype=np.uint64).astype(np.uint32)
+    special = np.array([0x00000000, 0x80000000, 0x00000001, 0x00008000, 0x00018000,
+                        0x00017FFF, 0x007FFFFF, 0x00800000, 0x807FFFFF, 0x3F808000,
+                        0x3F818000, 0x7F7F7FFF, 0x7F7F8000, 0x7F7FFFFF, 0xFF7FFFFF,
+                        0x7F800000, 0xFF800000, 0x7FC00000, 0x7F800001, 0xFFFFFFFF],
+                       dtype=np.uint32)
+    bits = np.concatenate([
+        rng.integers(0, 2 ** 32, 200_000, dtype=np.uint64).astype(np.uint32),
+        (hi << 16) | 0x8000, (hi << 16) | 0x7FFF, (hi << 16) | 0x8001, special])
+    x = bits.view(np.float32)
+    nan = np.isnan(x)
+    assert nan.sum() > 100
+    for m in (x.size, 7):  # an odd count too
+        got = api.round_bf16(x[:m])
+        want = orc.bf16_round(x[:m])
+        assert np.array_equal(np.isnan(got), nan[:m])
+        ok = ~nan[:m]
+        assert np.array_equal(got[ok].view(np.uint32), want[ok].view(np.uint32))
+
+
 def test_bf16_batches_match_oracle_at_rounded_rows():
     # the BF16 form's forces are the FP32 forces at X rounded to bfloat16
     # (every gather and the row's own distance), so on identical X they meet
