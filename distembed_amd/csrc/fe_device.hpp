@@ -12,10 +12,14 @@ namespace fe {
 //   whose k partials p0..p0+k-1 the last chunk to finish adds in index order.
 // A batch's items are its block items (one per block, cut into kSub
 // sub-ranges), then its wave items (one row per wave), then its light rows
-// {row, e0, deg, -1, pack_max_deg, 0, 0, 0} (one row per lane group, sorted
-// by degree, pack_max_deg = largest degree of the row's pack).
+// {row, e0, deg, pack_max_deg, 0, 0, 0, 1 + l} (one row per lane group,
+// sorted by degree, pack_max_deg = largest degree of the row's pack). Light
+// row l of the job (over all batches) has its deg <= kLightCols column ids
+// copied to lcols[l * kLightCols, ...), so a light pack reads its columns
+// without waiting for its items.
 constexpr int kItemInts = 8;
 constexpr int kSub = 4;
+constexpr int kLightCols = 16;
 
 // Row layout of a job's embedding (X and the BF16 shadow Xs). The rows are
 // cut into batches of B; batch k occupies one slot of B rows, and there are
@@ -45,6 +49,7 @@ struct alignas(16) StepDesc {
   uint16_t *Xsw;         // same buffer, refreshed with the master rows
   const int32_t *col;    // CSR column ids (all of the job's graph)
   const int32_t *items;  // this batch's work items
+  const int32_t *lcols;  // this batch's light rows' columns, kLightCols each
   float *S;              // rows x D: new rows or deltas (unfused steps)
   float *P;              // partial sums of heavy-row chunks
   int *counters;         // per split row of the batch: chunks finished

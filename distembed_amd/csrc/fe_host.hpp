@@ -24,7 +24,10 @@ fe_status build_csr(int64_t n, const int64_t *u, const int64_t *v, int64_t m, bo
 // A row of degree <= chunk is a wave item. A row of degree <= kSub*chunk is
 // one block item. A heavier row is cut into
 // k = ceil(deg / max(kSub*chunk, ceil(deg / max_chunks))) near-equal block
-// items. Each batch lists its block items first, then its wave items.
+// items. Each batch lists its block items first, then its wave items
+// (medium rows, by decreasing degree), then its light rows (degree <= light,
+// by decreasing degree; their columns are also copied to the light-column
+// array, see kLightCols).
 struct Plan {
   int B = 0;
   int batches = 0;
@@ -35,6 +38,7 @@ struct Plan {
   std::vector<int64_t> item_off;  // batches + 1, in items
   std::vector<int32_t> block_items; // per batch: leading block items
   std::vector<int32_t> light_rows;  // per batch: trailing light rows
+  std::vector<int64_t> light_off;   // batches + 1: first light row of batch j
   int light = 0;                    // rows of degree <= light go one per lane group
   int pack = 1;                     // lane groups per wave (64 / (d / 8))
   std::vector<int32_t> splits;    // {row, p0, k, 0} per heavy row

@@ -418,6 +418,7 @@ static StepDesc make_desc(const Job &j, int b, DescKind kind) {
   sd.Xsw = j.Xs;
   sd.col = j.col;
   sd.items = j.items + (size_t)kItemInts * j.plan.item_off[b];
+  sd.lcols = j.items + j.plan.items.size() + (size_t)kLightCols * j.plan.light_off[b];
   sd.S = j.S;
   sd.P = j.P;
   sd.counters = j.counters;
@@ -498,7 +499,10 @@ fe_status job_prepare(Job &j, hipStream_t st, bool slots) {
     if (j.rot != 0 && j.lay_B != j.plan.B && (s = job_normalize_layout(j, st)) != FE_OK) return s;
     j.lay = make_layout(j.plan.B, j.plan.batches);
     if (j.rot == 0) j.lay_B = j.plan.B;
-    if ((s = ensure_cap(j, j.items, j.cap_items, j.plan.items.size(), "work items")) != FE_OK)
+    // the items, then the light rows' columns
+    const size_t n_lcols = (size_t)kLightCols * j.plan.light_off[j.plan.batches];
+    if ((s = ensure_cap(j, j.items, j.cap_items, j.plan.items.size() + n_lcols, "work items")) !=
+        FE_OK)
       return s;
     if ((s = ensure_cap(j, j.S, j.cap_S, (size_t)j.B * j.d, "step rows")) != FE_OK) return s;
     if ((s = ensure_cap(j, j.P, j.cap_P, (size_t)std::max(j.plan.max_pslots, 1) * j.d,
@@ -510,6 +514,10 @@ fe_status job_prepare(Job &j, hipStream_t st, bool slots) {
     FE_HIP(j, hipMemcpyAsync(j.items, j.plan.items.data(), sizeof(int32_t) * j.plan.items.size(),
                              hipMemcpyHostToDevice, st),
            "work items copy");
+    if (n_lcols > 0)
+      FE_HIP(j, launch_fill_lcols(j.items, (int)(j.plan.items.size() / kItemInts), j.col,
+                                  j.items + j.plan.items.size(), st),
+             "light columns");
     if ((s = job_reset_counters(j, st)) != FE_OK) return s;
     FE_HIP(j, hipStreamSynchronize(st), "plan upload sync");
     j.plan_dirty = false;
@@ -628,7 +636,7 @@ fe_status job_loss(Job &j, uint64_t seed, int eval_ns, double *attr, double *rep
   ld.items = j.items;
   ld.lay = j.rot == 0 ? j.lay : make_layout(j.lay_B, (int)((j.n + j.lay_B - 1) / j.lay_B));
   ld.rot = j.rot;
-  ld.n_items = (int)(j.plan.items.size() / kItemInts);
+  ld.n_items = (int)j.plan.item_off[j.plan.batches];
   ld.D = j.d;
   ld.G = j.G;
   ld.c_limit = (int)j.c_limit;
@@ -678,7 +686,7 @@ int64_t job_bytes_estimate(const Job &j) {
   const int64_t nd = rows * (int64_t)j.d;
   int64_t b = nd * 4 + (j.dtype == FE_GATHER_BF16 ? nd * 2 : 0);
   b += std::max<int64_t>(j.nnz, 1) * 4;
-  b += (int64_t)j.plan.items.size() * 4;
+  b += ((int64_t)j.plan.items.size() + (int64_t)kLightCols * j.plan.light_off.back()) * 4;
   b += (int64_t)j.plan.B * j.d * 4;
   b += (int64_t)std::max(j.plan.max_pslots, 1) * j.d * 4;
   b += (int64_t)std::max(j.plan.max_splits, 1) * 4;

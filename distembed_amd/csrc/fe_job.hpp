@@ -24,6 +24,8 @@ hipError_t launch_apply(const StepDesc *steps, const int *blk_prefix, int n_step
 hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, int rot,
                         hipStream_t st);
 hipError_t launch_to_bf16(const float *x, uint16_t *xs, int64_t count, hipStream_t st);
+hipError_t launch_fill_lcols(const int32_t *items, int n_items, const int32_t *col,
+                             int32_t *lcols, hipStream_t st);
 hipError_t launch_loss_edges(const LossDesc &ld, double *block_out, int n_blocks, int ipw,
                              hipStream_t st);
 hipError_t launch_loss_negs(const LossDesc &ld, double *block_out, int n_blocks,

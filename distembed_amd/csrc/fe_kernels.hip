@@ -26,6 +26,7 @@
 // which none of its threads reads, so one launch per minibatch suffices.
 // No MFMA: the step is a sparse gather, not a dense contraction.
 #include <hip/hip_runtime.h>
+#include <algorithm>
 #include "fe_device.hpp"
 
 namespace fe {
@@ -517,10 +518,12 @@ __device__ __forceinline__ void finish_row(const StepDesc &sd, uint32_t out, con
 // row's degree read the row itself (no force).
 template <int U, int MODE, bool FAST>
 __device__ __forceinline__ void light_pack_t(const StepDesc &sd, const RowSrc &src, uint32_t out,
-                                           u4v ia, int kmax, bool valid, const float (&xr)[8],
-                                           const float *lds_neg, bool has_neg, int rep, int G,
-                                           int g, int q, bool &synced) {
-  const int r = (int)ia.x, e0 = (int)ia.y, deg = valid ? (int)ia.z : 0;
+                                           u4v ia, int lidx, int pc, int kmax, bool valid,
+                                           const float (&xr)[8], const float *lds_neg,
+                                           bool has_neg, int rep, int G, int g, int q,
+                                           bool &synced) {
+  const int r = (int)ia.x, deg = valid ? (int)ia.z : 0;
+  const int32_t *lc = sd.lcols + (size_t)lidx * kLightCols;
   kmax = __builtin_amdgcn_readfirstlane(kmax);
   const float lr = sd.lr, bound = sd.bound;
   const int c_limit = sd.c_limit;
@@ -532,7 +535,8 @@ __device__ __forceinline__ void light_pack_t(const StepDesc &sd, const RowSrc &s
   // position of edge kb+q (row r past the degree); k - kb >= G only for U > G
   const uint32_t pos_r = row_pos(src.m, (uint32_t)r);
   for (int kb = 0; kb < kmax; kb += G) {
-    int cl = (kb + q < deg) ? gld_int(sd.col + e0 + kb + q) : r;
+    // the first G columns came with the item (pc); kb + q < deg <= kLightCols
+    int cl = (kb + q < deg) ? (kb == 0 ? pc : gld_int(lc + kb + q)) : r;
     cl = cl < c_limit ? cl : r;
     const int pl = (int)row_pos(src.m, (uint32_t)cl);
     const int kend = min(kmax, kb + G);
@@ -593,15 +597,17 @@ __device__ __forceinline__ void gather_edges(const StepDesc &sd, const RowSrc &s
 
 template <int U, int MODE>
 __device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src, uint32_t out,
-                                           u4v ia, int kmax, bool valid, const float (&xr)[8],
-                                           const float *lds_neg, bool has_neg, int rep, int G,
-                                           int g, int q, bool &synced) {
+                                           u4v ia, int lidx, int pc, bool valid,
+                                           const float (&xr)[8], const float *lds_neg,
+                                           bool has_neg, int rep, int G, int g, int q,
+                                           bool &synced) {
+  const int kmax = (int)ia.w;
   if (sd.bound >= 2.0f)
-    light_pack_t<U, MODE, true>(sd, src, out, ia, kmax, valid, xr, lds_neg, has_neg, rep, G, g, q,
-                                synced);
+    light_pack_t<U, MODE, true>(sd, src, out, ia, lidx, pc, kmax, valid, xr, lds_neg, has_neg,
+                                rep, G, g, q, synced);
   else
-    light_pack_t<U, MODE, false>(sd, src, out, ia, kmax, valid, xr, lds_neg, has_neg, rep, G, g, q,
-                                 synced);
+    light_pack_t<U, MODE, false>(sd, src, out, ia, lidx, pc, kmax, valid, xr, lds_neg, has_neg,
+                                 rep, G, g, q, synced);
 }
 
 // One minibatch of one or more jobs. Blocks [0, n_block_items) of a step
@@ -648,6 +654,7 @@ __global__ __launch_bounds__(512) void force_kernel(
   // wave units: medium rows [0, n_wave_items), then light packs
   int u = (lb - nbi) * wpb + wave;
   u4v pre0 = {0u, 0u, 0u, 0u}, pre1 = {0u, 0u, 0u, 0u};
+  int pc = 0; // a light row's column kb + q = q (its first G columns)
   if (lb < nbi) {
     pre0 = gld_u4(sd.items + (size_t)kItemInts * lb);
     pre1 = gld_u4(sd.items + (size_t)kItemInts * lb + 4);
@@ -656,8 +663,9 @@ __global__ __launch_bounds__(512) void force_kernel(
   } else if (u - n_wave_items < n_packs) {
     const int pk = u - n_wave_items;
     const int idx = pk * NG + g;
-    pre0 = gld_u4(lrows + (size_t)kItemInts * (idx < n_light ? idx : pk * NG));
-    pre1 = gld_u4(lrows + (size_t)kItemInts * pk * NG + 4);
+    const int li = idx < n_light ? idx : pk * NG;
+    pre0 = gld_u4(lrows + (size_t)kItemInts * li);
+    if (q < kLightCols) pc = gld_int(sd.lcols + (size_t)li * kLightCols + q);
   }
   // this step's negative ids, one per staging piece (first blockDim pieces)
   const int total8 = sd.ns * G;
@@ -766,14 +774,15 @@ __global__ __launch_bounds__(512) void force_kernel(
     }
     for (int pk = u - n_wave_items; pk < n_packs; pk += W) {
       const int idx = pk * NG + g;
+      const int li = idx < n_light ? idx : pk * NG;
       if (!have) {
-        pre0 = gld_u4(lrows + (size_t)kItemInts * (idx < n_light ? idx : pk * NG));
-        pre1 = gld_u4(lrows + (size_t)kItemInts * pk * NG + 4);
+        pre0 = gld_u4(lrows + (size_t)kItemInts * li);
+        if (q < kLightCols) pc = gld_int(sd.lcols + (size_t)li * kLightCols + q);
         load8_f32(sd.X + (size_t)(pre0.x + own_delta) * D + 8 * q, xr0);
       }
       have = false;
-      light_pack<U, MODE>(sd, src, out, pre0, (int)pre1.x, idx < n_light, xr0, lds, has_neg,
-                          rep, G, g, q, synced);
+      light_pack<U, MODE>(sd, src, out, pre0, li, pc, idx < n_light, xr0, lds, has_neg, rep, G,
+                          g, q, synced);
     }
     if (!synced) __syncthreads();
   }
@@ -841,6 +850,22 @@ __global__ void epoch_kernel(const EpochDesc *__restrict__ jobs, int set, int va
     const int k = i - b * ed.ns;
     const int list = e + b;
     ed.negcur[i] = (list >= 0 && list < ed.n_lists) ? ed.negs[(size_t)list * ed.ns + k] : 0;
+  }
+}
+
+// Light rows' column ids into lcols (see kLightCols), one thread per item
+// and column slot; slots past a row's degree hold 0.
+__global__ void fill_lcols_kernel(const int32_t *__restrict__ items, int n_items,
+                                  const int32_t *__restrict__ col, int32_t *__restrict__ lcols) {
+  const int64_t total = (int64_t)n_items * kLightCols;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int64_t it = i / kLightCols;
+    const int k = (int)(i - it * kLightCols);
+    const int32_t *item = items + (size_t)kItemInts * it;
+    const int l = item[7] - 1;
+    if (l < 0) continue;
+    lcols[(size_t)l * kLightCols + k] = k < item[2] ? col[item[1] + k] : 0;
   }
 }
 
@@ -1023,6 +1048,15 @@ hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, int ro
 hipError_t launch_apply(const StepDesc *steps, const int *blk_prefix, int n_steps,
                         int n_blocks, hipStream_t st) {
   hipLaunchKernelGGL(apply_kernel, dim3(n_blocks), dim3(256), 0, st, steps, blk_prefix, n_steps);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_lcols(const int32_t *items, int n_items, const int32_t *col,
+                             int32_t *lcols, hipStream_t st) {
+  const int64_t total = (int64_t)n_items * kLightCols;
+  const int64_t blocks = std::max<int64_t>(1, std::min<int64_t>((total + 255) / 256, 65536));
+  hipLaunchKernelGGL(fill_lcols_kernel, dim3((unsigned)blocks), dim3(256), 0, st, items, n_items,
+                     col, lcols);
   return hipGetLastError();
 }
 

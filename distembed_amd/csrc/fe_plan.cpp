@@ -31,6 +31,7 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
   p.item_off.assign(p.batches + 1, 0);
   p.block_items.assign(p.batches, 0);
   p.light_rows.assign(p.batches, 0);
+  p.light_off.assign(p.batches + 1, 0);
   p.split_off.assign(p.batches + 1, 0);
   p.pslots.assign(p.batches, 0);
   p.edges_eff.assign(p.batches, 0);
@@ -40,7 +41,7 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
   for (int j = 0; j < p.batches; ++j) {
     const int chunk_j = cuts ? cuts[j].chunk : chunk;
     const int max_chunks_j = cuts ? cuts[j].max_chunks : max_chunks;
-    const int light_j = std::min(cuts ? cuts[j].light : light, chunk_j);
+    const int light_j = std::min(std::min(cuts ? cuts[j].light : light, chunk_j), kLightCols);
     if (chunk_j <= 0 || max_chunks_j <= 0 || light_j < 0) {
       set_host_error("build_plan: invalid cutting");
       return FE_ERR_INVALID_ARG;
@@ -97,12 +98,14 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
     for (size_t i = 0; i < lights.size(); ++i) {
       const int32_t r = lights[i].second;
       const int32_t pack_max = lights[(i / pack) * pack].first;
+      const int32_t l = (int32_t)(p.light_off[j] + (int64_t)i);
       p.items.insert(p.items.end(),
-                     {r, (int32_t)row_ptr[r], lights[i].first, -1, pack_max, 0, 0, 0});
+                     {r, (int32_t)row_ptr[r], lights[i].first, pack_max, 0, 0, 0, 1 + l});
     }
     p.item_off[j + 1] = (int64_t)p.items.size() / kItemInts;
     p.block_items[j] = nblk;
     p.light_rows[j] = (int32_t)lights.size();
+    p.light_off[j + 1] = p.light_off[j] + (int64_t)lights.size();
     p.split_off[j + 1] = (int64_t)p.splits.size() / 4;
     p.pslots[j] = pcount;
     p.max_pslots = std::max(p.max_pslots, (int)pcount);
