@@ -60,6 +60,11 @@ typedef __attribute__((address_space(1))) const int gcint;
 
 __device__ __forceinline__ int gld_int(const int32_t *p) { return *(gcint *)p; }
 
+// Streaming load (nontemporal) of data read once per epoch: column ids.
+__device__ __forceinline__ int gld_int_nt(const int32_t *p) {
+  return __builtin_nontemporal_load((gcint *)p);
+}
+
 __device__ __forceinline__ u4v gld_u4(const int32_t *p) { return *(gcu4 *)p; }
 
 __device__ __forceinline__ void load8_f32(const float *p, float (&o)[8]) {
@@ -83,6 +88,15 @@ __device__ __forceinline__ void store8_f32(float *p, const float (&o)[8]) {
   f4v b = {o[4], o[5], o[6], o[7]};
   ((gf4 *)p)[0] = a;
   ((gf4 *)p)[1] = b;
+}
+
+// Streaming store of a new row (nontemporal): the rows are read back only by
+// later launches, so they need not stay in this launch's L2.
+__device__ __forceinline__ void store8_f32_nt(float *p, const float (&o)[8]) {
+  f4v a = {o[0], o[1], o[2], o[3]};
+  f4v b = {o[4], o[5], o[6], o[7]};
+  __builtin_nontemporal_store(a, (gf4 *)p);
+  __builtin_nontemporal_store(b, (gf4 *)p + 1);
 }
 
 typedef __attribute__((address_space(3))) f4v lf4;
@@ -433,7 +447,7 @@ __device__ __forceinline__ void gather_edges_t(const StepDesc &sd, const RowSrc 
   const uint32_t pos_r = row_pos(src.m, (uint32_t)r);
   for (int eb = 0; eb < cnt; eb += 64) {
     const int m = min(64, cnt - eb);
-    int cl = lane < m ? gld_int(sd.col + e0 + eb + lane) : r;
+    int cl = lane < m ? gld_int_nt(sd.col + e0 + eb + lane) : r;
     cl = cl < c_limit ? cl : r;
     const int pl = (int)row_pos(src.m, (uint32_t)cl);
     const int rounds = (m + NG - 1) / NG;
@@ -484,14 +498,15 @@ __device__ __forceinline__ void write_row(const StepDesc &sd, uint32_t out, int 
     return;
   }
   const size_t pos = (size_t)(out + (uint32_t)t);
-  store8_f32(sd.Xw + pos * sd.D + 8 * q, o);
+  store8_f32_nt(sd.Xw + pos * sd.D + 8 * q, o);
   if (use_bf16<MODE>(sd)) {
     u4v w;
     w.x = bf16x2(o[0], o[1]);
     w.y = bf16x2(o[2], o[3]);
     w.z = bf16x2(o[4], o[5]);
     w.w = bf16x2(o[6], o[7]);
-    *(__attribute__((address_space(1))) u4v *)(sd.Xsw + pos * sd.D + 8 * q) = w;
+    __builtin_nontemporal_store(w, (__attribute__((address_space(1))) u4v *)(sd.Xsw + pos * sd.D +
+                                                                          8 * q));
   }
 }
 
@@ -536,7 +551,7 @@ __device__ __forceinline__ void light_pack_t(const StepDesc &sd, const RowSrc &s
   const uint32_t pos_r = row_pos(src.m, (uint32_t)r);
   for (int kb = 0; kb < kmax; kb += G) {
     // the first G columns came with the item (pc); kb + q < deg <= kLightCols
-    int cl = (kb + q < deg) ? (kb == 0 ? pc : gld_int(lc + kb + q)) : r;
+    int cl = (kb + q < deg) ? (kb == 0 ? pc : gld_int_nt(lc + kb + q)) : r;
     cl = cl < c_limit ? cl : r;
     const int pl = (int)row_pos(src.m, (uint32_t)cl);
     const int kend = min(kmax, kb + G);
@@ -665,7 +680,7 @@ __global__ __launch_bounds__(512) void force_kernel(
     const int idx = pk * NG + g;
     const int li = idx < n_light ? idx : pk * NG;
     pre0 = gld_u4(lrows + (size_t)kItemInts * li);
-    if (q < kLightCols) pc = gld_int(sd.lcols + (size_t)li * kLightCols + q);
+    if (q < kLightCols) pc = gld_int_nt(sd.lcols + (size_t)li * kLightCols + q);
   }
   // this step's negative ids, one per staging piece (first blockDim pieces)
   const int total8 = sd.ns * G;
@@ -777,7 +792,7 @@ __global__ __launch_bounds__(512) void force_kernel(
       const int li = idx < n_light ? idx : pk * NG;
       if (!have) {
         pre0 = gld_u4(lrows + (size_t)kItemInts * li);
-        if (q < kLightCols) pc = gld_int(sd.lcols + (size_t)li * kLightCols + q);
+        if (q < kLightCols) pc = gld_int_nt(sd.lcols + (size_t)li * kLightCols + q);
         load8_f32(sd.X + (size_t)(pre0.x + own_delta) * D + 8 * q, xr0);
       }
       have = false;
