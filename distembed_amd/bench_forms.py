@@ -9,14 +9,17 @@ batching: 512 independent SBM graphs (n log-uniform in [1e3, 2e5], 4-32
   graphs (fe_engine_run_graph), tiles chosen by the autotuner.
 serving: Poisson arrivals of R-MAT jobs (edge count log-uniform in
   [2e4, 2e7], d from {32, 64, 128}), 50 epochs, 5 negatives, admitted under
-  an HBM budget and interleaved round by round (fe_engine_run); job latency
-  is arrival -> result on the host, reported as p50/p99 with the aggregate
-  edges+negatives/s, at three offered loads.
+  an HBM budget and interleaved round by round (fe_engine_run); submission
+  and collection run on their own threads. Job latency is arrival -> result
+  on the host, reported as p50/p99 with the aggregate edges+negatives/s, at
+  three offered loads.
 """
 from __future__ import annotations
 
 import math
+import queue
 import sys
+import threading
 import time
 from concurrent.futures import ThreadPoolExecutor
 
@@ -170,6 +173,9 @@ def rmat_job_pool(count: int, seed: int = 1):
 def serving(qps_levels=(4.0, 8.0, 16.0), jobs_per_level: int = 48, epochs: int = 50,
             B: int = 16384, ns: int = 5, budget_gib: float = 32.0,
             rounds_per_call: int = 16) -> dict:
+    """One thread submits jobs as they arrive (retrying while the budget is
+    full), one collects finished jobs, and the main thread runs rounds: a
+    job's upload and download overlap the other jobs' rounds."""
     pool = rmat_job_pool(jobs_per_level)
     X0s = [api.init_uniform(g.n, d, -0.5, 0.5, seed=i) for i, (g, d) in enumerate(pool)]
     units = [_units(g, min(B, g.n), ns, epochs) for g, _ in pool]
@@ -182,36 +188,78 @@ def serving(qps_levels=(4.0, 8.0, 16.0), jobs_per_level: int = 48, epochs: int =
         arrivals -= arrivals[0]
         lat = [None] * len(pool)
         with api.Engine(int(budget_gib * 2 ** 30)) as eng:
-            nxt = 0
+            lock = threading.Lock()
+            freed = threading.Condition()
             running = {}  # job id -> pool index
-            waiting = []  # arrived, not admitted
-            done_units = 0
+            all_submitted = threading.Event()
+            to_collect: queue.Queue = queue.Queue()
+            errors = []
             t0 = time.perf_counter()
-            while nxt < len(pool) or waiting or running:
-                now = time.perf_counter() - t0
-                while nxt < len(pool) and arrivals[nxt] <= now:
-                    waiting.append(nxt)
-                    nxt += 1
-                while waiting:
-                    i = waiting[0]
-                    g, d = pool[i]
+
+            def submitter():
+                try:
+                    for i, (g, d) in enumerate(pool):
+                        wait = arrivals[i] - (time.perf_counter() - t0)
+                        if wait > 0:
+                            time.sleep(wait)
+                        while True:
+                            try:
+                                jid = eng.submit(g, X0s[i], min(B, g.n), epochs, ns=ns)
+                                break
+                            except api.FeError as e:
+                                if e.status != 4:  # not over budget
+                                    raise
+                                with freed:
+                                    freed.wait(0.002)
+                        with lock:
+                            running[jid] = i
+                except Exception as e:  # reported by the main thread
+                    errors.append(e)
+                finally:
+                    all_submitted.set()
+
+            def collector():
+                while True:
+                    item = to_collect.get()
+                    if item is None:
+                        return
+                    jid, i = item
                     try:
-                        jid = eng.submit(g, X0s[i], min(B, g.n), epochs, ns=ns)
-                    except api.FeError as e:
-                        if e.status == 4:  # over budget: wait for memory to free up
-                            break
-                        raise
-                    running[jid] = i
-                    waiting.pop(0)
-                if running:
-                    eng.run(rounds_per_call)
-                    for jid in [j for j in running if eng.done(j)]:
-                        i = running.pop(jid)
                         eng.collect(jid)
-                        lat[i] = time.perf_counter() - t0 - arrivals[i]
-                        done_units += units[i]
-                elif nxt < len(pool):
-                    time.sleep(max(0.0, min(0.01, arrivals[nxt] - (time.perf_counter() - t0))))
+                    except Exception as e:
+                        errors.append(e)
+                    lat[i] = time.perf_counter() - t0 - arrivals[i]
+                    with freed:
+                        freed.notify_all()
+
+            threads = [threading.Thread(target=submitter), threading.Thread(target=collector)]
+            for t in threads:
+                t.start()
+            done_units = 0
+            while not errors:
+                with lock:
+                    ids = list(running.items())
+                if not ids:
+                    if all_submitted.is_set():
+                        with lock:
+                            if not running:
+                                break
+                    time.sleep(0.0005)
+                    continue
+                left = eng.run(rounds_per_call)
+                finished = [(jid, i) for jid, i in ids if eng.done(jid)]
+                for jid, i in finished:
+                    with lock:
+                        running.pop(jid)
+                    done_units += units[i]
+                    to_collect.put((jid, i))
+                if left == 0 and not finished:
+                    time.sleep(0.0002)  # every round is enqueued: wait for the device
+            to_collect.put(None)
+            for t in threads:
+                t.join()
+            if errors:
+                raise errors[0]
             span = time.perf_counter() - t0
         lat_ms = np.array(lat) * 1e3
         levels.append({"offered_qps": qps, "jobs": len(pool), "span_s": span,
@@ -223,4 +271,4 @@ def serving(qps_levels=(4.0, 8.0, 16.0), jobs_per_level: int = 48, epochs: int =
         log(f"serving qps={qps}: p50 {levels[-1]['p50_ms']:.1f} ms, p99 {levels[-1]['p99_ms']:.1f} ms, "
             f"{done_units / span / 1e9:.2f} G units/s over {span:.1f} s")
     return {"form": "serving", "epochs": epochs, "batch": B, "ns": ns, "budget_gib": budget_gib,
-            "levels": levels}
+            "rounds_per_call": rounds_per_call, "levels": levels}

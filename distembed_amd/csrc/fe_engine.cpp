@@ -5,15 +5,21 @@
 // list), so admission is a pure accounting question and nothing else in the
 // process competes for that memory. Every engine round advances each
 // running job by one minibatch: the round's step descriptors (one per job)
-// go to the device in one copy and all jobs' work items share one force
-// launch and one apply launch. A job's work items, chunking and summation
-// order are fixed when it is submitted, and no job reads or writes another
-// job's memory, so its result does not depend on the mix it ran in.
+// go to the device in one copy and all jobs' work items share one fused
+// force launch. A job's work items, chunking and summation order are fixed
+// when it is submitted, and no job reads or writes another job's memory, so
+// its result does not depend on the mix it ran in.
+//
+// Threads: fe_engine_submit and fe_engine_collect do their host work and
+// their copies (on their own streams) without holding the engine's lock, so
+// a submitting or collecting thread does not stall the thread that runs
+// rounds. Every other call holds the lock for its whole duration.
 #include <algorithm>
 #include <chrono>
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -25,7 +31,7 @@ namespace {
 
 constexpr size_t kAlign = 256;
 
-// First-fit arena over one device allocation.
+// First-fit arena over one device allocation (thread-safe).
 class Arena : public DeviceAllocator {
 public:
   fe_status init(size_t bytes, std::string &err) {
@@ -45,6 +51,7 @@ public:
     if (base_) hipFree(base_);
   }
   hipError_t alloc(void **p, size_t bytes) override {
+    std::lock_guard<std::mutex> lk(mu_);
     bytes = std::max<size_t>((bytes + kAlign - 1) / kAlign * kAlign, kAlign);
     for (auto it = free_.begin(); it != free_.end(); ++it) {
       if (it->second >= bytes) {
@@ -61,6 +68,7 @@ public:
   }
   void release(void *p) override {
     if (!p) return;
+    std::lock_guard<std::mutex> lk(mu_);
     const size_t off = static_cast<char *>(p) - static_cast<char *>(base_);
     auto u = used_.find(off);
     if (u == used_.end()) return;
@@ -83,15 +91,20 @@ public:
     }
     free_[start] = len;
   }
-  size_t in_use() const { return in_use_; }
+  size_t in_use() const {
+    std::lock_guard<std::mutex> lk(mu_);
+    return in_use_;
+  }
   size_t size() const { return size_; }
   size_t largest_free() const {
+    std::lock_guard<std::mutex> lk(mu_);
     size_t m = 0;
     for (auto &f : free_) m = std::max(m, f.second);
     return m;
   }
 
 private:
+  mutable std::mutex mu_;
   void *base_ = nullptr;
   size_t size_ = 0;
   size_t in_use_ = 0;
@@ -106,6 +119,7 @@ struct EngineJob {
   int epoch = 0;        // next step (epoch, batch)
   int batch = 0;
   bool launched_all = false;
+  bool collecting = false; // a collect is downloading the result
   bool collected = false;
   hipEvent_t uploaded = nullptr; // uploads finished (on the upload stream)
   hipEvent_t done = nullptr;     // the job's last round finished
@@ -120,9 +134,11 @@ struct EngineJob {
 constexpr int kStages = 8;
 
 struct fe_engine {
+  std::mutex mu;                   // everything below except the streams
   int device = 0;
   hipStream_t stream = nullptr;    // rounds
-  hipStream_t upload = nullptr;    // job uploads and result downloads
+  hipStream_t upload = nullptr;    // job uploads
+  hipStream_t download = nullptr;  // result downloads
   std::unique_ptr<Arena> arena;
   std::vector<std::unique_ptr<EngineJob>> jobs;
   Tuning tun;
@@ -140,13 +156,13 @@ struct fe_engine {
   double graph_prepare_ms = 0; // last fe_engine_run_graph: schedule, capture, instantiate
   double graph_exec_ms = 0;    // ... and the replay itself (HIP events)
   int64_t graph_rounds = 0;
-  std::string err;
 };
 
-static thread_local std::string g_engine_create_error;
+// The calling thread's last engine failure (threads may share an engine).
+static thread_local std::string g_engine_error;
 
-static fe_status efail(fe_engine *e, fe_status s, const std::string &m) {
-  e->err = m;
+static fe_status efail(fe_engine *, fe_status s, const std::string &m) {
+  g_engine_error = m;
   return s;
 }
 
@@ -226,18 +242,18 @@ static fe_status plan_job(const fe_job_desc *d, const Tuning &tun, Job &tmp, std
 
 extern "C" fe_status fe_engine_create(fe_engine **out, int device, int64_t budget) {
   if (!out || budget <= 0) {
-    g_engine_create_error = "fe_engine_create: invalid argument";
+    g_engine_error = "fe_engine_create: invalid argument";
     return FE_ERR_INVALID_ARG;
   }
   *out = nullptr;
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) {
     (void)hipGetLastError();
-    g_engine_create_error = "fe_engine_create: no HIP device available";
+    g_engine_error = "fe_engine_create: no HIP device available";
     return FE_ERR_NO_DEVICE;
   }
   if (device < 0 || device >= count) {
-    g_engine_create_error = "fe_engine_create: device index out of range";
+    g_engine_error = "fe_engine_create: device index out of range";
     return FE_ERR_NO_DEVICE;
   }
   auto *e = new (std::nothrow) fe_engine();
@@ -246,15 +262,16 @@ extern "C" fe_status fe_engine_create(fe_engine **out, int device, int64_t budge
   fe_status s = ehip(e, hipSetDevice(device), "hipSetDevice");
   if (s == FE_OK) s = ehip(e, hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking), "stream");
   if (s == FE_OK) s = ehip(e, hipStreamCreateWithFlags(&e->upload, hipStreamNonBlocking), "stream");
+  if (s == FE_OK) s = ehip(e, hipStreamCreateWithFlags(&e->download, hipStreamNonBlocking), "stream");
   for (auto &st : e->stage)
     if (s == FE_OK) s = ehip(e, hipEventCreateWithFlags(&st.copied, hipEventDisableTiming), "event");
   if (s == FE_OK) {
     e->arena.reset(new Arena());
-    s = e->arena->init((size_t)budget, e->err);
+    s = e->arena->init((size_t)budget, g_engine_error);
   }
   if (s == FE_OK) s = ensure_stage(e, 64);
   if (s != FE_OK) {
-    g_engine_create_error = "fe_engine_create: " + e->err;
+    g_engine_error = "fe_engine_create: " + g_engine_error;
     fe_engine_destroy(e);
     return s;
   }
@@ -267,6 +284,7 @@ extern "C" void fe_engine_destroy(fe_engine *e) {
   hipSetDevice(e->device);
   if (e->stream) hipStreamSynchronize(e->stream);
   if (e->upload) hipStreamSynchronize(e->upload);
+  if (e->download) hipStreamSynchronize(e->download);
   for (auto &jp : e->jobs) {
     job_free(jp->job);
     if (jp->uploaded) hipEventDestroy(jp->uploaded);
@@ -281,24 +299,35 @@ extern "C" void fe_engine_destroy(fe_engine *e) {
   e->arena.reset();
   if (e->stream) hipStreamDestroy(e->stream);
   if (e->upload) hipStreamDestroy(e->upload);
+  if (e->download) hipStreamDestroy(e->download);
   delete e;
 }
 
-extern "C" const char *fe_engine_last_error(const fe_engine *e) {
-  return e ? e->err.c_str() : g_engine_create_error.c_str();
+extern "C" const char *fe_engine_last_error(const fe_engine *) {
+  return g_engine_error.c_str();
 }
 
-extern "C" int64_t fe_engine_job_bytes(const fe_engine *e, const fe_job_desc *d) {
+static Tuning engine_tuning(const fe_engine *e) {
+  if (!e) return Tuning();
+  std::lock_guard<std::mutex> lk(const_cast<fe_engine *>(e)->mu);
+  return e->tun;
+}
+
+static int64_t job_bytes_for(const fe_job_desc *d, const Tuning &tun) {
   Job tmp;
   std::string err;
-  const Tuning tun = e ? e->tun : Tuning();
   if (plan_job(d, tun, tmp, err) != FE_OK) return -1;
   // arena rounding: every buffer is a multiple of 256 bytes
   return job_bytes_estimate(tmp) + 16 * (int64_t)kAlign;
 }
 
+extern "C" int64_t fe_engine_job_bytes(const fe_engine *e, const fe_job_desc *d) {
+  return job_bytes_for(d, engine_tuning(e));
+}
+
 extern "C" fe_status fe_engine_set_tuning(fe_engine *e, const fe_tuning *t) {
   if (!e || !t) return FE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(e->mu);
   Tuning nt = e->tun;
   if (t->chunk) nt.chunk = t->chunk;
   if (t->max_chunks) nt.max_chunks = t->max_chunks;
@@ -313,16 +342,22 @@ extern "C" fe_status fe_engine_set_tuning(fe_engine *e, const fe_tuning *t) {
   return FE_OK;
 }
 
+// The job is built and uploaded without the engine's lock (the arena locks
+// itself); only its publication in the job list takes the lock.
 extern "C" fe_status fe_engine_submit(fe_engine *e, const fe_job_desc *d, int *job_id) {
   if (!e || !job_id) return FE_ERR_INVALID_ARG;
   fe_status s = ehip(e, hipSetDevice(e->device), "hipSetDevice");
   if (s != FE_OK) return s;
-  const int64_t need = fe_engine_job_bytes(e, d);
-  if (need < 0) {
+  const Tuning tun = engine_tuning(e);
+  // the plan sizes the job and is then handed to it (built once)
+  Job sizing;
+  std::string perr;
+  if (plan_job(d, tun, sizing, perr) != FE_OK) {
     std::string err;
     validate_desc(d, err);
-    return efail(e, FE_ERR_INVALID_ARG, "fe_engine_submit: " + (err.empty() ? std::string(host_error()) : err));
+    return efail(e, FE_ERR_INVALID_ARG, "fe_engine_submit: " + (err.empty() ? perr : err));
   }
+  const int64_t need = job_bytes_estimate(sizing) + 16 * (int64_t)kAlign;
   if ((size_t)need > e->arena->size() - e->arena->in_use())
     return efail(e, FE_ERR_OVER_BUDGET, "fe_engine_submit: job needs " + std::to_string(need) +
                                             " bytes, " +
@@ -331,8 +366,10 @@ extern "C" fe_status fe_engine_submit(fe_engine *e, const fe_job_desc *d, int *j
   auto ej = std::make_unique<EngineJob>();
   Job &j = ej->job;
   j.allocator = e->arena.get();
-  j.tun = e->tun;
+  j.tun = tun;
   s = job_init(j, d->n, d->d, d->gather_dtype, d->quirks);
+  j.pending_plan = std::move(sizing.plan);
+  j.pending_B = d->batch_size;
   // X at its full slot size from the start: no second copy while growing
   if (s == FE_OK) s = job_reserve_slots(j, d->batch_size);
   if (s == FE_OK) s = job_upload_csr(j, d->row_ptr, d->col_idx, d->nnz, e->upload);
@@ -359,7 +396,7 @@ extern "C" fe_status fe_engine_submit(fe_engine *e, const fe_job_desc *d, int *j
   if (s == FE_OK) s = ehip(e, hipEventCreateWithFlags(&ej->done, hipEventDisableTiming), "event");
   if (s == FE_OK) s = ehip(e, hipEventRecord(ej->uploaded, e->upload), "event");
   if (s != FE_OK) {
-    if (e->err.empty() || s != FE_ERR_HIP) e->err = "fe_engine_submit: " + j.err;
+    if (g_engine_error.empty() || s != FE_ERR_HIP) g_engine_error = "fe_engine_submit: " + j.err;
     if (s == FE_ERR_OOM) s = FE_ERR_OVER_BUDGET;
     hipStreamSynchronize(e->upload);
     job_free(j);
@@ -369,8 +406,9 @@ extern "C" fe_status fe_engine_submit(fe_engine *e, const fe_job_desc *d, int *j
   }
   ej->iterations = d->iterations;
   ej->launched_all = d->iterations == 0;
-  ej->id = (int)e->jobs.size();
   if (ej->launched_all) hipEventRecord(ej->done, e->upload);
+  std::lock_guard<std::mutex> lk(e->mu);
+  ej->id = (int)e->jobs.size();
   *job_id = ej->id;
   e->jobs.push_back(std::move(ej));
   return FE_OK;
@@ -440,6 +478,7 @@ static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *
 
 extern "C" fe_status fe_engine_run(fe_engine *e, int rounds, int *running) {
   if (!e) return FE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(e->mu);
   fe_status s = ehip(e, hipSetDevice(e->device), "hipSetDevice");
   if (s != FE_OK) return s;
   int done_rounds = 0;
@@ -493,10 +532,12 @@ extern "C" fe_status fe_engine_run(fe_engine *e, int rounds, int *running) {
 }
 
 extern "C" fe_status fe_engine_job_done(fe_engine *e, int job_id, int *done) {
-  if (!e || !done || job_id < 0 || job_id >= (int)e->jobs.size())
-    return e ? efail(e, FE_ERR_INVALID_ARG, "fe_engine_job_done: bad job id") : FE_ERR_INVALID_ARG;
+  if (!e || !done) return FE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(e->mu);
+  if (job_id < 0 || job_id >= (int)e->jobs.size())
+    return efail(e, FE_ERR_INVALID_ARG, "fe_engine_job_done: bad job id");
   EngineJob *ej = e->jobs[job_id].get();
-  if (ej->collected) {
+  if (ej->collected || ej->collecting) {
     *done = 1;
     return FE_OK;
   }
@@ -515,19 +556,33 @@ extern "C" fe_status fe_engine_job_done(fe_engine *e, int job_id, int *done) {
   return FE_OK;
 }
 
+// The download runs without the engine's lock, on its own stream, after the
+// job's last round: rounds of the other jobs go on meanwhile.
 extern "C" fe_status fe_engine_collect(fe_engine *e, int job_id, float *x_out) {
-  if (!e || !x_out || job_id < 0 || job_id >= (int)e->jobs.size())
-    return e ? efail(e, FE_ERR_INVALID_ARG, "fe_engine_collect: bad argument") : FE_ERR_INVALID_ARG;
+  if (!e || !x_out) return e ? efail(e, FE_ERR_INVALID_ARG, "fe_engine_collect: bad argument")
+                             : FE_ERR_INVALID_ARG;
   fe_status s = ehip(e, hipSetDevice(e->device), "hipSetDevice");
   if (s != FE_OK) return s;
-  EngineJob *ej = e->jobs[job_id].get();
-  if (ej->collected) return efail(e, FE_ERR_STATE, "fe_engine_collect: already collected");
-  if (!ej->launched_all) return efail(e, FE_ERR_STATE, "fe_engine_collect: job still running");
-  // the download waits for the job's last round only, on the upload stream
-  if ((s = ehip(e, hipStreamWaitEvent(e->upload, ej->done, 0), "wait done")) != FE_OK) return s;
+  EngineJob *ej = nullptr;
+  {
+    std::lock_guard<std::mutex> lk(e->mu);
+    if (job_id < 0 || job_id >= (int)e->jobs.size())
+      return efail(e, FE_ERR_INVALID_ARG, "fe_engine_collect: bad job id");
+    ej = e->jobs[job_id].get();
+    if (ej->collected || ej->collecting)
+      return efail(e, FE_ERR_STATE, "fe_engine_collect: already collected");
+    if (!ej->launched_all) return efail(e, FE_ERR_STATE, "fe_engine_collect: job still running");
+    ej->collecting = true;
+  }
   Job &j = ej->job;
-  s = job_download_x(j, x_out, e->upload);
-  if (s != FE_OK) return efail(e, s, j.err);
+  s = ehip(e, hipStreamWaitEvent(e->download, ej->done, 0), "wait done");
+  if (s == FE_OK) {
+    s = job_download_x(j, x_out, e->download);
+    if (s != FE_OK) efail(e, s, j.err);
+  }
+  std::lock_guard<std::mutex> lk(e->mu);
+  ej->collecting = false;
+  if (s != FE_OK) return s;
   job_free(j);
   hipEventDestroy(ej->uploaded);
   hipEventDestroy(ej->done);
@@ -539,6 +594,7 @@ extern "C" fe_status fe_engine_collect(fe_engine *e, int job_id, float *x_out) {
 extern "C" fe_status fe_engine_stats(const fe_engine *e, int64_t *bytes_in_use,
                                      int64_t *rounds_run, int64_t *units_done) {
   if (!e) return FE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(const_cast<fe_engine *>(e)->mu);
   if (bytes_in_use) *bytes_in_use = (int64_t)e->arena->in_use();
   if (rounds_run) *rounds_run = e->rounds_run;
   if (units_done) *units_done = e->units_done;
@@ -550,6 +606,7 @@ extern "C" fe_status fe_engine_stats(const fe_engine *e, int64_t *bytes_in_use,
 // graphs (segments of kGraphRounds rounds), each replayed once.
 extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
   if (!e) return FE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(e->mu);
   fe_status s = ehip(e, hipSetDevice(e->device), "hipSetDevice");
   if (s != FE_OK) return s;
   std::vector<EngineJob *> all;
@@ -667,6 +724,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
 extern "C" fe_status fe_engine_graph_timing(const fe_engine *e, double *prepare_ms,
                                             double *exec_ms, int64_t *rounds) {
   if (!e) return FE_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(const_cast<fe_engine *>(e)->mu);
   if (prepare_ms) *prepare_ms = e->graph_prepare_ms;
   if (exec_ms) *exec_ms = e->graph_exec_ms;
   if (rounds) *rounds = e->graph_rounds;

@@ -3,6 +3,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 
 #include "fe_job.hpp"
 
@@ -237,21 +238,30 @@ fe_status job_upload_csr(Job &j, const int64_t *row_ptr, const int32_t *col, int
     j.err = "fe_upload_csr: row_ptr[0] must be 0 and row_ptr[n] must equal nnz";
     return FE_ERR_INVALID_ARG;
   }
+  // row_ptr monotone first (then every row range lies in [0, nnz)), then
+  // the column ids and each row's count of skipped columns, in parallel
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int64_t r = 0; r < j.n; ++r) bad |= row_ptr[r + 1] < row_ptr[r] ? 1 : 0;
+  if (bad) {
+    j.err = "fe_upload_csr: row_ptr is not monotone";
+    return FE_ERR_INVALID_ARG;
+  }
   std::vector<int32_t> skip((size_t)j.n, 0);
-  for (int64_t r = 0; r < j.n; ++r) {
-    if (row_ptr[r + 1] < row_ptr[r]) {
-      j.err = "fe_upload_csr: row_ptr is not monotone";
-      return FE_ERR_INVALID_ARG;
-    }
+  const int64_t n = j.n, c_limit = j.c_limit;
+#pragma omp parallel for schedule(dynamic, 4096) reduction(| : bad)
+  for (int64_t r = 0; r < n; ++r) {
     int32_t sk = 0;
     for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e) {
-      if (col[e] < 0 || col[e] >= j.n) {
-        j.err = "fe_upload_csr: column id out of [0, n)";
-        return FE_ERR_INVALID_ARG;
-      }
-      sk += (col[e] >= j.c_limit) ? 1 : 0;
+      const int32_t c = col[e];
+      bad |= (c < 0 || c >= n) ? 1 : 0;
+      sk += (c >= c_limit) ? 1 : 0;
     }
     skip[r] = sk;
+  }
+  if (bad) {
+    j.err = "fe_upload_csr: column id out of [0, n)";
+    return FE_ERR_INVALID_ARG;
   }
   fe_status s = ensure_cap(j, j.col, j.cap_col, (size_t)nnz, "CSR columns");
   if (s != FE_OK) return s;
@@ -285,6 +295,79 @@ fe_status job_upload_x(Job &j, const float *x, hipStream_t st) {
   return FE_OK;
 }
 
+// Device-to-host copies through pinned memory: two 16 MiB chunks per
+// device (allocated on first use, kept for the process), the DMA of one
+// overlapping the parallel host copy of the other out of the other. Copies
+// on one device share its chunks and take turns.
+namespace {
+struct PinnedStage {
+  std::mutex mu;
+  char *buf[2] = {nullptr, nullptr};
+  hipEvent_t ev[2] = {nullptr, nullptr};
+};
+constexpr size_t kStageChunk = (size_t)16 << 20;
+constexpr int kMaxDevices = 64;
+PinnedStage g_stage[kMaxDevices];
+
+struct CopySeg {
+  void *dst;
+  const void *src;
+  size_t bytes;
+};
+
+void host_copy(void *dst, const void *src, size_t bytes) {
+  const size_t block = (size_t)1 << 20;
+  const int64_t nblk = (int64_t)((bytes + block - 1) / block);
+#pragma omp parallel for schedule(static) if (nblk > 1)
+  for (int64_t b = 0; b < nblk; ++b) {
+    const size_t off = (size_t)b * block;
+    std::memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off,
+                std::min(block, bytes - off));
+  }
+}
+
+hipError_t d2h_staged(const std::vector<CopySeg> &segs, hipStream_t st) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  PinnedStage &ps = g_stage[dev];
+  std::lock_guard<std::mutex> lk(ps.mu);
+  for (int s = 0; s < 2 && e == hipSuccess; ++s) {
+    if (!ps.buf[s]) e = hipHostMalloc((void **)&ps.buf[s], kStageChunk, hipHostMallocDefault);
+    if (e == hipSuccess && !ps.ev[s]) e = hipEventCreateWithFlags(&ps.ev[s], hipEventDisableTiming);
+  }
+  if (e != hipSuccess) return e;
+  CopySeg held[2] = {{nullptr, nullptr, 0}, {nullptr, nullptr, 0}}; // piece in each chunk
+  auto drain = [&](int s) -> hipError_t {
+    if (!held[s].bytes) return hipSuccess;
+    hipError_t r = hipEventSynchronize(ps.ev[s]);
+    if (r == hipSuccess) host_copy(held[s].dst, ps.buf[s], held[s].bytes);
+    held[s].bytes = 0;
+    return r;
+  };
+  int k = 0;
+  for (const CopySeg &sg : segs) {
+    for (size_t off = 0; off < sg.bytes && e == hipSuccess; off += kStageChunk) {
+      const int s = k & 1;
+      e = drain(s);
+      const size_t len = std::min(kStageChunk, sg.bytes - off);
+      const char *src = static_cast<const char *>(sg.src) + off;
+      if (e == hipSuccess) e = hipMemcpyAsync(ps.buf[s], src, len, hipMemcpyDeviceToHost, st);
+      if (e == hipSuccess) e = hipEventRecord(ps.ev[s], st);
+      if (e == hipSuccess) held[s] = {static_cast<char *>(sg.dst) + off, src, len};
+      ++k;
+    }
+  }
+  // the older of the two chunks first
+  for (int i = 0; i < 2; ++i) {
+    hipError_t r = drain((k + i) & 1);
+    if (e == hipSuccess) e = r;
+  }
+  return e;
+}
+} // namespace
+
 fe_status job_download_x(Job &j, float *x, hipStream_t st) {
   if (!x) {
     j.err = "fe_download_x: null pointer";
@@ -294,21 +377,20 @@ fe_status job_download_x(Job &j, float *x, hipStream_t st) {
     j.err = "fe_download_x: X was never uploaded";
     return FE_ERR_STATE;
   }
+  std::vector<CopySeg> segs;
   if (j.rot == 0) {
-    FE_HIP(j, hipMemcpyAsync(x, j.X, sizeof(float) * (size_t)j.n * j.d, hipMemcpyDeviceToHost, st),
-           "fe_download_x: copy");
+    segs.push_back({x, j.X, sizeof(float) * (size_t)j.n * j.d});
   } else {
-    // one copy per batch from its slot
+    // one piece per batch, from its slot
     const Layout L = make_layout(j.lay_B, (int)((j.n + j.lay_B - 1) / j.lay_B));
     for (int k = 0; k < L.P - 1; ++k) {
       const int64_t r0 = (int64_t)k * L.B;
       const int64_t rows = std::min<int64_t>(r0 + L.B, j.n) - r0;
-      FE_HIP(j, hipMemcpyAsync(x + r0 * j.d, j.X + rest_pos(j, L, j.rot, r0) * j.d,
-                               sizeof(float) * rows * j.d, hipMemcpyDeviceToHost, st),
-             "fe_download_x: copy");
+      segs.push_back({x + r0 * j.d, j.X + rest_pos(j, L, j.rot, r0) * j.d,
+                      sizeof(float) * (size_t)rows * j.d});
     }
   }
-  FE_HIP(j, hipStreamSynchronize(st), "fe_download_x: sync");
+  FE_HIP(j, d2h_staged(segs, st), "fe_download_x: copy");
   return FE_OK;
 }
 
@@ -478,13 +560,20 @@ fe_status job_prepare(Job &j, hipStream_t st, bool slots) {
     Plan p;
     const int nbp = (int)((j.n + j.B - 1) / j.B);
     const bool per_batch = (int)j.bcut.size() == nbp;
-    fe_status s = build_plan(j.n, j.row_ptr.data(), nullptr, j.B, j.tun.chunk,
-                             j.tun.max_chunks, j.tun.light, 64 / j.G, j.c_limit, &p,
-                             per_batch ? j.bcut.data() : nullptr);
-    if (s != FE_OK) {
-      j.err = host_error();
-      return s;
+    fe_status s = FE_OK;
+    if (j.pending_B == j.B && !per_batch) {
+      p = std::move(j.pending_plan);
+    } else {
+      s = build_plan(j.n, j.row_ptr.data(), nullptr, j.B, j.tun.chunk,
+                               j.tun.max_chunks, j.tun.light, 64 / j.G, j.c_limit, &p,
+                               per_batch ? j.bcut.data() : nullptr);
+      if (s != FE_OK) {
+        j.err = host_error();
+        return s;
+      }
     }
+    j.pending_plan = Plan();
+    j.pending_B = 0;
     // processed entries per batch from the per-row skip counts
     p.edges_eff_total = 0;
     for (int b = 0; b < p.batches; ++b) {

@@ -88,6 +88,10 @@ struct Job {
   std::vector<Shape> bshape; // per batch (from the autotuner); empty: tun's shape
   std::vector<Cut> bcut;     // per batch cutting; empty: tun's chunk / max_chunks / light
   Plan plan;
+  // a plan built ahead (by the engine's admission) for batch size
+  // pending_B and the tuning in force; taken by the next plan rebuild
+  Plan pending_plan;
+  int pending_B = 0;
   std::vector<int> force_blocks; // per batch
   std::vector<int> apply_blocks; // per batch
   // device buffers

@@ -28,36 +28,37 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
     p.batches = (int)(n / B) + 1;
     p.last = (int)(n - (int64_t)B * (p.batches - 1));
   }
-  p.item_off.assign(p.batches + 1, 0);
-  p.block_items.assign(p.batches, 0);
-  p.light_rows.assign(p.batches, 0);
-  p.light_off.assign(p.batches + 1, 0);
-  p.split_off.assign(p.batches + 1, 0);
-  p.pslots.assign(p.batches, 0);
-  p.edges_eff.assign(p.batches, 0);
-  p.items.reserve((size_t)kItemInts * (size_t)(n + (row_ptr[n] / chunk) + 16));
-  std::vector<std::pair<int32_t, int32_t>> waves;  // (degree, row)
-  std::vector<std::pair<int32_t, int32_t>> lights; // (degree, row)
-  for (int j = 0; j < p.batches; ++j) {
-    const int chunk_j = cuts ? cuts[j].chunk : chunk;
-    const int max_chunks_j = cuts ? cuts[j].max_chunks : max_chunks;
-    const int light_j = std::min(std::min(cuts ? cuts[j].light : light, chunk_j), kLightCols);
-    if (chunk_j <= 0 || max_chunks_j <= 0 || light_j < 0) {
+  for (int j = 0; cuts && j < p.batches; ++j) {
+    if (cuts[j].chunk <= 0 || cuts[j].max_chunks <= 0 || cuts[j].light < 0) {
       set_host_error("build_plan: invalid cutting");
       return FE_ERR_INVALID_ARG;
     }
+  }
+  // the batches are planned independently (in parallel), then concatenated
+  struct BatchPlan {
+    std::vector<int32_t> items;  // block items, then medium rows, then light rows
+    std::vector<int32_t> splits; // {row, p0, k, 0} per heavy row
+    int32_t nblk = 0, nlight = 0, pcount = 0, scount = 0;
+    int64_t eff = 0;
+  };
+  std::vector<BatchPlan> bp((size_t)p.batches);
+#pragma omp parallel for schedule(dynamic, 1)
+  for (int j = 0; j < p.batches; ++j) {
+    BatchPlan &o = bp[(size_t)j];
+    const int chunk_j = cuts ? cuts[j].chunk : chunk;
+    const int max_chunks_j = cuts ? cuts[j].max_chunks : max_chunks;
+    const int light_j = std::min(std::min(cuts ? cuts[j].light : light, chunk_j), kLightCols);
     const int64_t block_edges = (int64_t)kSub * chunk_j;
     const int64_t r0 = (int64_t)j * B;
     const int64_t r1 = std::min<int64_t>(r0 + B, n);
-    int32_t pcount = 0, scount = 0, nblk = 0;
-    int64_t eff = 0;
-    waves.clear();
-    lights.clear();
+    std::vector<std::pair<int32_t, int32_t>> waves;  // (degree, row)
+    std::vector<std::pair<int32_t, int32_t>> lights; // (degree, row)
+    o.items.reserve((size_t)kItemInts * (size_t)(r1 - r0 + 16));
     for (int64_t r = r0; r < r1; ++r) {
       const int64_t e0 = row_ptr[r];
       const int64_t deg = row_ptr[r + 1] - e0;
       if (col) {
-        for (int64_t e = e0; e < e0 + deg; ++e) eff += (col[e] < c_limit) ? 1 : 0;
+        for (int64_t e = e0; e < e0 + deg; ++e) o.eff += (col[e] < c_limit) ? 1 : 0;
       }
       if (deg <= light_j) {
         lights.push_back({(int32_t)deg, (int32_t)r});
@@ -68,21 +69,21 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
         continue;
       }
       if (deg <= block_edges) {
-        p.items.insert(p.items.end(), {(int32_t)r, (int32_t)e0, (int32_t)deg, -1, 0, 0, 0, 0});
-        ++nblk;
+        o.items.insert(o.items.end(), {(int32_t)r, (int32_t)e0, (int32_t)deg, -1, 0, 0, 0, 0});
+        ++o.nblk;
         continue;
       }
       const int64_t per = std::max<int64_t>(block_edges, (deg + max_chunks_j - 1) / max_chunks_j);
       const int64_t k = (deg + per - 1) / per;
       for (int64_t i = 0; i < k; ++i) {
         const int64_t lo = deg * i / k, hi = deg * (i + 1) / k;
-        p.items.insert(p.items.end(), {(int32_t)r, (int32_t)(e0 + lo), (int32_t)(hi - lo),
-                                       (int32_t)(pcount + i), scount, pcount, (int32_t)k, 0});
+        o.items.insert(o.items.end(), {(int32_t)r, (int32_t)(e0 + lo), (int32_t)(hi - lo),
+                                       (int32_t)(o.pcount + i), o.scount, o.pcount, (int32_t)k, 0});
       }
-      nblk += (int32_t)k;
-      p.splits.insert(p.splits.end(), {(int32_t)r, pcount, (int32_t)k, 0});
-      pcount += (int32_t)k;
-      ++scount;
+      o.nblk += (int32_t)k;
+      o.splits.insert(o.splits.end(), {(int32_t)r, o.pcount, (int32_t)k, 0});
+      o.pcount += (int32_t)k;
+      ++o.scount;
     }
     // medium and light rows by decreasing degree (row id breaks ties): the
     // grid-stride walk takes the longest units first, and the rows of a
@@ -92,26 +93,50 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
     };
     std::sort(waves.begin(), waves.end(), longer);
     for (const auto &w : waves)
-      p.items.insert(p.items.end(),
+      o.items.insert(o.items.end(),
                      {w.second, (int32_t)row_ptr[w.second], w.first, -1, 0, 0, 0, 0});
     std::sort(lights.begin(), lights.end(), longer);
     for (size_t i = 0; i < lights.size(); ++i) {
       const int32_t r = lights[i].second;
       const int32_t pack_max = lights[(i / pack) * pack].first;
-      const int32_t l = (int32_t)(p.light_off[j] + (int64_t)i);
-      p.items.insert(p.items.end(),
-                     {r, (int32_t)row_ptr[r], lights[i].first, pack_max, 0, 0, 0, 1 + l});
+      // word 7: 1 + light index within the batch (the job-wide index below)
+      o.items.insert(o.items.end(),
+                     {r, (int32_t)row_ptr[r], lights[i].first, pack_max, 0, 0, 0, 1 + (int32_t)i});
     }
+    o.nlight = (int32_t)lights.size();
+  }
+  p.item_off.assign(p.batches + 1, 0);
+  p.block_items.assign(p.batches, 0);
+  p.light_rows.assign(p.batches, 0);
+  p.light_off.assign(p.batches + 1, 0);
+  p.split_off.assign(p.batches + 1, 0);
+  p.pslots.assign(p.batches, 0);
+  p.edges_eff.assign(p.batches, 0);
+  size_t n_items = 0, n_splits = 0;
+  for (const BatchPlan &o : bp) {
+    n_items += o.items.size();
+    n_splits += o.splits.size();
+  }
+  p.items.reserve(n_items);
+  p.splits.reserve(n_splits);
+  for (int j = 0; j < p.batches; ++j) {
+    BatchPlan &o = bp[(size_t)j];
+    const size_t first_light = o.items.size() - (size_t)kItemInts * (size_t)o.nlight;
+    for (size_t w = first_light + 7; w < o.items.size(); w += kItemInts)
+      o.items[w] += (int32_t)p.light_off[j];
+    p.items.insert(p.items.end(), o.items.begin(), o.items.end());
+    p.splits.insert(p.splits.end(), o.splits.begin(), o.splits.end());
     p.item_off[j + 1] = (int64_t)p.items.size() / kItemInts;
-    p.block_items[j] = nblk;
-    p.light_rows[j] = (int32_t)lights.size();
-    p.light_off[j + 1] = p.light_off[j] + (int64_t)lights.size();
+    p.block_items[j] = o.nblk;
+    p.light_rows[j] = o.nlight;
+    p.light_off[j + 1] = p.light_off[j] + (int64_t)o.nlight;
     p.split_off[j + 1] = (int64_t)p.splits.size() / 4;
-    p.pslots[j] = pcount;
-    p.max_pslots = std::max(p.max_pslots, (int)pcount);
-    p.max_splits = std::max(p.max_splits, (int)scount);
-    p.edges_eff[j] = eff;
-    p.edges_eff_total += eff;
+    p.pslots[j] = o.pcount;
+    p.max_pslots = std::max(p.max_pslots, (int)o.pcount);
+    p.max_splits = std::max(p.max_splits, (int)o.scount);
+    p.edges_eff[j] = o.eff;
+    p.edges_eff_total += o.eff;
+    std::vector<int32_t>().swap(o.items);
   }
   *out = std::move(p);
   return FE_OK;

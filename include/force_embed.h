@@ -237,14 +237,18 @@ fe_status fe_write_embedding(const char *path, const float *x, int64_t n,
 /* ---- multi-job engine: batching and serving forms ------------------------
  * No counterpart in the reference, which embeds one graph per process run
  * (cpp/dist_embed.cpp:330-347); each job below is one such run.
- * An engine owns one device, two streams (rounds; uploads and downloads) and
+ * An engine owns one device, three streams (rounds, uploads, downloads) and
  * an HBM budget held as one arena. Jobs (independent graphs with their own
  * CSR, X, d, gather dtype, batch size, epochs and negative lists) are
  * admitted while their device state fits the free budget. Every engine round
  * advances each running job by one minibatch, all jobs' work items sharing
- * one force launch and one apply launch. A job's chunking is fixed when it is
- * submitted and no job touches another job's memory, so its result does not
- * depend on which other jobs share its rounds.                              */
+ * one fused force launch. A job's chunking is fixed when it is submitted and
+ * no job touches another job's memory, so its result does not depend on
+ * which other jobs share its rounds.
+ * Threads: fe_engine_submit and fe_engine_collect may run on other threads
+ * while one thread calls fe_engine_run; they build, upload and download
+ * without blocking the rounds. The other calls are serialised internally.
+ * fe_engine_last_error reports the calling thread's last failure.          */
 typedef struct fe_engine fe_engine;
 typedef struct fe_job_desc {
   int64_t n;

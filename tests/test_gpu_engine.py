@@ -5,6 +5,10 @@ with other jobs in shared rounds, through the captured-graph schedule, or
 as a single-graph context (fe_train): the work items and summation order of
 a job are fixed at submission and no job touches another's memory.
 """
+import queue
+import threading
+import time
+
 import numpy as np
 import pytest
 
@@ -112,3 +116,56 @@ def test_jobs_admitted_while_others_run():
     for s, x in ((specs[0], x0), (specs[1], x1)):
         with api.Engine(2 * GiB) as e:
             np.testing.assert_array_equal(_run_all(e, [_submit(e, s)])[0], x)
+
+
+def test_submit_and_collect_on_other_threads():
+    """fe_engine_submit and fe_engine_collect from their own threads while
+    the main thread runs rounds: every result equals the job run alone."""
+    specs = _jobs()
+    alone = []
+    for s in specs:
+        with api.Engine(2 * GiB) as e:
+            alone.append(_run_all(e, [_submit(e, s)])[0])
+    results, errors = {}, []
+    with api.Engine(2 * GiB) as e:
+        ids, lock, todo = {}, threading.Lock(), queue.Queue()
+
+        def submitter():
+            try:
+                for k, s in enumerate(specs):
+                    jid = _submit(e, s)
+                    with lock:
+                        ids[jid] = k
+            except Exception as ex:
+                errors.append(ex)
+
+        def collector():
+            while True:
+                item = todo.get()
+                if item is None:
+                    return
+                try:
+                    results[item[1]] = e.collect(item[0])
+                except Exception as ex:
+                    errors.append(ex)
+
+        ts, tc = threading.Thread(target=submitter), threading.Thread(target=collector)
+        ts.start()
+        tc.start()
+        handed = set()
+        deadline = time.time() + 120
+        while len(handed) < len(specs) and not errors:
+            assert time.time() < deadline, "jobs did not finish"
+            e.run(4)
+            with lock:
+                cur = dict(ids)
+            for jid, k in cur.items():
+                if jid not in handed and e.done(jid):
+                    handed.add(jid)
+                    todo.put((jid, k))
+        ts.join()
+        todo.put(None)
+        tc.join()
+    assert not errors, errors
+    for k, a in enumerate(alone):
+        np.testing.assert_array_equal(results[k], a)
