@@ -89,6 +89,7 @@ SIGNATURES = {
     "fe_set_timing": (C.c_int, [_P, C.c_int]),
     "fe_kernel_times": (C.c_int, [_P, _DP, _DP, _DP, _I64P]),
     "fe_round_bf16": (C.c_int, [C.c_int, _FP, C.POINTER(C.c_uint16), C.c_int64]),
+    "fe_spmm": (C.c_int, [_P, C.c_int, C.c_int, C.c_float, _FP]),
     "fe_autotune": (C.c_int, [_P, C.c_int, C.c_int, C.POINTER(fe_tuning), _DP, C.c_int,
                               C.POINTER(C.c_int), C.POINTER(fe_tuning), _DP]),
     "fe_batch_tuning": (C.c_int, [_P, C.c_int, C.POINTER(fe_tuning)]),

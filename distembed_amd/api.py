@@ -214,6 +214,12 @@ class ForceEmbed:
         self._check(self._lib.fe_train(self._ctx, iterations, batch_size, ns, lr,
                                        1 if use_graph else 0))
 
+    def spmm(self, iterations: int, batch_size: int, lr: float) -> np.ndarray:
+        """SpMMAlgo::algo_spmm (fe_spmm): Y = lr * A' X, X unchanged."""
+        y = np.empty((self.n, self.d), dtype=np.float32)
+        self._check(self._lib.fe_spmm(self._ctx, iterations, batch_size, lr, _ptr(y, C.c_float)))
+        return y
+
     def synchronize(self) -> None:
         self._check(self._lib.fe_synchronize(self._ctx))
 

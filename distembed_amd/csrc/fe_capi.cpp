@@ -244,6 +244,14 @@ extern "C" fe_status fe_step(fe_ctx *c, int epoch, int batch) {
   return s;
 }
 
+extern "C" fe_status fe_spmm(fe_ctx *c, int iterations, int batch_size, float lr, float *y_out) {
+  CTX_GUARD(c);
+  Job &j = c->job;
+  fe_status s = fe_configure(c, batch_size, lr, j.bound);
+  if (s != FE_OK) return s;
+  return from_job(c, job_spmm(j, iterations, lr, y_out, c->stream));
+}
+
 extern "C" fe_status fe_step_delta(fe_ctx *c, int epoch, int batch, float *delta_out) {
   CTX_GUARD(c);
   if (!delta_out) return fail(c, FE_ERR_INVALID_ARG, "fe_step_delta: NULL output");

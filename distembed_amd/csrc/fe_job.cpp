@@ -214,13 +214,14 @@ fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks) {
 
 void job_free(Job &j) {
   void *bufs[] = {j.X, j.Xs, j.col, j.items, j.S, j.P, j.counters, j.negs, j.epoch, j.desc,
-                  j.prefix, j.negcur, j.edesc};
+                  j.prefix, j.negcur, j.edesc, j.Y};
   for (void *p : bufs) dev_free(j, p);
   j.X = nullptr;
   j.Xs = nullptr;
   j.col = j.items = j.negs = j.negcur = nullptr;
   j.edesc = nullptr;
-  j.S = j.P = nullptr;
+  j.S = j.P = j.Y = nullptr;
+  j.cap_Y = 0;
   j.counters = j.epoch = j.prefix = nullptr;
   j.desc = nullptr;
   j.cap_items = j.cap_S = j.cap_P = j.cap_ctr = j.cap_negs = j.cap_desc = 0;
@@ -703,6 +704,34 @@ fe_status job_launch_force(Job &j, int b, DescKind kind, hipStream_t st, hipEven
                          j.dtype == FE_GATHER_BF16 ? 1 : 0, j.G, st),
          "force kernel launch");
   if (ev) FE_HIP(j, record_event(ev[1], st), "event");
+  return FE_OK;
+}
+
+fe_status job_spmm(Job &j, int iterations, float lr, float *y_out, hipStream_t st) {
+  if (iterations < 0 || !y_out) {
+    j.err = "fe_spmm: negative iterations or NULL output";
+    return FE_ERR_INVALID_ARG;
+  }
+  if (lr != j.lr) {
+    j.lr = lr;
+    j.steps_dirty = true;
+  }
+  fe_status s = job_prepare(j, st);
+  if (s != FE_OK) return s;
+  if ((s = ensure_cap(j, j.Y, j.cap_Y, (size_t)j.n * j.d, "SpMM output")) != FE_OK) return s;
+  // state[1] = rot: the kernel reads X through the layout at rest
+  if ((s = job_set_epoch(j, 0, st)) != FE_OK) return s;
+  for (int it = 0; it < iterations; ++it) {
+    for (int b = 0; b < j.plan.batches; ++b) {
+      const Shape sh = job_shape(j, b);
+      FE_HIP(j, launch_spmm(job_desc(j, b, DESC_SINGLE), j.prefix + 4 * b, 1, j.force_blocks[b],
+                            sh.waves_per_block, sh.unroll, j.dtype == FE_GATHER_BF16 ? 1 : 0, j.G,
+                            j.Y, st),
+             "SpMM kernel launch");
+    }
+  }
+  if (iterations > 0)
+    FE_HIP(j, d2h_staged({{y_out, j.Y, sizeof(float) * (size_t)j.n * j.d}}, st), "fe_spmm: copy");
   return FE_OK;
 }
 

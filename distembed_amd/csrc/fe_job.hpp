@@ -18,6 +18,8 @@ namespace fe {
 hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
                         int wpb, int ipw, int unroll, size_t lds_bytes, int mode, int g_const,
                         hipStream_t st);
+hipError_t launch_spmm(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
+                       int wpb, int unroll, int mode, int g_const, float *Y, hipStream_t st);
 hipError_t launch_apply(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
                         hipStream_t st);
 // set != 0: every job's epoch counter := value; else += 1. Refreshes negcur.
@@ -100,6 +102,8 @@ struct Job {
   int32_t *col = nullptr;
   int32_t *items = nullptr;
   float *S = nullptr;        // B x d: new rows or deltas of unfused steps
+  float *Y = nullptr;        // n x d: SpMM output (fe_spmm)
+  size_t cap_Y = 0;
   float *P = nullptr;
   int *counters = nullptr;
   int32_t *negs = nullptr;
@@ -151,6 +155,10 @@ fe_status job_normalize_layout(Job &j, hipStream_t st);
 fe_status job_set_epoch(Job &j, int value, hipStream_t st);
 // Epoch counter += 1 (start of an epoch; may be captured in a graph).
 fe_status job_tick(Job &j, hipStream_t st);
+// SpMMAlgo::algo_spmm at world size 1: Y = lr * A' X (A' without the
+// columns the quirk skips) into y_out, `iterations` times; X is unchanged.
+fe_status job_spmm(Job &j, int iterations, float lr, float *y_out, hipStream_t st);
+
 // Zeroes the split counters (outside any graph capture).
 fe_status job_reset_counters(Job &j, hipStream_t st);
 const StepDesc *job_desc(const Job &j, int b, DescKind kind);

@@ -804,6 +804,204 @@ __global__ __launch_bounds__(512) void force_kernel(
   (void)ipw;
 }
 
+// ---- SpMM sibling: SpMMAlgo::algo_spmm at world size 1 ----------------------
+// Y[r] = sum over the CSR entries c of row r with c < c_limit of lr * X[c]
+// (cpp/algo/spmm.hpp:341-412: dst_id < dst_end_index skips column N-1 like
+// the force step; :415-431 assigns the batch's rows to the output). X is
+// read through the slot map of the layout at rest and never written, so the
+// batches are independent. The work decomposition is the force step's: block
+// items (kSub sub-ranges summed in order through LDS, heavy-row chunks
+// summed in index order by the last to finish), medium rows one per wave,
+// light rows one per lane group. Slots past a row's end or skipped by the
+// quirk add nothing (their loads are masked off).
+
+template <int U, int MODE>
+__device__ __forceinline__ void sum_round(const RowBuf<MODE> (&xb)[U], const bool (&ok)[U],
+                                          float lr, float (&acc)[8]) {
+#pragma unroll
+  for (int uu = 0; uu < U; ++uu) {
+    float xc[8];
+    unpack_buf<MODE>(xb[uu], xc);
+    const float w = ok[uu] ? lr : 0.0f;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) acc[k] = fmaf(w, xc[k], acc[k]);
+  }
+}
+
+// A masked-off slot holds zeros (no load), so a non-finite value elsewhere
+// in X cannot leak into the sum through the zero weight.
+template <int MODE>
+__device__ __forceinline__ void load_or_zero(const RowSrc &src, uint32_t pos, bool ok,
+                                             RowBuf<MODE> &b) {
+  if constexpr (MODE == 1) {
+    b.w = u4v{0u, 0u, 0u, 0u};
+  } else {
+#pragma unroll
+    for (int k = 0; k < 8; ++k) b.f[k] = 0.0f;
+  }
+  if (ok) load_at<MODE>(src, pos, b);
+}
+
+// Edges [e0, e0+cnt) of one row: group g takes edges g, g+NG, ... in order.
+template <int U, int MODE>
+__device__ __forceinline__ void sum_edges(const StepDesc &sd, const RowSrc &src, int e0, int cnt,
+                                          int G, int NG, int g, int lane, float (&acc)[8]) {
+  const int c_limit = sd.c_limit;
+  const float lr = sd.lr;
+  for (int eb = 0; eb < cnt; eb += 64) {
+    const int m = min(64, cnt - eb);
+    int cl = lane < m ? gld_int_nt(sd.col + e0 + eb + lane) : c_limit;
+    const bool lv = cl < c_limit;
+    const int pl = lv ? (int)row_pos(src.m, (uint32_t)cl) : -1;
+    const int rounds = (m + NG - 1) / NG;
+    for (int u0 = 0; u0 < rounds; u0 += U) {
+      RowBuf<MODE> xb[U];
+      bool ok[U];
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) {
+        const int k = (u0 + uu) * NG + g;
+        const int pos = __builtin_amdgcn_ds_bpermute(k << 2, pl);
+        ok[uu] = k < m && pos >= 0;
+        load_or_zero<MODE>(src, (uint32_t)pos, ok[uu], xb[uu]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      sum_round<U, MODE>(xb, ok, lr, acc);
+    }
+  }
+}
+
+// One light pack: group g sums its row's deg <= kLightCols columns (the
+// first G of them came with the item, pc), then writes Y.
+template <int U, int MODE>
+__device__ __forceinline__ void sum_light_pack(const StepDesc &sd, const RowSrc &src, float *Y,
+                                               u4v ia, int lidx, int pc, bool valid, int G,
+                                               int g, int q) {
+  const int r = (int)ia.x, deg = valid ? (int)ia.z : 0;
+  const int kmax = __builtin_amdgcn_readfirstlane((int)ia.w);
+  const int32_t *lc = sd.lcols + (size_t)lidx * kLightCols;
+  const int c_limit = sd.c_limit;
+  float acc[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+  for (int kb = 0; kb < kmax; kb += G) {
+    const int cl = (kb + q < deg) ? (kb == 0 ? pc : gld_int_nt(lc + kb + q)) : c_limit;
+    const int pl = cl < c_limit ? (int)row_pos(src.m, (uint32_t)cl) : -1;
+    const int kend = min(kmax, kb + G);
+    for (int k0 = kb; k0 < kend; k0 += U) {
+      RowBuf<MODE> xb[U];
+      bool ok[U];
+#pragma unroll
+      for (int uu = 0; uu < U; ++uu) {
+        const int k = k0 + uu;
+        const int pos = __builtin_amdgcn_ds_bpermute((g * G + (k - kb)) << 2, pl);
+        ok[uu] = k < kend && (k - kb) < G && pos >= 0;
+        load_or_zero<MODE>(src, (uint32_t)pos, ok[uu], xb[uu]);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+      sum_round<U, MODE>(xb, ok, sd.lr, acc);
+    }
+  }
+  if (valid) store8_f32_nt(Y + (size_t)r * sd.D + 8 * q, acc);
+}
+
+template <int U, int MODE, int GT>
+__global__ __launch_bounds__(512) void spmm_kernel(const StepDesc *__restrict__ steps,
+                                                   const int *__restrict__ blk_prefix,
+                                                   int n_steps, float *__restrict__ Y) {
+  extern __shared__ float lds[];
+  const int b = blockIdx.x;
+  const int si = find_step(blk_prefix, n_steps, b);
+  const StepDesc &sd = steps[si];
+  const int lb = b - blk_prefix[si];
+  const int nb = blk_prefix[si + 1] - blk_prefix[si];
+  const int D = sd.D;
+  const int G = GT > 0 ? GT : sd.G;
+  const int NG = 64 / G;
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int wpb = blockDim.x >> 6;
+  const int g = lane / G;
+  const int q = lane & (G - 1);
+  const int nbi = sd.n_block_items;
+  const int n_light = sd.n_light;
+  const int n_wave_items = sd.n_items - nbi - n_light;
+  const int32_t *witems = sd.items + (size_t)kItemInts * nbi;
+  const int32_t *lrows = witems + (size_t)kItemInts * n_wave_items;
+  const int n_packs = (n_light + NG - 1) / NG;
+  // X at rest: batch k in slot (k - rot) mod P
+  const SlotMap m = make_map(sd.lay, sd.state[1], 0);
+  const RowSrc src = make_src<MODE>(sd, m, q, G);
+
+  if (lb < nbi) {
+    const u4v it0 = gld_u4(sd.items + (size_t)kItemInts * lb);
+    const u4v it1 = gld_u4(sd.items + (size_t)kItemInts * lb + 4);
+    const int r = (int)it0.x, e0 = (int)it0.y, cnt = (int)it0.z, pslot = (int)it0.w;
+    float acc[8];
+    for (int s = wave; s < kSub; s += wpb) {
+      const int lo = (int)((int64_t)cnt * s / kSub), hi = (int)((int64_t)cnt * (s + 1) / kSub);
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+      sum_edges<U, MODE>(sd, src, e0 + lo, hi - lo, G, NG, g, lane, acc);
+      combine_groups(acc, G);
+      if (g == 0) store8_lds(lds + (size_t)s * D + 8 * q, acc);
+    }
+    __syncthreads();
+    if (wave == 0) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+      if (g == 0) {
+        for (int s = 0; s < kSub; ++s) {
+          const float *p = lds + (size_t)s * D + 8 * q;
+#pragma unroll
+          for (int k = 0; k < 8; ++k) acc[k] += p[k];
+        }
+      }
+      bool finish = pslot < 0;
+      if (!finish) {
+        const int sid = (int)it1.x, p0 = (int)it1.y, kparts = (int)it1.z;
+        if (g == 0) store8_coherent(sd.P + (size_t)pslot * D + 8 * q, acc);
+        wait_stores();
+        int old = 0;
+        if (lane == 0)
+          old = __hip_atomic_fetch_add((gint *)(sd.counters + sid), 1, __ATOMIC_RELAXED,
+                                       __HIP_MEMORY_SCOPE_AGENT);
+        old = __shfl(old, 0);
+        if (old == kparts - 1) {
+#pragma unroll
+          for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+          sum_partials<U>(sd, p0, kparts, NG, g, q, acc);
+          combine_groups(acc, G);
+          if (lane == 0)
+            __hip_atomic_store((gint *)(sd.counters + sid), 0, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+          finish = true;
+        }
+      }
+      if (finish && g == 0) store8_f32_nt(Y + (size_t)r * D + 8 * q, acc);
+    }
+  } else {
+    const int W = (nb - nbi) * wpb;
+    for (int u = (lb - nbi) * wpb + wave; u < n_wave_items + n_packs; u += W) {
+      if (u < n_wave_items) {
+        const u4v it0 = gld_u4(witems + (size_t)kItemInts * u);
+        float acc[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+        sum_edges<U, MODE>(sd, src, (int)it0.y, (int)it0.z, G, NG, g, lane, acc);
+        combine_groups(acc, G);
+        if (g == 0) store8_f32_nt(Y + (size_t)it0.x * D + 8 * q, acc);
+      } else {
+        const int pk = u - n_wave_items;
+        const int idx = pk * NG + g;
+        const int li = idx < n_light ? idx : pk * NG;
+        const u4v it0 = gld_u4(lrows + (size_t)kItemInts * li);
+        const int pc = q < kLightCols ? gld_int_nt(sd.lcols + (size_t)li * kLightCols + q) : 0;
+        sum_light_pack<U, MODE>(sd, src, Y, it0, li, pc, idx < n_light, G, g, q);
+      }
+    }
+  }
+}
+
 // X[batch] = S (cpp/algo/algo.hpp:651-666 with the add already folded into
 // S) and the BF16 shadow refresh of the same rows, at the batch's slot in
 // the layout at rest.
@@ -1057,6 +1255,28 @@ hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_step
 hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, int rot,
                         hipStream_t st) {
   hipLaunchKernelGGL(epoch_kernel, dim3(n), dim3(256), 0, st, jobs, set, value, rot);
+  return hipGetLastError();
+}
+
+template <int U, int MODE>
+static void launch_spmm_m(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int g_const,
+                          const StepDesc *steps, const int *blk_prefix, int n_steps, float *Y) {
+  if (g_const == 16) hipLaunchKernelGGL((spmm_kernel<U, MODE, 16>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, Y);
+  else hipLaunchKernelGGL((spmm_kernel<U, MODE, 0>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, Y);
+}
+
+hipError_t launch_spmm(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
+                       int wpb, int unroll, int mode, int g_const, float *Y, hipStream_t st) {
+  dim3 grid(n_blocks), block(64 * wpb);
+  const size_t lds = (size_t)kSub * 512 * sizeof(float); // kSub sub-range sums of d <= 512
+  const bool bf = mode == 1;
+  if (unroll <= 2) {
+    if (bf) launch_spmm_m<2, 1>(grid, block, lds, st, g_const, steps, blk_prefix, n_steps, Y);
+    else launch_spmm_m<2, 0>(grid, block, lds, st, g_const, steps, blk_prefix, n_steps, Y);
+  } else {
+    if (bf) launch_spmm_m<4, 1>(grid, block, lds, st, g_const, steps, blk_prefix, n_steps, Y);
+    else launch_spmm_m<4, 0>(grid, block, lds, st, g_const, steps, blk_prefix, n_steps, Y);
+  }
   return hipGetLastError();
 }
 

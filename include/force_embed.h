@@ -135,6 +135,16 @@ fe_status fe_train(fe_ctx *ctx, int iterations, int batch_size, int ns,
                    float lr, int use_graph);
 fe_status fe_synchronize(fe_ctx *ctx);
 
+/* SpMMAlgo::algo_spmm(iterations, batch_size, lr) at world size 1
+ * (cpp/algo/spmm.hpp:52-165; row kernel :341-412, assignment :415-431):
+ * Y[r] = sum of lr * X[c] over the CSR entries c of row r, in CSR order,
+ * skipping columns c >= N-1 under FE_QUIRK_SKIP_LAST_COL as the reference's
+ * dst_id < dst_end_index does. y_out (n x d, row-major) receives Y; X is not
+ * changed. Batches are independent, so Y does not depend on batch_size,
+ * which only sets the launches; `iterations` repeats the computation as the
+ * reference's loop does. Uses the gather dtype of the context.             */
+fe_status fe_spmm(fe_ctx *ctx, int iterations, int batch_size, float lr, float *y_out);
+
 /* Force-model loss at the current X, reduced in FP64 on the device:
  *   attr = sum over CSR entries (r,c) taken by the attractive kernel of
  *          log(1 + |x_r - x_c|^2)

@@ -12,6 +12,7 @@
 //   cpp/algo/algo.hpp:651-666    update X[batch] += prev
 //   cpp/algo/algo.hpp:65-72      scale() clamp
 //   cpp/core/common.cpp:37-53    generate_random_numbers (orc_std_sampler)
+//   cpp/algo/spmm.hpp:341-431    SpMMAlgo row kernel and assignment (orc_spmm_*)
 // The arithmetic keeps the reference's expressions: the squared distance is a
 // serial sum in VALUE_TYPE, d1 is computed from double literals and converted
 // to VALUE_TYPE, and each component accumulates sequentially (edges in CSR
@@ -23,6 +24,11 @@
 // every read of X inside a batch sees X rounded to bfloat16 (round to nearest
 // even), the forces are computed from those values in float exactly as above,
 // and the batch's update is added to the float master X.
+//
+// SpMM (orc_spmm_*): out[r][k] = sum over e in row r (CSR order) with
+// col[e] < c_limit of lr * X[col[e]][k], accumulated serially in VALUE_TYPE
+// from zero (cpp/algo/spmm.hpp:358-363, 395-405), then assigned (:415-431).
+// Pinned against scipy.sparse's CSR product and a closed form.
 //
 // Pinning (tests/test_oracle_*.py): the sampler against std:: compiled with
 // g++; the per-batch update against torch autograd of the force-model loss in
@@ -238,6 +244,25 @@ uint64_t mix64(uint64_t z) {
 
 } // namespace
 
+// SpMMAlgo::algo_spmm at world size 1: rows are independent, so batching
+// does not enter. quirks bit 2 (FE_QUIRK_SKIP_LAST_COL) skips c >= n-1.
+template <typename T>
+static void spmm_impl(int64_t n, int d, const int64_t *rp, const int32_t *col, const T *X, T lr,
+                      unsigned quirks, T *out, int threads) {
+  const int64_t c_limit = (quirks & 2u) ? n - 1 : n;
+  if (threads > 0) omp_set_num_threads(threads);
+#pragma omp parallel for schedule(static)
+  for (int64_t r = 0; r < n; ++r) {
+    T *o = out + r * d;
+    for (int k = 0; k < d; ++k) o[k] = T(0);
+    for (int64_t e = rp[r]; e < rp[r + 1]; ++e) {
+      const int64_t c = col[e];
+      if (c >= c_limit) continue;
+      for (int k = 0; k < d; ++k) o[k] += lr * X[c * d + k];
+    }
+  }
+}
+
 extern "C" {
 
 // The reference's sampler call, verbatim in meaning: std::minstd_rand seeded
@@ -328,6 +353,16 @@ int orc_loss_f64(int64_t n, int d, const int64_t *rp, const int32_t *col, const 
   *attr = a;
   *rep = r;
   return 0;
+}
+
+void orc_spmm_f32(int64_t n, int d, const int64_t *rp, const int32_t *col, const float *X,
+                  float lr, unsigned quirks, float *out, int threads) {
+  spmm_impl<float>(n, d, rp, col, X, lr, quirks, out, threads);
+}
+
+void orc_spmm_f64(int64_t n, int d, const int64_t *rp, const int32_t *col, const double *X,
+                  double lr, unsigned quirks, double *out, int threads) {
+  spmm_impl<double>(n, d, rp, col, X, lr, quirks, out, threads);
 }
 
 } // extern "C"

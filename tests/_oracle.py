@@ -42,6 +42,11 @@ def load():
     lib.orc_train_f64.argtypes = [C.c_int64, C.c_int, I64P, I32P, DP, C.c_int, C.c_int, I32P,
                                   C.c_int, C.c_int, C.c_double, C.c_double, C.c_uint, C.c_int]
     lib.orc_train_bf16.argtypes = lib.orc_train_f32.argtypes
+    lib.orc_spmm_f32.argtypes = [C.c_int64, C.c_int, I64P, I32P, FP, C.c_float, C.c_uint, FP, C.c_int]
+    lib.orc_spmm_f32.restype = None
+    lib.orc_spmm_f64.argtypes = [C.c_int64, C.c_int, I64P, I32P, DP, C.c_double, C.c_uint, DP,
+                                 C.c_int]
+    lib.orc_spmm_f64.restype = None
     lib.orc_bf16_round.argtypes = [FP, FP, C.c_int64]
     lib.orc_bf16_round.restype = None
     lib.orc_loss_f64.argtypes = [C.c_int64, C.c_int, I64P, I32P, DP, C.c_uint, C.c_uint64, C.c_int,
@@ -131,6 +136,22 @@ def train(g, X, B, iterations, lists, lr=0.02, bound=5.0, quirks=3, threads=0, b
                                threads)
     if st != 0:
         raise ValueError(f"oracle train failed ({st})")
+
+
+def spmm(g, X, lr, quirks=3, threads=0):
+    """SpMMAlgo::algo_spmm at world size 1: lr * A' X in X's dtype (float32
+    or float64), accumulated serially in CSR order."""
+    lib = load()
+    rp = np.ascontiguousarray(g.row_ptr, dtype=np.int64)
+    ci = np.ascontiguousarray(g.col_idx, dtype=np.int32)
+    out = np.empty_like(X)
+    if X.dtype == np.float32:
+        lib.orc_spmm_f32(g.n, X.shape[1], _p(rp, C.c_int64), _p(ci, C.c_int32), _p(X, C.c_float),
+                         lr, quirks, _p(out, C.c_float), threads)
+    else:
+        lib.orc_spmm_f64(g.n, X.shape[1], _p(rp, C.c_int64), _p(ci, C.c_int32),
+                         _p(X, C.c_double), lr, quirks, _p(out, C.c_double), threads)
+    return out
 
 
 def loss(g, X, quirks=3, seed=0, eval_ns=5, threads=0):

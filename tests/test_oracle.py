@@ -316,3 +316,50 @@ def test_loss_closed_form():
 def test_eval_negative_same_in_library_and_oracle():
     for (seed, r, k, n) in [(0, 0, 0, 10), (5, 123456, 3, 1_048_576), (2**63, 7, 9, 4_194_304)]:
         assert api.eval_negative(seed, r, k, n) == orc.eval_negative(seed, r, k, n)
+
+
+# ---- SpMM sibling (SpMMAlgo::algo_spmm, cpp/algo/spmm.hpp:341-431) ----------
+
+def _masked_csr(g, quirks):
+    import scipy.sparse as sp
+    rp = np.asarray(g.row_ptr, dtype=np.int64)
+    ci = np.asarray(g.col_idx, dtype=np.int64)
+    data = np.ones(ci.size)
+    if quirks & 2:
+        data[ci >= g.n - 1] = 0.0  # dst_id < dst_end_index skips column n-1
+    return sp.csr_matrix((data, ci, rp), shape=(g.n, g.n))
+
+
+@pytest.mark.parametrize("quirks", [3, 0])
+def test_spmm_matches_scipy_csr_product(quirks):
+    """lr * A' X from scipy.sparse's CSR product (a library routine) against
+    the oracle's serial restatement, in FP64."""
+    g = api.Graph.sbm(700, 5, 0.04, 0.004, seed=3)
+    X = np.random.default_rng(1).uniform(-1, 1, (g.n, 16))
+    lr = 0.37
+    want = lr * (_masked_csr(g, quirks) @ X)
+    got = orc.spmm(g, X, lr, quirks=quirks)
+    np.testing.assert_allclose(got, want, rtol=1e-12, atol=1e-14)
+    # the quirk matters on this graph: vertex n-1 has neighbours
+    assert (np.asarray(g.col_idx) == g.n - 1).any()
+
+
+def test_spmm_closed_form_path_graph():
+    """Path 0-1-2: Y0 = lr X1, Y1 = lr (X0 + X2) (lr X0 when column 2 = n-1 is
+    skipped), Y2 = lr X1."""
+    g = _graph_from_pairs(3, [(0, 1), (1, 2)])
+    X = np.array([[1.0, 2.0], [3.0, -4.0], [0.5, 0.25]])
+    lr = 0.5
+    y0 = orc.spmm(g, X, lr, quirks=0)
+    np.testing.assert_array_equal(y0, [[1.5, -2.0], [0.75, 1.125], [1.5, -2.0]])
+    y3 = orc.spmm(g, X, lr, quirks=3)
+    np.testing.assert_array_equal(y3, [[1.5, -2.0], [0.5, 1.0], [1.5, -2.0]])
+
+
+def test_spmm_fp32_tracks_fp64():
+    g = api.Graph.rmat(10, 8, seed=2)
+    X = np.random.default_rng(4).uniform(-0.5, 0.5, (g.n, 32))
+    y64 = orc.spmm(g, X, 0.02)
+    y32 = orc.spmm(g, X.astype(np.float32), 0.02)
+    assert y32.dtype == np.float32
+    np.testing.assert_allclose(y32, y64, rtol=1e-4, atol=1e-6)
