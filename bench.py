@@ -22,7 +22,8 @@ cpu_baseline: the oracle's FP64 restatement of the reference loop (OpenMP,
 schedule(static)) on a bounded sample of the same workload.
 
 --form precision | batching | serving runs the other workloads instead
-(distembed_amd/bench_forms.py) and prints their results as one JSON line.
+(distembed_amd/bench_forms.py) and prints their results as one JSON line;
+--form spmm times the SpMM sibling (fe_spmm) on the same R-MAT graph.
 """
 from __future__ import annotations
 
@@ -75,6 +76,60 @@ def cpu_baseline(g, lists, d, B, ns, lr, budget_s: float, max_epochs: int):
             "cores": threads, "kind": "port",
             "sample": f"{done} epoch(s) of the same R-MAT-20 workload, FP64 (the reference's "
                       f"VALUE_TYPE), {threads} OpenMP threads, {t_total:.1f} s"}
+
+
+def spmm_form(api, args) -> dict:
+    """The SpMM sibling (fe_spmm, SpMMAlgo::algo_spmm): Y = lr * A' X on the
+    R-MAT graph at d = args.d, FP32 and BF16 gathers, B = args.batch. One
+    repetition is one full product (every batch); its time is the difference
+    between (steps + 1) repetitions and one, which removes the final
+    download. Algorithmic bytes: processed entries x (row + column id),
+    row pointers, and the output rows written. The CPU baseline is the
+    oracle's FP64 product on the host cores (one repetition)."""
+    t0 = time.perf_counter()
+    g = api.Graph.rmat(args.scale, 16, 0.57, 0.19, 0.19, seed=0)
+    log(f"spmm: graph n={g.n} nnz={g.nnz} ({time.perf_counter() - t0:.1f} s)")
+    d, B, lr, k = args.d, args.batch, args.lr, max(args.steps, 1)
+    X = api.init_uniform(g.n, d, -0.5, 0.5, seed=0)
+    edges = int((g.col_idx < g.n - 1).sum())
+    res = {"metric": "SpMM edges/s (lr * A' X, d=%d)" % d, "form": "spmm", "unit": "edges/s",
+           "higher_is_better": True, "n_gpus": 1, "data": "synthetic", "steps": k,
+           "config": {"workload": f"rmat{args.scale}_d{d}_spmm", "n": g.n, "nnz": g.nnz,
+                      "edges_processed": edges, "d": d, "batch": B, "lr": lr}}
+    for name, gather, s in (("fp32", api.FP32, 4), ("bf16", api.BF16, 2)):
+        alg = edges * (d * s + 4) + (g.n + 1) * 8 + g.n * d * 4
+        with api.ForceEmbed(g.n, d, gather=gather) as ctx:
+            ctx.upload_csr(g)
+            ctx.upload_x(X)
+            ctx.configure(B, lr, 5.0)
+            ctx.spmm(2, B, lr)  # plan and warm-up
+            t1 = time.perf_counter()
+            ctx.spmm(1, B, lr)
+            t2 = time.perf_counter()
+            ctx.spmm(k + 1, B, lr)
+            t3 = time.perf_counter()
+        per = ((t3 - t2) - (t2 - t1)) / k
+        res[name] = {"value": edges / per, "ms_per_product": per * 1e3,
+                     "roofline": {"bound": "hbm", "achieved": alg / per / 1e9, "peak": HBM_PEAK_GBPS,
+                                  "unit": "GB/s", "frac": alg / per / 1e9 / HBM_PEAK_GBPS,
+                                  "algorithmic_bytes": alg}}
+        log(f"spmm {name}: {per * 1e3:.3f} ms per product, {edges / per / 1e9:.2f} G edges/s, "
+            f"{alg / per / 1e9:.0f} GB/s algorithmic")
+    res["value"] = res["fp32"]["value"]
+    res["bf16_speedup"] = res["fp32"]["ms_per_product"] / res["bf16"]["ms_per_product"]
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import _oracle as orc  # the checker, timed as the CPU baseline
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
+        X64 = X.astype(np.float64)
+        t1 = time.perf_counter()
+        orc.spmm(g, X64, lr, quirks=3, threads=threads)
+        dt = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": edges / dt, "unit": "edges/s", "cores": threads,
+                               "kind": "port",
+                               "sample": f"one product on the same graph, FP64, {threads} "
+                                         f"OpenMP threads, {dt:.1f} s"}
+    return res
 
 
 def run_form(api, g, X0, lists, gather, args, tuning=None):
@@ -149,7 +204,7 @@ def main() -> int:
                     help='json {"tuning": {fe_tuning fields}, "per_batch": [[unroll, '
                          'waves_per_block, items_per_wave, chunk, max_chunks, light] per '
                          'batch]} (as reported in config.tuning); skips the autotuner')
-    ap.add_argument("--form", choices=["plain", "precision", "batching", "serving"],
+    ap.add_argument("--form", choices=["plain", "precision", "batching", "serving", "spmm"],
                     default="plain")
     ap.add_argument("--epochs", type=int, default=50, help="epochs of the batching/serving jobs")
     ap.add_argument("--graphs", type=int, default=512, help="graphs of the batching form")
@@ -168,6 +223,9 @@ def main() -> int:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 3
 
+    if args.form == "spmm":
+        print(json.dumps(spmm_form(api, args)), flush=True)
+        return 0
     if args.form != "plain":
         from distembed_amd import bench_forms as bf
         if args.form == "precision":

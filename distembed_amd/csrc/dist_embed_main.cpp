@@ -6,8 +6,13 @@
 //   -input FILE.mtx  -output DIR  -batch B (16384)  -iter T (30)  -lr (0.02)
 //   -nsamples ns (5)  -alpha (0)  -beta (0.25)  -dataset NAME  -col_major 0|1
 //   -sync_comm 0|1  -fix_batch_training 0|1
-// The other algorithms behind -spmm/-spgemm/-sparse_embedding/-msbfs are not
-// part of this library; asking for one exits with status 2.
+// -spmm 1 runs the SpMM sibling as BaselineSpMM::execute does
+// (cpp/algo/baseline_spmm.hpp:51-88, batch = all rows, cpp/dist_embed.cpp:179-181):
+// each iteration builds a fresh U(-1,1) dense matrix (seed init_seed + i) and
+// computes lr * A' X into an output matrix with SpMMAlgo::algo_spmm(1, ...),
+// recording one perf_stats entry per iteration; no embedding file is written.
+// The algorithms behind -spgemm/-sparse_embedding/-msbfs are not part of this
+// library; asking for one exits with status 2.
 // Additional flags:
 //   -rmat SCALE [-edge_factor 16] [-seed S]     R-MAT graph (a=0.57, b=c=0.19)
 //   -sbm N [-blocks 10] [-p_in P] [-p_out P] [-seed S]   stochastic block model
@@ -51,6 +56,7 @@ struct Options {
   bool col_major = false;
   bool sync_comm = false;
   bool fix_batch_training = false;
+  bool spmm = false;
   bool other_algo = false;
   // additions
   int rmat_scale = 0;
@@ -84,7 +90,67 @@ bool make_dir(const std::string &p) {
   return mkdir(p.c_str(), 0755) == 0 || errno == EEXIST;
 }
 
+void append_record(const Options &o, int batch_size, size_t dim, int world_size,
+                   const std::string &stats_json) {
+  std::ostringstream js;
+  js << "{\n    \"alpha\": " << o.alpha << ",\n    \"algo\": \"Embedding\",\n"
+     << "    \"batch_size\": " << batch_size << ",\n    \"beta\": " << o.beta << ",\n"
+     << "    \"d\": " << dim << ",\n    \"data_set\": \"" << json_escape(o.data_set_name)
+     << "\",\n    \"gather\": \"" << (o.gather == FE_GATHER_BF16 ? "bf16" : "fp32") << "\",\n"
+     << "    \"p\": " << world_size << ",\n    \"perf_stats\": " << stats_json
+     << ",\n    \"tile_width_fraction\": 1\n}";
+  std::ofstream fout("perf_output", std::ios_base::app);
+  fout << js.str() << "," << std::endl;
+  std::cout << js.str() << std::endl;
+}
+
+std::string stats_object(const std::map<std::string, double> &m, const char *indent) {
+  std::ostringstream js;
+  js << "{";
+  bool first = true;
+  for (const auto &kv : m) {
+    js << (first ? "\n" : ",\n") << indent << "\"" << kv.first << "\": " << kv.second;
+    first = false;
+  }
+  js << "\n" << (indent + 4) << "}";
+  return js.str();
+}
+
+template <size_t DIM> int run_spmm(const Options &o, SpMat<double> *mat) {
+  using INDEX_TYPE = uint64_t;
+  using VALUE_TYPE = double;
+  auto grid = std::unique_ptr<Process3DGrid>(new Process3DGrid(1, 1, 1, 1));
+  grid->device = o.device;
+  const int localARows = (int)mat->gRows;
+  const int batch_size = localARows; // full batch (cpp/dist_embed.cpp:179-181)
+  mat->batch_size = batch_size;
+  mat->proc_row_width = localARows;
+  mat->proc_col_width = (int)mat->gCols;
+  mat->initialize_CSR_blocks(true);
+  std::ostringstream arr;
+  arr << "[";
+  for (int i = 0; i < o.iterations; ++i) {
+    auto dense_mat = std::make_shared<DenseMat<INDEX_TYPE, VALUE_TYPE, DIM>>(grid.get(), localARows);
+    dense_mat->init_seed = o.init_seed + (uint64_t)i;
+    dense_mat->reinitialize();
+    auto dense_out = std::make_shared<DenseMat<INDEX_TYPE, VALUE_TYPE, DIM>>(grid.get(), localARows);
+    SpMMAlgo<INDEX_TYPE, VALUE_TYPE, DIM> algo(mat, mat, mat, dense_mat.get(), dense_out.get(),
+                                               grid.get(), o.alpha, o.beta, o.col_major,
+                                               o.sync_comm);
+    algo.set_gather_dtype(o.gather);
+    algo.set_quirks(o.quirks);
+    std::cout << " rank 0 spmm algo started  " << std::endl;
+    algo.algo_spmm(1, batch_size, (VALUE_TYPE)o.lr);
+    arr << (i ? ",\n        " : "\n        ") << stats_object(algo.perf_stats(), "            ");
+  }
+  arr << "\n    ]";
+  std::cout << " rank 0 SpMM algo completed  " << std::endl;
+  append_record(o, batch_size, DIM, 1, arr.str());
+  return 0;
+}
+
 template <size_t DIM> int run(const Options &o, SpMat<double> *mat) {
+  if (o.spmm) return run_spmm<DIM>(o, mat);
   using INDEX_TYPE = uint64_t;
   using VALUE_TYPE = double;
   auto grid = std::unique_ptr<Process3DGrid>(new Process3DGrid(1, 1, 1, 1));
@@ -122,21 +188,8 @@ template <size_t DIM> int run(const Options &o, SpMat<double> *mat) {
   writer.parallel_write(o.output_file + "/embedding.txt", dense_mat->nCoordinates, localARows,
                         (int)DIM, grid.get(), mat);
 
-  std::ostringstream js;
-  js << "{\n    \"alpha\": " << o.alpha << ",\n    \"algo\": \"Embedding\",\n"
-     << "    \"batch_size\": " << batch_size << ",\n    \"beta\": " << o.beta << ",\n"
-     << "    \"d\": " << DIM << ",\n    \"data_set\": \"" << json_escape(o.data_set_name)
-     << "\",\n    \"gather\": \"" << (o.gather == FE_GATHER_BF16 ? "bf16" : "fp32") << "\",\n"
-     << "    \"p\": " << world_size << ",\n    \"perf_stats\": {";
-  bool first = true;
-  for (const auto &kv : embedding_algo->perf_stats()) {
-    js << (first ? "\n" : ",\n") << "        \"" << kv.first << "\": " << kv.second;
-    first = false;
-  }
-  js << "\n    },\n    \"tile_width_fraction\": 1\n}";
-  std::ofstream fout("perf_output", std::ios_base::app);
-  fout << js.str() << "," << std::endl;
-  std::cout << js.str() << std::endl;
+  append_record(o, batch_size, DIM, world_size,
+                stats_object(embedding_algo->perf_stats(), "        "));
   return 0;
 }
 
@@ -158,8 +211,9 @@ int main(int argc, char **argv) {
     else if (!strcmp(a, "-col_major")) o.col_major = atoi(v) != 0;
     else if (!strcmp(a, "-sync_comm")) o.sync_comm = atoi(v) != 0;
     else if (!strcmp(a, "-fix_batch_training")) o.fix_batch_training = atoi(v) == 1;
-    else if (!strcmp(a, "-spmm") || !strcmp(a, "-spgemm") || !strcmp(a, "-sparse_embedding") ||
-             !strcmp(a, "-msbfs") || !strcmp(a, "-save_results"))
+    else if (!strcmp(a, "-spmm")) o.spmm = atoi(v) == 1;
+    else if (!strcmp(a, "-spgemm") || !strcmp(a, "-sparse_embedding") || !strcmp(a, "-msbfs") ||
+             !strcmp(a, "-save_results"))
       o.other_algo = o.other_algo || atoi(v) == 1;
     else if (!strcmp(a, "-rmat")) o.rmat_scale = atoi(v);
     else if (!strcmp(a, "-edge_factor")) o.edge_factor = atoi(v);
@@ -175,7 +229,7 @@ int main(int argc, char **argv) {
     else if (!strcmp(a, "-quirks")) o.quirks = (unsigned)atoi(v);
   }
   if (o.other_algo) {
-    std::cerr << "dist_embed: only the force-model embedding is built (no -spmm, -spgemm, "
+    std::cerr << "dist_embed: only the force-model embedding and -spmm are built (no -spgemm, "
                  "-sparse_embedding, -msbfs or -save_results)" << std::endl;
     return 2;
   }

@@ -214,7 +214,7 @@ fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks) {
 
 void job_free(Job &j) {
   void *bufs[] = {j.X, j.Xs, j.col, j.items, j.S, j.P, j.counters, j.negs, j.epoch, j.desc,
-                  j.prefix, j.negcur, j.edesc, j.Y};
+                  j.prefix, j.negcur, j.edesc, j.Y, j.sitems, j.sdesc, j.sprefix};
   for (void *p : bufs) dev_free(j, p);
   j.X = nullptr;
   j.Xs = nullptr;
@@ -222,6 +222,11 @@ void job_free(Job &j) {
   j.edesc = nullptr;
   j.S = j.P = j.Y = nullptr;
   j.cap_Y = 0;
+  j.sitems = nullptr;
+  j.sdesc = nullptr;
+  j.sprefix = nullptr;
+  j.cap_sitems = j.cap_sdesc = j.cap_sprefix = 0;
+  j.s_gen = ~0ull;
   j.counters = j.epoch = j.prefix = nullptr;
   j.desc = nullptr;
   j.cap_items = j.cap_S = j.cap_P = j.cap_ctr = j.cap_negs = j.cap_desc = 0;
@@ -272,6 +277,7 @@ fe_status job_upload_csr(Job &j, const int64_t *row_ptr, const int32_t *col, int
   FE_HIP(j, hipStreamSynchronize(st), "fe_upload_csr: sync");
   j.row_ptr.assign(row_ptr, row_ptr + j.n + 1);
   j.skip_deg.swap(skip);
+  ++j.csr_gen;
   j.nnz = nnz;
   j.plan_dirty = true;
   j.steps_dirty = true;
@@ -719,17 +725,78 @@ fe_status job_spmm(Job &j, int iterations, float lr, float *y_out, hipStream_t s
   fe_status s = job_prepare(j, st);
   if (s != FE_OK) return s;
   if ((s = ensure_cap(j, j.Y, j.cap_Y, (size_t)j.n * j.d, "SpMM output")) != FE_OK) return s;
+  // the whole-graph plan, as one batch of n rows
+  const int light = std::min(j.tun.light, kLightCols);
+  if (j.s_gen != j.csr_gen || j.s_chunk != j.tun.chunk || j.s_max_chunks != j.tun.max_chunks ||
+      j.s_light_cut != light || !j.sitems) {
+    Plan sp;
+    const int B = (int)std::min<int64_t>(j.n, 2147483647LL);
+    if ((s = build_plan(j.n, j.row_ptr.data(), nullptr, B, j.tun.chunk, j.tun.max_chunks, light,
+                        64 / j.G, j.c_limit, &sp)) != FE_OK) {
+      j.err = host_error();
+      return s;
+    }
+    if (sp.batches != 1) {
+      j.err = "fe_spmm: more than 2^31 - 1 rows";
+      return FE_ERR_INVALID_ARG;
+    }
+    const size_t n_lcols = (size_t)kLightCols * sp.light_off[1];
+    if ((s = ensure_cap(j, j.sitems, j.cap_sitems, sp.items.size() + n_lcols, "SpMM items")) !=
+        FE_OK)
+      return s;
+    // partials and split counters are shared with the force step (same stream)
+    float *p0 = j.P;
+    int *c0 = j.counters;
+    if ((s = ensure_cap(j, j.P, j.cap_P, (size_t)std::max(sp.max_pslots, 1) * j.d, "partials")) !=
+        FE_OK)
+      return s;
+    if ((s = ensure_cap(j, j.counters, j.cap_ctr, (size_t)std::max(sp.max_splits, 1),
+                        "split counters")) != FE_OK)
+      return s;
+    if (j.P != p0 || j.counters != c0) j.steps_dirty = true;
+    FE_HIP(j, hipMemcpyAsync(j.sitems, sp.items.data(), sizeof(int32_t) * sp.items.size(),
+                             hipMemcpyHostToDevice, st),
+           "SpMM items copy");
+    if (n_lcols > 0)
+      FE_HIP(j, launch_fill_lcols(j.sitems, (int)(sp.items.size() / kItemInts), j.col,
+                                  j.sitems + sp.items.size(), st),
+             "SpMM light columns");
+    if ((s = job_reset_counters(j, st)) != FE_OK) return s;
+    FE_HIP(j, hipStreamSynchronize(st), "SpMM plan sync");
+    j.s_items = (int)sp.item_off[1];
+    j.s_block_items = sp.block_items[0];
+    j.s_light = sp.light_rows[0];
+    j.s_item_ints = sp.items.size();
+    j.s_gen = j.csr_gen;
+    j.s_chunk = j.tun.chunk;
+    j.s_max_chunks = j.tun.max_chunks;
+    j.s_light_cut = light;
+    // the force step's descriptors may hold the old partials or counters
+    if ((s = job_prepare(j, st)) != FE_OK) return s;
+  }
+  // one descriptor: the force step's batch-0 descriptor over the whole plan
+  StepDesc sd = j.host_desc[(size_t)j.plan.batches]; // the unfused descriptor of batch 0
+  sd.items = j.sitems;
+  sd.lcols = j.sitems + j.s_item_ints;
+  sd.n_items = j.s_items;
+  sd.n_block_items = j.s_block_items;
+  sd.n_light = j.s_light;
+  sd.row_begin = 0;
+  sd.rows = (int)j.n;
+  sd.batch = 0;
+  const int wpb = j.tun.waves_per_block;
+  const int blocks = force_blocks_for(sd, wpb * j.tun.items_per_wave);
+  const int pf[2] = {0, blocks};
+  if ((s = ensure_cap(j, j.sdesc, j.cap_sdesc, 1, "SpMM descriptor")) != FE_OK) return s;
+  if ((s = ensure_cap(j, j.sprefix, j.cap_sprefix, 2, "SpMM prefix")) != FE_OK) return s;
+  FE_HIP(j, hipMemcpyAsync(j.sdesc, &sd, sizeof sd, hipMemcpyHostToDevice, st), "SpMM desc");
+  FE_HIP(j, hipMemcpyAsync(j.sprefix, pf, sizeof pf, hipMemcpyHostToDevice, st), "SpMM prefix");
   // state[1] = rot: the kernel reads X through the layout at rest
   if ((s = job_set_epoch(j, 0, st)) != FE_OK) return s;
-  for (int it = 0; it < iterations; ++it) {
-    for (int b = 0; b < j.plan.batches; ++b) {
-      const Shape sh = job_shape(j, b);
-      FE_HIP(j, launch_spmm(job_desc(j, b, DESC_SINGLE), j.prefix + 4 * b, 1, j.force_blocks[b],
-                            sh.waves_per_block, sh.unroll, j.dtype == FE_GATHER_BF16 ? 1 : 0, j.G,
-                            j.Y, st),
-             "SpMM kernel launch");
-    }
-  }
+  for (int it = 0; it < iterations; ++it)
+    FE_HIP(j, launch_spmm(j.sdesc, j.sprefix, 1, blocks, wpb, j.tun.unroll,
+                          j.dtype == FE_GATHER_BF16 ? 1 : 0, j.G, j.Y, st),
+           "SpMM kernel launch");
   if (iterations > 0)
     FE_HIP(j, d2h_staged({{y_out, j.Y, sizeof(float) * (size_t)j.n * j.d}}, st), "fe_spmm: copy");
   return FE_OK;

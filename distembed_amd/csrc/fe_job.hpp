@@ -104,6 +104,19 @@ struct Job {
   float *S = nullptr;        // B x d: new rows or deltas of unfused steps
   float *Y = nullptr;        // n x d: SpMM output (fe_spmm)
   size_t cap_Y = 0;
+  // SpMM keeps its own plan over all rows at once (its rows are independent,
+  // so one launch covers the graph): items then light columns, one
+  // descriptor, its block prefix; rebuilt when the CSR or the cutting changes
+  int32_t *sitems = nullptr;
+  size_t cap_sitems = 0;
+  StepDesc *sdesc = nullptr;
+  size_t cap_sdesc = 0;
+  int *sprefix = nullptr;
+  size_t cap_sprefix = 0;
+  int s_items = 0, s_block_items = 0, s_light = 0;
+  size_t s_item_ints = 0;
+  uint64_t csr_gen = 0, s_gen = ~0ull;
+  int s_chunk = 0, s_max_chunks = 0, s_light_cut = -1;
   float *P = nullptr;
   int *counters = nullptr;
   int32_t *negs = nullptr;

@@ -248,6 +248,90 @@ public:
     stats["Edges+Negatives/s"] = train_ms > 0 ? units / (train_ms * 1e-3) : 0.0;
   }
 };
+
+// SpMMAlgo (cpp/algo/spmm.hpp:11-52): dense_local_output = lr * A' X at
+// world size 1, the SpMM sibling of the force step (fe_spmm). X is read from
+// dense_local and left unchanged; A' skips the columns the reference's
+// dst_id < dst_end_index skips (column N-1).
+template <typename INDEX_TYPE, typename VALUE_TYPE, size_t embedding_dim> class SpMMAlgo {
+  static_assert(embedding_dim % 8 == 0 && embedding_dim >= 8 && embedding_dim <= 512,
+                "embedding_dim must be a multiple of 8 in [8, 512]");
+
+  DenseMat<INDEX_TYPE, VALUE_TYPE, embedding_dim> *dense_local_output;
+  DenseMat<INDEX_TYPE, VALUE_TYPE, embedding_dim> *dense_local;
+  SpMat<VALUE_TYPE> *sp_local_receiver;
+  SpMat<VALUE_TYPE> *sp_local_sender;
+  SpMat<VALUE_TYPE> *sp_local_native;
+  Process3DGrid *grid;
+  double alpha = 0;
+  double beta = 1.0;
+  bool sync = false;
+  bool col_major = false;
+  int gather_dtype = FE_GATHER_FP32;
+  uint32_t quirks = FE_QUIRKS_REFERENCE;
+  fe_ctx *ctx = nullptr;
+  std::map<std::string, double> stats;
+
+public:
+  // per-row time of the reference's row loop (cpp/algo/spmm.hpp:398-406); the
+  // device kernel has no per-row clock, so its time is spread evenly
+  std::vector<double> timing_info;
+
+  SpMMAlgo(SpMat<VALUE_TYPE> *sp_local_native, SpMat<VALUE_TYPE> *sp_local_receiver,
+           SpMat<VALUE_TYPE> *sp_local_sender,
+           DenseMat<INDEX_TYPE, VALUE_TYPE, embedding_dim> *dense_local,
+           DenseMat<INDEX_TYPE, VALUE_TYPE, embedding_dim> *dense_local_output,
+           Process3DGrid *grid, double alpha, double beta, bool col_major, bool sync_comm)
+      : dense_local_output(dense_local_output), dense_local(dense_local),
+        sp_local_receiver(sp_local_receiver), sp_local_sender(sp_local_sender),
+        sp_local_native(sp_local_native), grid(grid), alpha(alpha), beta(beta),
+        sync(sync_comm), col_major(col_major) {}
+  SpMMAlgo(const SpMMAlgo &) = delete;
+  SpMMAlgo &operator=(const SpMMAlgo &) = delete;
+  ~SpMMAlgo() {
+    if (ctx) fe_destroy(ctx);
+  }
+
+  void set_gather_dtype(int dtype) { gather_dtype = dtype; }
+  void set_quirks(uint32_t q) { quirks = q; }
+  // "Total Time" (ms), "Local SpMM" (ms, device part), "Edges/s".
+  const std::map<std::string, double> &perf_stats() const { return stats; }
+
+  void algo_spmm(int iterations, int batch_size, VALUE_TYPE lr) {
+    const auto t0 = std::chrono::steady_clock::now();
+    if (grid && grid->world_size != 1)
+      throw std::runtime_error("algo_spmm: one GPU per process; a grid of " +
+                               std::to_string(grid->world_size) + " ranks is not supported");
+    const int64_t n = (int64_t)dense_local->rows;
+    const int d = (int)embedding_dim;
+    if ((int64_t)sp_local_native->gRows != n || sp_local_native->row_ptr.size() != (size_t)n + 1 ||
+        (int64_t)dense_local_output->rows != n)
+      throw std::runtime_error("algo_spmm: CSR rows differ from the dense rows");
+    if (!ctx) {
+      detail::check(fe_create(&ctx, grid ? grid->device : 0, n, d, gather_dtype, quirks),
+                    nullptr, "fe_create");
+      detail::check(fe_upload_csr(ctx, sp_local_native->row_ptr.data(),
+                                  sp_local_native->col_idx.data(),
+                                  (int64_t)sp_local_native->col_idx.size()),
+                    ctx, "fe_upload_csr");
+    }
+    std::vector<float> x((size_t)n * d), y((size_t)n * d);
+    for (size_t i = 0; i < x.size(); ++i) x[i] = (float)dense_local->nCoordinates[i];
+    detail::check(fe_upload_x(ctx, x.data()), ctx, "fe_upload_x");
+    const auto t1 = std::chrono::steady_clock::now();
+    detail::check(fe_spmm(ctx, iterations, batch_size, (float)lr, y.data()), ctx, "fe_spmm");
+    const auto t2 = std::chrono::steady_clock::now();
+    if (iterations > 0)
+      for (size_t i = 0; i < y.size(); ++i) dense_local_output->nCoordinates[i] = (VALUE_TYPE)y[i];
+    const double spmm_ms = std::chrono::duration<double, std::milli>(t2 - t1).count();
+    timing_info.assign((size_t)n, n > 0 ? spmm_ms / (double)n : 0.0);
+    stats["Total Time"] =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats["Local SpMM"] = spmm_ms;
+    stats["Edges/s"] =
+        spmm_ms > 0 ? (double)sp_local_native->col_idx.size() * iterations / (spmm_ms * 1e-3) : 0.0;
+  }
+};
 } // namespace algo
 
 } // namespace distembed_amd

@@ -66,7 +66,7 @@ def _driver(*args, cwd=None):
 
 
 def test_driver_rejects_algorithms_not_built(tmp_path):
-    for flag in ("-spmm", "-spgemm", "-sparse_embedding", "-msbfs"):
+    for flag in ("-spgemm", "-sparse_embedding", "-msbfs"):
         r = _driver(flag, "1", "-sbm", "100", cwd=tmp_path)
         assert r.returncode == 2 and "only the force-model embedding" in r.stderr
 

@@ -68,3 +68,18 @@ def test_driver_bf16_gather_matches_bf16_oracle(tmp_path):
                   "-gather", "bf16")
     want = _oracle_x(_sbm(n), d, B, T, 5, bf16=True)
     np.testing.assert_allclose(got, want, rtol=0, atol=5e-6 + 1e-3 * np.abs(want).max())
+
+
+def test_driver_spmm_runs_one_record_per_iteration(tmp_path):
+    """-spmm 1: BaselineSpMM::execute's loop (cpp/algo/baseline_spmm.hpp:51-88),
+    one perf_stats entry per iteration, no embedding file."""
+    if not build.DRIVER.exists():
+        pytest.fail("dist_embed driver is not built")
+    r = subprocess.run([str(build.DRIVER), "-sbm", "900", "-dim", "64", "-spmm", "1", "-iter",
+                        "2", "-lr", "0.5"], capture_output=True, text=True, cwd=tmp_path,
+                       timeout=300)
+    assert r.returncode == 0, r.stderr
+    perf = (tmp_path / "perf_output").read_text()
+    assert perf.count('"Local SpMM"') == 2 and perf.count('"Edges/s"') == 2
+    assert '"batch_size": 900' in perf and perf.rstrip().endswith(",")
+    assert not (tmp_path / "embedding.txt").exists()
