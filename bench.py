@@ -204,6 +204,8 @@ def main() -> int:
                     help='json {"tuning": {fe_tuning fields}, "per_batch": [[unroll, '
                          'waves_per_block, items_per_wave, chunk, max_chunks, light] per '
                          'batch]} (as reported in config.tuning); skips the autotuner')
+    ap.add_argument("--bf16-tuning", type=str, default="",
+                    help="as --tuning, for the BF16 pass (skips its autotuner)")
     ap.add_argument("--form", choices=["plain", "precision", "batching", "serving", "spmm"],
                     default="plain")
     ap.add_argument("--epochs", type=int, default=50, help="epochs of the batching/serving jobs")
@@ -298,7 +300,8 @@ def main() -> int:
         "loss_fp64_after_timed_epochs": f32["loss"],
     }
     if not args.no_bf16:
-        b16 = run_form(api, g, X0, lists, api.BF16, args, None)
+        b16 = run_form(api, g, X0, lists, api.BF16, args,
+                       json.loads(args.bf16_tuning) if args.bf16_tuning else None)
         kb = b16["kt"]
         log(f"bf16: {args.steps} epochs in {b16['dt']:.3f} s; force {kb['force_ms'] * 1e3:.1f} us "
             f"per launch; loss {b16['loss']:.6e}")

@@ -1,11 +1,13 @@
 """Summarise rocprofv3 CSV output into the short text kept under profiles/.
 
-  python -m distembed_amd.profile_summary trace  <kernel_stats.csv> <kernel_trace.csv> <out> <command>
-  python -m distembed_amd.profile_summary pmc <fetch csv> <write csv> <out> <command> <copyprobe fetch csv> <copyprobe write csv>
+  python -m distembed_amd.profile_summary trace  <kernel_stats.csv> <kernel_trace.csv> <out> <command> [<regex>]
+  python -m distembed_amd.profile_summary pmc <fetch csv> <write csv> <out> <command> <copyprobe fetch csv> <copyprobe write csv> [<regex>]
   python -m distembed_amd.profile_summary copyprobe
 
 trace: per-kernel launch count, total and average duration (from the
---kernel-trace --stats run), plus the force kernel's per-launch average.
+--kernel-trace --stats run), plus the force kernel's per-launch average
+(only the instantiations whose name matches <regex>, when given: e.g.
+"force_kernel<\d+, 1," for the BF16-gather ones).
 pmc: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the force kernel,
 scaled by what the same counters read for a cold 1 GiB device-to-device
 copy (copyprobe), and the resulting HBM bytes per force launch
@@ -14,6 +16,7 @@ copy (copyprobe), and the resulting HBM bytes per force launch
 from __future__ import annotations
 
 import csv
+import re
 import statistics
 import sys
 from collections import defaultdict
@@ -23,7 +26,7 @@ def _short(name: str) -> str:
     return name.split("(")[0].replace("void ", "").strip()
 
 
-def trace(stats_csv: str, trace_csv: str, out: str, command: str) -> None:
+def trace(stats_csv: str, trace_csv: str, out: str, command: str, match: str = "") -> None:
     rows = list(csv.DictReader(open(stats_csv)))
     lines = [f"# rocprofv3 kernel statistics", f"command: {command}", ""]
     lines.append(f"{'kernel':60s} {'calls':>7s} {'total_ms':>10s} {'avg_us':>9s}")
@@ -40,7 +43,7 @@ def trace(stats_csv: str, trace_csv: str, out: str, command: str) -> None:
     lines.append("")
     allv = []
     for k, v in sorted(durs.items()):
-        if "force_kernel" in k:
+        if "force_kernel" in k and (not match or re.search(match, k)):
             allv += v
             lines.append(f"force launches {k}: n={len(v)} avg_us={statistics.mean(v):.2f} "
                          f"median_us={statistics.median(v):.2f} max_us={max(v):.2f}")
@@ -61,7 +64,7 @@ def _per_dispatch(path: str, counter: str):
 
 
 def pmc(fetch_csv: str, write_csv: str, out: str, command: str, cal_fetch_csv: str,
-        cal_write_csv: str) -> None:
+        cal_write_csv: str, match: str = "") -> None:
     fetch = _per_dispatch(fetch_csv, "FETCH_SIZE")
     write = _per_dispatch(write_csv, "WRITE_SIZE")
     cal_f = calibration(cal_fetch_csv, "FETCH_SIZE")
@@ -76,7 +79,7 @@ def pmc(fetch_csv: str, write_csv: str, out: str, command: str, cal_fetch_csv: s
     tot_f = tot_w = 0.0
     tot_n = 0
     for k in sorted(fetch):
-        if "force_kernel" not in k:
+        if "force_kernel" not in k or (match and not re.search(match, k)):
             continue
         f = statistics.mean(fetch[k].values())
         w = statistics.mean(write.get(k, {0: 0.0}).values())
@@ -133,8 +136,10 @@ def calibration(path: str, counter: str, nbytes: int = 1 << 30) -> float:
 if __name__ == "__main__":
     mode = sys.argv[1]
     if mode == "trace":
-        trace(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5])
+        trace(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5],
+              sys.argv[6] if len(sys.argv) > 6 else "")
     elif mode == "copyprobe":
         copyprobe()
     else:
-        pmc(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5], sys.argv[6], sys.argv[7])
+        pmc(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5], sys.argv[6], sys.argv[7],
+            sys.argv[8] if len(sys.argv) > 8 else "")
