@@ -174,8 +174,9 @@ static fe_status ehip(fe_engine *e, hipError_t h, const char *what) {
 
 // Round layout in the staging buffers for K steps:
 //   StepDesc[K] | int state[4K] | int force_prefix[K+1] | int apply_prefix[K+1]
-// state of step k: {epoch, rot 0, shift 0, unused} (engine jobs keep the
-// layout at rest with rotation 0).
+// state of step k: {epoch, rot, (rot + epoch) mod P, unused} of its job: a
+// fused step reads its job's rotating slot layout and writes the free slot
+// (the apply prefix is kept for the layout of the buffer; no apply runs).
 constexpr int kStateInts = 4;
 static size_t round_bytes(size_t K) {
   size_t b = K * sizeof(StepDesc);
