@@ -92,3 +92,28 @@ def test_spmm_after_training_reads_the_rotated_layout():
         y = ctx.spmm(1, 1024, 0.05)
         np.testing.assert_array_equal(ctx.download_x(), X)  # X unchanged
     _check(y, g, X, 0.05, what="after training")
+
+
+def test_spmm_edge_cases():
+    """No edges (Y = 0), two vertices (column n-1 skipped by the quirk),
+    d = 8 (one lane per row, 64 rows per wave), and isolated rows mixed with a
+    heavy row."""
+    g0 = api.Graph.from_edges(500, np.array([], dtype=np.int64), np.array([], dtype=np.int64))
+    X0 = api.init_uniform(500, 64, seed=1)
+    np.testing.assert_array_equal(_spmm(g0, X0, 128, 0.5), np.zeros_like(X0))
+    g1 = api.Graph.from_edges(2, np.array([0]), np.array([1]))
+    X1 = api.init_uniform(2, 16, seed=2)
+    y = _spmm(g1, X1, 1, 0.5, quirks=0)  # one row per batch
+    np.testing.assert_array_equal(y, 0.5 * X1[::-1])
+    y = _spmm(g1, X1, 2, 0.5)  # the quirk skips column n-1 = 1
+    np.testing.assert_array_equal(y, np.stack([np.zeros(16, np.float32), 0.5 * X1[0]]))
+    for gather in (api.FP32, api.BF16):
+        g = api.Graph.sbm(3000, 6, 0.01, 0.001, seed=4)
+        X = api.init_uniform(g.n, 8, seed=3)
+        Xr = X if gather == api.FP32 else orc.bf16_round(X)
+        _check(_spmm(g, X, 1000, 0.02, gather=gather), g, Xr, 0.02, what=f"d=8 gather={gather}")
+    star = np.arange(1, 4000)
+    g = api.Graph.from_edges(8000, np.zeros_like(star), star)  # vertex 0: degree 3999
+    X = api.init_uniform(g.n, 128, seed=5)
+    _check(_spmm(g, X, 2048, 0.1, tuning=dict(chunk=16, max_chunks=1024)), g, X, 0.1,
+           what="star")
