@@ -453,3 +453,36 @@ def test_bf16_training_tracks_bf16_oracle():
     orc.train(g, Xr, B, T, lists, bf16=True)
     assert np.linalg.norm(Xg - Xr) <= 1e-5 * np.linalg.norm(Xr)
     assert np.abs(Xg - Xr).max() <= 1e-3 * np.abs(Xr).max()
+
+
+def test_gpu_matches_committed_golden():
+    """The HIP path against the committed fixtures (tests/golden): FP32
+    training to the FP32 bar with the fixture's FP64 result as the rounding
+    reference, the BF16 form as in the BF16 training test, and SpMM."""
+    import pathlib
+    import types
+    files = sorted((pathlib.Path(__file__).parent / "golden").glob("*.npz"))
+    assert files
+    for p in files:
+        z = np.load(p, allow_pickle=False)
+        g = types.SimpleNamespace(n=int(z["n"]), row_ptr=z["row_ptr"], col_idx=z["col_idx"])
+        gg = api.Graph.from_edges(g.n, np.repeat(np.arange(g.n), np.diff(g.row_ptr)), g.col_idx)
+        assert np.array_equal(gg.row_ptr, g.row_ptr) and np.array_equal(gg.col_idx, g.col_idx)
+        d, B, T, ns, lr = int(z["d"]), int(z["B"]), int(z["epochs"]), int(z["ns"]), float(z["lr"])
+        x0 = z["x0"].astype(np.float32)
+        for gather, want in ((api.FP32, z["x_fp32"]), (api.BF16, z["x_bf16"])):
+            ctx = _ctx(gg, d, x0, B, z["lists"], gather=gather, lr=lr)
+            if gather == api.FP32:
+                y = ctx.spmm(1, B, lr)  # on X0, which SpMM leaves unchanged
+            ctx.train(T, B, ns, lr)
+            xg = ctx.download_x()
+            ctx.close()
+            if gather == api.FP32:
+                bound = RTOL * (np.abs(want) + np.sqrt(np.mean(want.astype(np.float64) ** 2))) \
+                    + 2 * np.abs(want.astype(np.float64) - z["x_fp64"]) + 1e-7
+                assert (np.abs(xg.astype(np.float64) - want) <= bound).all(), p.stem
+                y64 = z["spmm_fp64"]
+                assert np.allclose(y, y64, rtol=1e-5, atol=1e-6 * np.abs(y64).max()), p.stem
+            else:
+                assert np.linalg.norm(xg - want) <= 1e-5 * np.linalg.norm(want), p.stem
+                assert np.abs(xg - want).max() <= 1e-3 * np.abs(want).max(), p.stem

@@ -363,3 +363,41 @@ def test_spmm_fp32_tracks_fp64():
     y32 = orc.spmm(g, X.astype(np.float32), 0.02)
     assert y32.dtype == np.float32
     np.testing.assert_allclose(y32, y64, rtol=1e-4, atol=1e-6)
+
+
+# ---- committed fixtures (tests/golden, written by make_golden.py) ----------
+
+def _golden():
+    import pathlib
+    import types
+    out = []
+    for p in sorted((pathlib.Path(__file__).parent / "golden").glob("*.npz")):
+        z = np.load(p, allow_pickle=False)
+        g = types.SimpleNamespace(n=int(z["n"]), row_ptr=z["row_ptr"], col_idx=z["col_idx"])
+        out.append((p.stem, z, g))
+    assert out, "no fixtures under tests/golden"
+    return out
+
+
+def test_oracle_reproduces_committed_golden():
+    """The oracle recomputes every committed fixture bit for bit: FP64 and
+    FP32 training, the BF16-shadow form, batch 0's FP64 delta, and SpMM."""
+    for name, z, g in _golden():
+        B, T, lr = int(z["B"]), int(z["epochs"]), float(z["lr"])
+        lists = z["lists"]
+        x = z["x0"].copy()
+        orc.train(g, x, B, T, lists, lr=lr)
+        np.testing.assert_array_equal(x, z["x_fp64"], err_msg=name)
+        x = z["x0"].astype(np.float32)
+        orc.train(g, x, B, T, lists, lr=lr)
+        np.testing.assert_array_equal(x, z["x_fp32"], err_msg=name)
+        x = z["x0"].astype(np.float32)
+        orc.train(g, x, B, T, lists, lr=lr, bf16=True)
+        np.testing.assert_array_equal(x, z["x_bf16"], err_msg=name)
+        dl = orc.step(g, z["x0"].copy(), B, 0, 0, lists, lr=lr, apply=False, want_delta=True,
+                      dtype=np.float64)
+        np.testing.assert_array_equal(dl, z["delta0_fp64"], err_msg=name)
+        np.testing.assert_array_equal(orc.spmm(g, z["x0"], lr), z["spmm_fp64"], err_msg=name)
+        # the fixtures hold real work: training moved X, SpMM is not zero
+        assert np.abs(z["x_fp64"] - z["x0"]).max() > 1e-3
+        assert np.abs(z["spmm_fp64"]).max() > 0
