@@ -6,7 +6,8 @@ batching: 512 independent SBM graphs (n log-uniform in [1e3, 2e5], 4-32
   blocks, average degree uniform in [8, 40] with 90 % of it inside the
   block), d=64, B=1024, 5 negatives, 50 epochs, FP32 and BF16; all graphs
   resident, advanced together round by round, the rounds captured in HIP
-  graphs (fe_engine_run_graph), tiles chosen by the autotuner.
+  graphs (fe_engine_run_graph), tiles chosen by an empirical search on the
+  mix itself (tune_engine).
 serving: Poisson arrivals of R-MAT jobs (edge count log-uniform in
   [2e4, 2e7], d from {32, 64, 128}), 50 epochs, 5 negatives, admitted under
   an HBM budget and interleaved round by round (fe_engine_run); submission
@@ -107,6 +108,28 @@ def sbm_pool(count: int, seed: int = 0):
         return list(ex.map(lambda s: api.Graph.sbm(*s), specs))
 
 
+def tune_engine(graphs, X0s, B: int, epochs: int, ns: int, gather: int, budget_gib: float):
+    """Empirical search of the engine's launch tiles on the mixed workload
+    itself: each candidate runs the given jobs through fe_engine_run_graph
+    and the fastest replay (HIP-event time) wins. A tile that suits one graph
+    alone can lose on rounds that mix hundreds of jobs of other sizes."""
+    cands = [dict(unroll=u, waves_per_block=w, items_per_wave=i, chunk=64, max_chunks=1024,
+                  light=li)
+             for u in (2, 4) for w in (4, 8) for i in (1, 2) for li in (8, 16)]
+    best, best_ms, table = None, float("inf"), []
+    for t in cands:
+        with api.Engine(int(budget_gib * 2 ** 30)) as eng:
+            eng.set_tuning(**t)
+            for g, x in zip(graphs, X0s):
+                eng.submit(g, x, B, epochs, ns=ns, gather=gather)
+            eng.run_graph()
+            ms = eng.graph_timing()["exec_ms"]
+        table.append((t, ms))
+        if ms < best_ms:
+            best, best_ms = t, ms
+    return best, best_ms, table
+
+
 def batching(epochs: int, count: int = 512, d: int = 64, B: int = 1024, ns: int = 5,
              budget_gib: float = 96.0) -> dict:
     t0 = time.perf_counter()
@@ -117,18 +140,15 @@ def batching(epochs: int, count: int = 512, d: int = 64, B: int = 1024, ns: int 
         f"({time.perf_counter() - t0:.1f} s)")
     X0s = [api.init_uniform(g.n, d, -0.5, 0.5, seed=i) for i, g in enumerate(graphs)]
     units = sum(_units(g, B, ns, epochs) for g in graphs)
-    # tiles for this shape: autotune on the median-sized graph
-    med = sorted(range(count), key=lambda i: graphs[i].n)[count // 2]
+    # tiles for this mix: searched on every 2nd graph for 2 epochs
     res = {"form": "batching", "graphs": count, "vertices": n_tot, "csr_entries": nnz_tot,
            "d": d, "batch": B, "ns": ns, "epochs": epochs, "units": units}
     for name, gather in (("fp32", api.FP32), ("bf16", api.BF16)):
-        with api.ForceEmbed(graphs[med].n, d, gather=gather) as ctx:
-            ctx.upload_csr(graphs[med])
-            ctx.upload_x(X0s[med])
-            ctx.configure(B, 0.02, 5.0)
-            ctx.set_negatives(api.reference_negatives(graphs[med].n, 2 + -(-graphs[med].n // B), ns)[0])
-            best, best_ms, _ = ctx.autotune(epochs=2, search_chunking=True)
-        log(f"batching {name}: autotuned on n={graphs[med].n}: {best}")
+        t0 = time.perf_counter()
+        best, best_ms, table = tune_engine(graphs[::2], X0s[::2], B, min(epochs, 2), ns, gather,
+                                           budget_gib)
+        log(f"batching {name}: tiles {best} ({best_ms:.1f} ms on the sample; {len(table)} "
+            f"candidates in {time.perf_counter() - t0:.1f} s)")
         with api.Engine(int(budget_gib * 2 ** 30)) as eng:
             eng.set_tuning(**best)
             t1 = time.perf_counter()
