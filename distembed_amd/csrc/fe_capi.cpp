@@ -17,6 +17,7 @@ struct fe_ctx {
   hipGraphExec_t graph_exec[2] = {nullptr, nullptr};
   bool graph_valid = false;
   bool graph_timed = false;
+  uint64_t graph_gen = 0; // descriptor generation the graph was captured with
   // force-kernel timing with HIP events on the context's stream
   bool timing = false;
   std::vector<hipEvent_t> events; // 2 per timed force launch
@@ -307,6 +308,7 @@ static fe_status capture_epochs(fe_ctx *c) {
     return fail(c, FE_ERR_STATE, "kernel timing: unexpected event count in the captured epoch");
   c->graph_valid = true;
   c->graph_timed = c->timing;
+  c->graph_gen = c->job.desc_gen;
   return FE_OK;
 }
 
@@ -336,7 +338,7 @@ extern "C" fe_status fe_train(fe_ctx *c, int iterations, int batch_size, int ns,
     return s;
   int done = 0;
   if (use_graph) {
-    if (!c->graph_valid || c->graph_timed != c->timing) {
+    if (!c->graph_valid || c->graph_timed != c->timing || c->graph_gen != j.desc_gen) {
       c->events_used = 0;
       if ((s = capture_epochs(c)) != FE_OK) return s;
     }

@@ -121,6 +121,7 @@ struct EngineJob {
   bool launched_all = false;
   bool collecting = false; // a collect is downloading the result
   bool collected = false;
+  bool failed = false;     // a round of this job failed: it has no result
   hipEvent_t uploaded = nullptr; // uploads finished (on the upload stream)
   hipEvent_t done = nullptr;     // the job's last round finished
   bool waited_upload = false;
@@ -189,6 +190,8 @@ static fe_status ensure_stage(fe_engine *e, size_t K) {
   size_t cap = std::max<size_t>(64, e->stage_cap);
   while (cap < K) cap *= 2;
   hipStreamSynchronize(e->stream);
+  // a failure below leaves no stage usable: the next call allocates again
+  e->stage_cap = 0;
   for (auto &s : e->stage) {
     if (s.host) hipHostFree(s.host);
     s.host = nullptr;
@@ -477,11 +480,33 @@ static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *
   return FE_OK;
 }
 
+// Jobs of a round that could not be enqueued have lost steps: they end
+// without a result (collect reports the failure and frees them).
+static void mark_failed(const std::vector<EngineJob *> &run) {
+  for (EngineJob *ej : run) {
+    ej->failed = true;
+    ej->launched_all = true;
+  }
+}
+
+static fe_status run_rounds(fe_engine *e, int rounds);
+
 extern "C" fe_status fe_engine_run(fe_engine *e, int rounds, int *running) {
   if (!e) return FE_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> lk(e->mu);
   fe_status s = ehip(e, hipSetDevice(e->device), "hipSetDevice");
-  if (s != FE_OK) return s;
+  if (s == FE_OK) s = run_rounds(e, rounds);
+  if (running) {
+    int n = 0;
+    for (auto &jp : e->jobs)
+      if (!jp->launched_all && !jp->collected) ++n;
+    *running = n;
+  }
+  return s;
+}
+
+static fe_status run_rounds(fe_engine *e, int rounds) {
+  fe_status s = FE_OK;
   int done_rounds = 0;
   while (rounds <= 0 || done_rounds < rounds) {
     std::vector<EngineJob *> run;
@@ -504,30 +529,27 @@ extern "C" fe_status fe_engine_run(fe_engine *e, int rounds, int *running) {
     size_t lds = 0;
     std::vector<EngineJob *> finished;
     build_round(e, run, stg.host, K, lds, finished);
+    // from here on the jobs have advanced: a failure ends them all
     const int *hpf = reinterpret_cast<const int *>(stg.host + K * sizeof(StepDesc)) + kStateInts * K;
     const int fblocks = hpf[K];
-    if ((s = ehip(e, hipMemcpyAsync(e->dev_round, stg.host, round_bytes(K), hipMemcpyHostToDevice,
-                                    e->stream),
-                  "round copy")) != FE_OK)
-      return s;
-    if ((s = ehip(e, hipEventRecord(stg.copied, e->stream), "event")) != FE_OK) return s;
+    s = ehip(e, hipMemcpyAsync(e->dev_round, stg.host, round_bytes(K), hipMemcpyHostToDevice,
+                               e->stream),
+             "round copy");
+    if (s == FE_OK) s = ehip(e, hipEventRecord(stg.copied, e->stream), "event");
     const StepDesc *dsd = reinterpret_cast<const StepDesc *>(e->dev_round);
     const int *dpf = reinterpret_cast<const int *>(e->dev_round + K * sizeof(StepDesc)) + kStateInts * K;
-    if ((s = ehip(e, launch_force(dsd, dpf, K, fblocks, e->tun.waves_per_block,
-                                  e->tun.items_per_wave, e->tun.unroll, lds, round_mode(run),
-                                  round_g(run), e->stream),
-                  "force launch")) != FE_OK)
-      return s;
+    if (s == FE_OK)
+      s = ehip(e, launch_force(dsd, dpf, K, fblocks, e->tun.waves_per_block, e->tun.items_per_wave,
+                               e->tun.unroll, lds, round_mode(run), round_g(run), e->stream),
+               "force launch");
     for (EngineJob *ej : finished)
-      if ((s = ehip(e, hipEventRecord(ej->done, e->stream), "event")) != FE_OK) return s;
+      if (s == FE_OK) s = ehip(e, hipEventRecord(ej->done, e->stream), "event");
+    if (s != FE_OK) {
+      mark_failed(run);
+      return s;
+    }
     ++e->rounds_run;
     ++done_rounds;
-  }
-  if (running) {
-    int n = 0;
-    for (auto &jp : e->jobs)
-      if (!jp->launched_all && !jp->collected) ++n;
-    *running = n;
   }
   return FE_OK;
 }
@@ -538,7 +560,7 @@ extern "C" fe_status fe_engine_job_done(fe_engine *e, int job_id, int *done) {
   if (job_id < 0 || job_id >= (int)e->jobs.size())
     return efail(e, FE_ERR_INVALID_ARG, "fe_engine_job_done: bad job id");
   EngineJob *ej = e->jobs[job_id].get();
-  if (ej->collected || ej->collecting) {
+  if (ej->collected || ej->collecting || ej->failed) {
     *done = 1;
     return FE_OK;
   }
@@ -573,6 +595,16 @@ extern "C" fe_status fe_engine_collect(fe_engine *e, int job_id, float *x_out) {
     if (ej->collected || ej->collecting)
       return efail(e, FE_ERR_STATE, "fe_engine_collect: already collected");
     if (!ej->launched_all) return efail(e, FE_ERR_STATE, "fe_engine_collect: job still running");
+    if (ej->failed) {
+      // its memory is freed once the device no longer uses it
+      hipStreamSynchronize(e->stream);
+      job_free(ej->job);
+      hipEventDestroy(ej->uploaded);
+      hipEventDestroy(ej->done);
+      ej->uploaded = ej->done = nullptr;
+      ej->collected = true;
+      return efail(e, FE_ERR_STATE, "fe_engine_collect: a round of this job failed; no result");
+    }
     ej->collecting = true;
   }
   Job &j = ej->job;
@@ -648,8 +680,10 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
     total += rb;
   }
   char *dsched = nullptr;
-  if ((s = ehip(e, hipMalloc((void **)&dsched, std::max<size_t>(total, 256)), "schedule buffer")) != FE_OK)
+  if ((s = ehip(e, hipMalloc((void **)&dsched, std::max<size_t>(total, 256)), "schedule buffer")) != FE_OK) {
+    mark_failed(all);
     return s;
+  }
   // rebase the per-round epoch pointers (built against dev_round) onto the schedule
   for (const RoundInfo &ri : rounds) {
     StepDesc *sd = reinterpret_cast<StepDesc *>(host.data() + ri.offset);
@@ -709,6 +743,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
   if (s == FE_OK) s = ehip(e, hipStreamSynchronize(e->stream), "graph sync");
   float ms = 0.f;
   if (s == FE_OK) s = ehip(e, hipEventElapsedTime(&ms, ev0, ev1), "event time");
+  if (s != FE_OK) mark_failed(all);
   e->graph_exec_ms = ms;
   e->graph_prepare_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
   e->graph_rounds = (int64_t)rounds.size();

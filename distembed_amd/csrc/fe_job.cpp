@@ -678,6 +678,7 @@ fe_status job_prepare(Job &j, hipStream_t st, bool slots) {
            "prefix copy");
     FE_HIP(j, hipStreamSynchronize(st), "descriptor sync");
     j.host_desc.swap(hd);
+    ++j.desc_gen;
     j.steps_dirty = false;
   }
   return FE_OK;
@@ -705,9 +706,11 @@ fe_status job_launch_force(Job &j, int b, DescKind kind, hipStream_t st, hipEven
   const size_t lds = force_lds_bytes(j.ns, j.d);
   if (ev) FE_HIP(j, record_event(ev[0], st), "event");
   const Shape sh = job_shape(j, b);
-  FE_HIP(j, launch_force(job_desc(j, b, kind), j.prefix + 4 * b, 1, j.force_blocks[b],
-                         sh.waves_per_block, sh.items_per_wave, sh.unroll, lds,
-                         j.dtype == FE_GATHER_BF16 ? 1 : 0, j.G, st),
+  // the descriptor by value: a graph captured here must be dropped when the
+  // descriptors are rebuilt (desc_gen)
+  const StepDesc &sd = j.host_desc[(size_t)j.plan.batches * kind + b];
+  FE_HIP(j, launch_force1(sd, j.force_blocks[b], sh.waves_per_block, sh.unroll, lds,
+                          j.dtype == FE_GATHER_BF16 ? 1 : 0, j.G, st),
          "force kernel launch");
   if (ev) FE_HIP(j, record_event(ev[1], st), "event");
   return FE_OK;

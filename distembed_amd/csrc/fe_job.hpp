@@ -18,6 +18,9 @@ namespace fe {
 hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
                         int wpb, int ipw, int unroll, size_t lds_bytes, int mode, int g_const,
                         hipStream_t st);
+// One step whose descriptor is passed by value (mode 0 or 1).
+hipError_t launch_force1(const StepDesc &sd, int n_blocks, int wpb, int unroll, size_t lds_bytes,
+                         int mode, int g_const, hipStream_t st);
 hipError_t launch_spmm(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
                        int wpb, int unroll, int mode, int g_const, float *Y, hipStream_t st);
 hipError_t launch_apply(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
@@ -133,6 +136,8 @@ struct Job {
   size_t cap_prefix = 0, cap_negcur = 0, cap_edesc = 0, cap_col = 0, cap_X = 0, cap_Xs = 0, cap_epoch = 0;
   bool plan_dirty = true;  // plan must be rebuilt (batch size, chunking, CSR)
   bool steps_dirty = true; // descriptors must be rebuilt
+  uint64_t desc_gen = 0;    // bumped whenever the descriptors are rebuilt: a
+                            // captured launch holds a copy of its descriptor
   int64_t device_bytes = 0;
   std::string err;
 };

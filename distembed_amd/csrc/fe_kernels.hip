@@ -639,17 +639,11 @@ __device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src
 // if it has no unit), so staging overlaps the first gathers. A fused step
 // writes its new rows into the layout's free slot, which no thread of the
 // launch reads. GT > 0: every step of the launch has G = GT lanes per row (a
-// compile-time constant); 0: per step.
+// compile-time constant); 0: per step. lb, nb: this block's index among the
+// step's blocks and their count.
 template <int U, int MODE, int GT>
-__global__ __launch_bounds__(512) void force_kernel(
-    const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
-    int n_steps, int ipw) {
+__device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
   extern __shared__ float lds[];
-  const int b = blockIdx.x;
-  const int si = find_step(blk_prefix, n_steps, b);
-  const StepDesc &sd = steps[si];
-  const int lb = b - blk_prefix[si];
-  const int nb = blk_prefix[si + 1] - blk_prefix[si];
   const int D = sd.D;
   const int G = GT > 0 ? GT : sd.G;
   const int NG = 64 / G;
@@ -801,7 +795,26 @@ __global__ __launch_bounds__(512) void force_kernel(
     }
     if (!synced) __syncthreads();
   }
+}
+
+// Steps of one or more jobs in one launch (engine rounds): the block finds
+// its step in the block prefix and reads the step's descriptor from memory.
+template <int U, int MODE, int GT>
+__global__ __launch_bounds__(512) void force_kernel(
+    const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
+    int n_steps, int ipw) {
+  const int b = blockIdx.x;
+  const int si = find_step(blk_prefix, n_steps, b);
+  force_body<U, MODE, GT>(steps[si], b - blk_prefix[si], blk_prefix[si + 1] - blk_prefix[si]);
   (void)ipw;
+}
+
+// One step (a single graph's minibatch): the descriptor travels in the
+// kernel arguments, so the first loads of the items, the columns and the
+// epoch state do not wait for a descriptor load.
+template <int U, int MODE, int GT>
+__global__ __launch_bounds__(512) void force_kernel1(const StepDesc sd) {
+  force_body<U, MODE, GT>(sd, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ---- SpMM sibling: SpMMAlgo::algo_spmm at world size 1 ----------------------
@@ -1226,6 +1239,34 @@ static void launch_force_g(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t 
   if (mode == 0) hipLaunchKernelGGL((force_kernel<U, 0, GT>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
   else if (mode == 1) hipLaunchKernelGGL((force_kernel<U, 1, GT>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
   else hipLaunchKernelGGL((force_kernel<U, 2, GT>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
+}
+
+template <int U, int GT>
+static void launch_force1_g(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
+                            const StepDesc &sd) {
+  if (mode == 1) hipLaunchKernelGGL((force_kernel1<U, 1, GT>), grid, block, lds_bytes, st, sd);
+  else hipLaunchKernelGGL((force_kernel1<U, 0, GT>), grid, block, lds_bytes, st, sd);
+}
+
+template <int U>
+static void launch_force1_u(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
+                            int g_const, const StepDesc &sd) {
+  if (g_const == 16) launch_force1_g<U, 16>(grid, block, lds_bytes, st, mode, sd);
+  else if (g_const == 8) launch_force1_g<U, 8>(grid, block, lds_bytes, st, mode, sd);
+  else if (g_const == 4) launch_force1_g<U, 4>(grid, block, lds_bytes, st, mode, sd);
+  else launch_force1_g<U, 0>(grid, block, lds_bytes, st, mode, sd);
+}
+
+hipError_t launch_force1(const StepDesc &sd, int n_blocks, int wpb, int unroll, size_t lds_bytes,
+                         int mode, int g_const, hipStream_t st) {
+  dim3 grid(n_blocks), block(64 * wpb);
+  switch (unroll) {
+  case 1: launch_force1_u<1>(grid, block, lds_bytes, st, mode, g_const, sd); break;
+  case 2: launch_force1_u<2>(grid, block, lds_bytes, st, mode, g_const, sd); break;
+  case 8: launch_force1_u<8>(grid, block, lds_bytes, st, mode, g_const, sd); break;
+  default: launch_force1_u<4>(grid, block, lds_bytes, st, mode, g_const, sd); break;
+  }
+  return hipGetLastError();
 }
 
 template <int U>
