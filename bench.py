@@ -291,7 +291,7 @@ def main() -> int:
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": (achieved / HBM_PEAK_GBPS) if achieved else None,
             "traffic": traffic,
-            "kernel": "force_kernel",
+            "kernel": "force_kernel1",
             "algorithmic_bytes_per_launch": bytes_per_launch,
             "avg_launch_ms": kt["force_ms"],
             "launches_timed": kt["launches"],

@@ -7,7 +7,7 @@
 trace: per-kernel launch count, total and average duration (from the
 --kernel-trace --stats run), plus the force kernel's per-launch average
 (only the instantiations whose name matches <regex>, when given: e.g.
-"force_kernel<\d+, 1," for the BF16-gather ones).
+"force_kernel1?<\d+, 1," for the BF16-gather ones).
 pmc: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the force kernel,
 scaled by what the same counters read for a cold 1 GiB device-to-device
 copy (copyprobe), and the resulting HBM bytes per force launch
