@@ -295,10 +295,13 @@ fe_status fe_engine_run(fe_engine *e, int rounds, int *running);
 fe_status fe_engine_run_graph(fe_engine *e);
 fe_status fe_engine_graph_timing(const fe_engine *e, double *prepare_ms,
                                  double *exec_ms, int64_t *rounds);
-/* *done = 1 once the job's last minibatch has finished on the device.     */
+/* *done = 1 once the job's last minibatch has finished on the device (or a
+ * round holding the job failed to enqueue: collect then reports it).      */
 fe_status fe_engine_job_done(fe_engine *e, int job_id, int *done);
 /* Copies a finished job's X (n x d) to host and returns its memory to the
- * budget. Waits only for that job's last round.                           */
+ * budget. Waits only for that job's last round. A job whose round failed in
+ * fe_engine_run or fe_engine_run_graph has no result: FE_ERR_STATE, and its
+ * memory returns to the budget.                                           */
 fe_status fe_engine_collect(fe_engine *e, int job_id, float *x_out);
 /* Launch shape for later rounds and chunking for later submissions.        */
 fe_status fe_engine_set_tuning(fe_engine *e, const fe_tuning *t);
