@@ -115,16 +115,22 @@ __device__ __forceinline__ void store8_lds(float *p, const float (&o)[8]) {
 typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(1))) int gint;
 
-__device__ __forceinline__ void store8_coherent(float *p, const float (&o)[8]) {
-  gfloat *g = (gfloat *)p;
+// A heavy-row partial (D = 8G floats) in component-major order: lane q of
+// the row's group keeps its component k at base[k*G + q], so each of the 8
+// device-coherent 4-byte accesses of the group covers one contiguous span
+// of 4G bytes. (The row layout, 8 consecutive floats per lane, would make
+// every such access write 4 bytes out of each 32 and pass the whole row to
+// memory 8 times over.)
+__device__ __forceinline__ void store8_coherent(float *base, int q, int G, const float (&o)[8]) {
+  gfloat *g = (gfloat *)base + q;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) __hip_atomic_store(g + k, o[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int k = 0; k < 8; ++k) __hip_atomic_store(g + k * G, o[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void load8_coherent(const float *p, float (&o)[8]) {
-  gfloat *g = (gfloat *)p;
+__device__ __forceinline__ void load8_coherent(const float *base, int q, int G, float (&o)[8]) {
+  gfloat *g = (gfloat *)base + q;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) o[k] = __hip_atomic_load(g + k, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int k = 0; k < 8; ++k) o[k] = __hip_atomic_load(g + k * G, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // s_waitcnt vmcnt(0): this wave's stores have completed.
@@ -418,7 +424,7 @@ __device__ __forceinline__ void sum_partials(const StepDesc &sd, int p0, int k, 
     for (int uu = 0; uu < U; ++uu) {
       const int idx = (u0 + uu) * NG + g;
       ok[uu] = idx < k;
-      if (ok[uu]) load8_coherent(sd.P + (size_t)(p0 + idx) * D + 8 * q, pv[uu]);
+      if (ok[uu]) load8_coherent(sd.P + (size_t)(p0 + idx) * D, q, D >> 3, pv[uu]);
     }
 #pragma unroll
     for (int uu = 0; uu < U; ++uu) {
@@ -736,7 +742,7 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
       bool finish = pslot < 0;
       if (!finish) {
         const int sid = (int)pre1.x, p0 = (int)pre1.y, kparts = (int)pre1.z;
-        if (g == 0) store8_coherent(sd.P + (size_t)pslot * D + 8 * q, acc);
+        if (g == 0) store8_coherent(sd.P + (size_t)pslot * D, q, G, acc);
         wait_stores();
         int old = 0;
         if (lane == 0)
@@ -972,7 +978,7 @@ __global__ __launch_bounds__(512) void spmm_kernel(const StepDesc *__restrict__ 
       bool finish = pslot < 0;
       if (!finish) {
         const int sid = (int)it1.x, p0 = (int)it1.y, kparts = (int)it1.z;
-        if (g == 0) store8_coherent(sd.P + (size_t)pslot * D + 8 * q, acc);
+        if (g == 0) store8_coherent(sd.P + (size_t)pslot * D, q, G, acc);
         wait_stores();
         int old = 0;
         if (lane == 0)
