@@ -268,8 +268,12 @@ class ForceEmbed:
             t = fe_tuning(int(ch), int(mc), int(u), int(w), int(i), -1 if int(li) == 0 else int(li))
             self._check(self._lib.fe_set_batch_tuning(self._ctx, b, C.byref(t)))
 
-    def set_timing(self, enable: bool) -> None:
-        self._check(self._lib.fe_set_timing(self._ctx, 1 if enable else 0))
+    def set_timing(self, enable) -> None:
+        """True or 1: an event pair around every force launch; 2: one event
+        pair around each train call (divided by its force launches); False
+        or 0: off."""
+        mode = int(enable) if not isinstance(enable, bool) else (1 if enable else 0)
+        self._check(self._lib.fe_set_timing(self._ctx, mode))
 
     def kernel_times(self):
         f, x, a, n = C.c_double(), C.c_double(), C.c_double(), C.c_int64()

@@ -18,8 +18,11 @@ struct fe_ctx {
   bool graph_valid = false;
   bool graph_timed = false;
   uint64_t graph_gen = 0; // descriptor generation the graph was captured with
-  // force-kernel timing with HIP events on the context's stream
+  // force-kernel timing with HIP events on the context's stream: an event
+  // pair per force launch (timing), or one around each fe_train call (region)
   bool timing = false;
+  bool region = false;
+  hipEvent_t region_ev[2] = {nullptr, nullptr};
   std::vector<hipEvent_t> events; // 2 per timed force launch
   size_t events_used = 0;
   double t_force = 0;
@@ -103,6 +106,8 @@ extern "C" void fe_destroy(fe_ctx *c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   drop_graph(c);
   for (hipEvent_t ev : c->events) hipEventDestroy(ev);
+  for (hipEvent_t ev : c->region_ev)
+    if (ev) hipEventDestroy(ev);
   job_free(c->job);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -336,6 +341,13 @@ extern "C" fe_status fe_train(fe_ctx *c, int iterations, int batch_size, int ns,
   // the tick at the start of every epoch brings the counter to 0, 1, ...
   if ((s = from_job(c, job_set_epoch(j, -1, c->stream))) != FE_OK)
     return s;
+  const bool region = c->region && !c->timing;
+  if (region) {
+    for (hipEvent_t &ev : c->region_ev)
+      if (!ev && (s = hip_ok(c, hipEventCreate(&ev), "region event")) != FE_OK) return s;
+    if ((s = hip_ok(c, hipEventRecord(c->region_ev[0], c->stream), "region event")) != FE_OK)
+      return s;
+  }
   int done = 0;
   if (use_graph) {
     if (!c->graph_valid || c->graph_timed != c->timing || c->graph_gen != j.desc_gen) {
@@ -367,7 +379,17 @@ extern "C" fe_status fe_train(fe_ctx *c, int iterations, int batch_size, int ns,
   // every epoch moved each batch down one slot
   job_advance_layout(j, done);
   if (s != FE_OK) return s;
+  if (region && (s = hip_ok(c, hipEventRecord(c->region_ev[1], c->stream), "region event")) != FE_OK)
+    return s;
   if ((s = hip_ok(c, hipStreamSynchronize(c->stream), "fe_train sync")) != FE_OK) return s;
+  if (region) {
+    float ms = 0.f;
+    if ((s = hip_ok(c, hipEventElapsedTime(&ms, c->region_ev[0], c->region_ev[1]), "region time")) !=
+        FE_OK)
+      return s;
+    c->t_force += ms;
+    c->n_timed += (int64_t)done * batches;
+  }
   if (c->timing && !use_graph) return harvest_events(c);
   return FE_OK;
 }
@@ -451,7 +473,8 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
   hipEventCreate(&e0);
   hipEventCreate(&e1);
   s = hip_ok(c, hipMemcpyAsync(backup, j.X, xbytes, hipMemcpyDeviceToDevice, c->stream), "backup");
-  const bool was_timing = c->timing;
+  const bool was_timing = c->timing, was_region = c->region;
+  c->region = false; // trials keep their own timing
   int n = 0;
   auto restore = [&]() -> fe_status {
     fe_status r = hip_ok(c, hipMemcpyAsync(j.X, backup, xbytes, hipMemcpyDeviceToDevice, c->stream),
@@ -567,6 +590,7 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
   if (s == FE_OK) s = time_tiling(c, epochs, false, &best_t, nullptr, e0, e1);
   if (s == FE_OK) s = restore();
   c->timing = was_timing;
+  c->region = was_region;
   drop_graph(c);
   if (s != FE_OK) {
     job_set_tuning(j, orig);
@@ -664,8 +688,10 @@ extern "C" fe_status fe_epoch_units(fe_ctx *c, int ns, int64_t *edges, int64_t *
 
 extern "C" fe_status fe_set_timing(fe_ctx *c, int enable) {
   CTX_GUARD(c);
+  if (enable < 0 || enable > 2) return fail(c, FE_ERR_INVALID_ARG, "fe_set_timing: enable must be 0, 1 or 2");
   hipStreamSynchronize(c->stream);
-  c->timing = enable != 0;
+  c->timing = enable == 1;
+  c->region = enable == 2;
   c->t_force = 0;
   c->n_timed = 0;
   std::fill(c->t_batch.begin(), c->t_batch.end(), 0.0);

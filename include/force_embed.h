@@ -159,9 +159,12 @@ fe_status fe_loss(fe_ctx *ctx, uint64_t eval_seed, int eval_ns,
  * entries, (row, negative) pairs in the first epoch and in later epochs.   */
 fe_status fe_epoch_units(fe_ctx *ctx, int ns, int64_t *edges,
                          int64_t *negs_epoch0, int64_t *negs_later);
-/* Average duration (ms) of the force kernel over the launches timed since
- * the last reset, measured with HIP events on the context's stream.
- * Timing is enabled with fe_set_timing(ctx, 1).                            */
+/* Force-kernel timing with HIP events on the context's stream, reset by
+ * every call. enable 1: an event pair around every force launch (the
+ * per-launch view; the extra event nodes add about 2 us per launch).
+ * enable 2: one event pair around each fe_train call, divided by the force
+ * launches it ran (the epoch ticks, one short launch per epoch, included).
+ * enable 0: off. fe_kernel_times reports the average.                      */
 fe_status fe_set_timing(fe_ctx *ctx, int enable);
 
 /* Autotuner (no counterpart in the reference). Trials time `epochs` epochs of

@@ -205,6 +205,36 @@ def test_graph_replay_equals_stream_launches_and_steps():
     np.testing.assert_array_equal(res[0], res[2])
 
 
+def test_kernel_timing_modes():
+    """Region timing (one event pair per train call) counts every force
+    launch of the call and leaves the results unchanged; the per-launch
+    pairs time the same launches."""
+    g = api.Graph.sbm(1500, 6, 0.02, 0.001, seed=4)
+    X = api.init_uniform(g.n, 64, seed=8)
+    B, T = 200, 4
+    lists = api.reference_negatives(g.n, T + 7, 5)[0]
+    res, times = [], []
+    for mode in (0, 2, 1):
+        ctx = _ctx(g, 64, X, B, lists)
+        ctx.set_timing(mode)
+        ctx.train(T, B, 5, 0.02, use_graph=True)
+        kt = ctx.kernel_times()
+        res.append(ctx.download_x())
+        times.append(kt)
+        ctx.close()
+    np.testing.assert_array_equal(res[0], res[1])
+    np.testing.assert_array_equal(res[0], res[2])
+    assert times[0]["launches"] == 0
+    assert times[1]["launches"] == T * 8 and times[2]["launches"] == T * 8
+    assert times[1]["force_ms"] > 0 and times[2]["force_ms"] > 0
+    with pytest.raises(api.FeError):
+        ctx = _ctx(g, 64, X, B, lists)
+        try:
+            ctx.set_timing(3)
+        finally:
+            ctx.close()
+
+
 def test_training_tracks_oracle():
     g = api.Graph.sbm(1000, 10, 18 / 99, 2 / 900, seed=0)
     X = api.init_uniform(g.n, 128, seed=0)
