@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -282,44 +283,65 @@ extern "C" fe_status fe_gen_sbm(int64_t n, int blocks, double p_in, double p_out
   for (int i = 0; i < blocks; ++i)
     bstart[i + 1] = bstart[i] + n / blocks + (i < n % blocks ? 1 : 0);
   const int64_t npairs = (int64_t)blocks * (blocks + 1) / 2;
+  // the expected edge count must fit the CSR (fewer than 2^31 entries)
+  double expect = 0.0;
+  for (int i = 0; i < blocks; ++i) {
+    const double si = (double)(bstart[i + 1] - bstart[i]);
+    expect += p_in * si * (si - 1) / 2 + p_out * si * (double)(n - bstart[i + 1]);
+  }
+  if (2 * expect >= 2147483647.0) {
+    set_host_error("fe_gen_sbm: the expected edge count exceeds the CSR limit of 2^31-1 entries");
+    return FE_ERR_INVALID_ARG;
+  }
   std::vector<std::vector<int64_t>> eu(npairs), ev(npairs);
-#pragma omp parallel for schedule(dynamic, 1)
+  int oom = 0; // an exception cannot leave the parallel region
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : oom)
   for (int64_t pidx = 0; pidx < npairs; ++pidx) {
-    // decode pidx -> (bi, bj), bi <= bj
-    int64_t bi = 0, rem = pidx;
-    while (rem >= blocks - bi) {
-      rem -= blocks - bi;
-      ++bi;
-    }
-    const int64_t bj = bi + rem;
-    const double p = (bi == bj) ? p_in : p_out;
-    if (p <= 0.0) continue;
-    const int64_t si = bstart[bi + 1] - bstart[bi];
-    const int64_t sj = bstart[bj + 1] - bstart[bj];
-    const int64_t cand = si * sj; // (i, j) grid; within a block only i < j is kept
-    uint64_t st = splitmix64_mix(seed ^ 0x5B3C0DEull) ^ splitmix64_mix((uint64_t)bi * 1000003ull + (uint64_t)bj + 1);
-    const double lq = (p < 1.0) ? std::log1p(-p) : 0.0;
-    int64_t idx = -1;
-    while (true) {
-      if (p >= 1.0) {
-        idx += 1;
-      } else {
-        const double uni = ((splitmix64_next(st) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
-        const double skip = std::floor(std::log(uni) / lq);
-        if (skip > (double)cand) break;
-        idx += 1 + (int64_t)skip;
+    try {
+      // decode pidx -> (bi, bj), bi <= bj
+      int64_t bi = 0, rem = pidx;
+      while (rem >= blocks - bi) {
+        rem -= blocks - bi;
+        ++bi;
       }
-      if (idx >= cand) break;
-      const int64_t i = idx / sj, j = idx % sj;
-      if (bi == bj && i >= j) continue;
-      eu[pidx].push_back(bstart[bi] + i);
-      ev[pidx].push_back(bstart[bj] + j);
+      const int64_t bj = bi + rem;
+      const double p = (bi == bj) ? p_in : p_out;
+      if (p <= 0.0) continue;
+      const int64_t si = bstart[bi + 1] - bstart[bi];
+      const int64_t sj = bstart[bj + 1] - bstart[bj];
+      const int64_t cand = si * sj; // (i, j) grid; within a block only i < j is kept
+      uint64_t st = splitmix64_mix(seed ^ 0x5B3C0DEull) ^ splitmix64_mix((uint64_t)bi * 1000003ull + (uint64_t)bj + 1);
+      const double lq = (p < 1.0) ? std::log1p(-p) : 0.0;
+      int64_t idx = -1;
+      while (true) {
+        if (p >= 1.0) {
+          idx += 1;
+        } else {
+          const double uni = ((splitmix64_next(st) >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+          const double skip = std::floor(std::log(uni) / lq);
+          if (skip > (double)cand) break;
+          idx += 1 + (int64_t)skip;
+        }
+        if (idx >= cand) break;
+        const int64_t i = idx / sj, j = idx % sj;
+        if (bi == bj && i >= j) continue;
+        eu[pidx].push_back(bstart[bi] + i);
+        ev[pidx].push_back(bstart[bj] + j);
+      }
+    } catch (...) {
+      oom = 1;
     }
   }
   std::vector<int64_t> u, v;
-  for (int64_t pidx = 0; pidx < npairs; ++pidx) {
-    u.insert(u.end(), eu[pidx].begin(), eu[pidx].end());
-    v.insert(v.end(), ev[pidx].begin(), ev[pidx].end());
+  try {
+    if (oom) throw std::bad_alloc();
+    for (int64_t pidx = 0; pidx < npairs; ++pidx) {
+      u.insert(u.end(), eu[pidx].begin(), eu[pidx].end());
+      v.insert(v.end(), ev[pidx].begin(), ev[pidx].end());
+    }
+  } catch (...) {
+    set_host_error("fe_gen_sbm: out of host memory");
+    return FE_ERR_OOM;
   }
   return build_csr(n, u.data(), v.data(), (int64_t)u.size(), true, true, out);
 }
@@ -388,26 +410,36 @@ extern "C" fe_status fe_read_mtx(const char *path, fe_graph *out) {
     }
     break;
   }
-  if (rows <= 0 || rows != cols || nnz < 0) {
+  if (rows <= 0 || rows != cols || nnz < 0 || rows >= 2147483647LL) {
     fclose(f);
-    set_host_error("fe_read_mtx: the matrix must be square with a valid size line");
+    set_host_error("fe_read_mtx: the matrix must be square, with 1 to 2^31-2 rows and a valid "
+                   "size line");
     return FE_ERR_INVALID_ARG;
   }
   std::vector<int64_t> u, v;
-  u.reserve((size_t)nnz);
-  v.reserve((size_t)nnz);
   long long read = 0;
-  while (read < nnz && fgets(line, sizeof line, f)) {
-    if (line[0] == '%' || line[0] == '\n' || line[0] == '\r') continue;
-    long long i, j;
-    if (sscanf(line, "%lld %lld", &i, &j) != 2 || i < 1 || j < 1 || i > rows || j > cols) {
-      fclose(f);
-      set_host_error("fe_read_mtx: bad entry line");
-      return FE_ERR_INVALID_ARG;
+  try {
+    // the announced count is only a hint: a bad header must not reserve
+    // more than the entries that follow
+    const size_t hint = (size_t)std::min<long long>(nnz, 1LL << 24);
+    u.reserve(hint);
+    v.reserve(hint);
+    while (read < nnz && fgets(line, sizeof line, f)) {
+      if (line[0] == '%' || line[0] == '\n' || line[0] == '\r') continue;
+      long long i, j;
+      if (sscanf(line, "%lld %lld", &i, &j) != 2 || i < 1 || j < 1 || i > rows || j > cols) {
+        fclose(f);
+        set_host_error("fe_read_mtx: bad entry line");
+        return FE_ERR_INVALID_ARG;
+      }
+      u.push_back(i - 1);
+      v.push_back(j - 1);
+      ++read;
     }
-    u.push_back(i - 1);
-    v.push_back(j - 1);
-    ++read;
+  } catch (...) {
+    fclose(f);
+    set_host_error("fe_read_mtx: out of host memory");
+    return FE_ERR_OOM;
   }
   fclose(f);
   if (read != nnz) {

@@ -114,6 +114,34 @@ def test_mtx_ingest(tmp_path):
         api.Graph.read_mtx(bad)
 
 
+def test_mtx_bad_headers_are_errors_not_crashes(tmp_path):
+    # a header announcing far more entries than follow reserves nothing huge
+    big = tmp_path / "big.mtx"
+    big.write_text("%%MatrixMarket matrix coordinate pattern general\n"
+                   "3 3 1000000000000000\n1 2\n")
+    with pytest.raises(api.FeError, match="fewer entries"):
+        api.Graph.read_mtx(big)
+    # vertex ids must fit the int32 column ids
+    wide = tmp_path / "wide.mtx"
+    wide.write_text("%%MatrixMarket matrix coordinate pattern general\n"
+                    "3000000000 3000000000 1\n1 2\n")
+    with pytest.raises(api.FeError):
+        api.Graph.read_mtx(wide)
+    for text in ("%%MatrixMarket matrix array real general\n2 2\n1\n2\n3\n4\n",
+                 "%%MatrixMarket matrix coordinate real skew-symmetric\n2 2 1\n2 1 1\n",
+                 "%%MatrixMarket matrix coordinate pattern general\n2 2 1\n3 1\n",
+                 "%%MatrixMarket matrix coordinate pattern general\n2 2\n"):
+        f = tmp_path / "x.mtx"
+        f.write_text(text)
+        with pytest.raises(api.FeError):
+            api.Graph.read_mtx(f)
+
+
+def test_sbm_rejects_graphs_beyond_the_csr_limit():
+    with pytest.raises(api.FeError, match="expected edge count"):
+        api.Graph.sbm(200_000, 1, 1.0, 0.0, seed=0)
+
+
 def test_embedding_writer_format(tmp_path):
     x = np.array([[0.5, -1.25], [1e-6, 3.0]], dtype=np.float32)
     p = tmp_path / "emb.txt"
