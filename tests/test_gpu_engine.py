@@ -104,6 +104,28 @@ def test_admission_by_budget_and_reuse():
         assert err.value.status == 4
 
 
+def test_bad_job_is_rejected_and_engine_stays_usable():
+    g = api.Graph.sbm(1200, 4, 0.02, 0.001, seed=3)
+    x0 = api.init_uniform(g.n, 32, seed=1)
+    rp = g.row_ptr.copy()
+    rp[5], rp[6] = rp[6], rp[5] + 1  # not monotone
+    bad = api.Graph(g.n, rp, g.col_idx)
+    with api.Engine(1 * GiB) as e:
+        with pytest.raises(api.FeError) as err:
+            e.submit(bad, x0, 256, 2)
+        assert err.value.status == 1
+        assert e.stats()["bytes_in_use"] == 0
+        jid = e.submit(g, x0, 256, 2)
+        while e.run(8) > 0:
+            pass
+        x = e.collect(jid)
+    with api.Engine(1 * GiB) as e:
+        ref = e.submit(g, x0, 256, 2)
+        while e.run(8) > 0:
+            pass
+        np.testing.assert_array_equal(x, e.collect(ref))
+
+
 def test_jobs_admitted_while_others_run():
     specs = _jobs()
     with api.Engine(2 * GiB) as e:
