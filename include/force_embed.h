@@ -294,7 +294,9 @@ fe_status fe_engine_submit(fe_engine *e, const fe_job_desc *desc, int *job_id);
 fe_status fe_engine_run(fe_engine *e, int rounds, int *running);
 /* Batching form: runs every submitted job to completion from a schedule
  * built up front, its rounds captured in HIP graphs and replayed back to
- * back. Timing of the last call: fe_engine_graph_timing.                   */
+ * back. Timing of the last call: fe_engine_graph_timing. The schedule
+ * (about 232 bytes per job per round) is a device allocation of its own,
+ * outside the budget, freed before the call returns.                      */
 fe_status fe_engine_run_graph(fe_engine *e);
 fe_status fe_engine_graph_timing(const fe_engine *e, double *prepare_ms,
                                  double *exec_ms, int64_t *rounds);
