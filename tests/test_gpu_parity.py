@@ -86,7 +86,7 @@ def test_sbm_batches(d):
     _batch_parity(g, d, 256, [(0, 0), (0, 3), (0, 7), (1, 0), (1, 7), (4, 2)])
 
 
-@pytest.mark.parametrize("d", [8, 16, 256])
+@pytest.mark.parametrize("d", [8, 16, 256, 512])
 def test_other_widths(d):
     g = api.Graph.sbm(700, 7, 0.05, 0.004, seed=3)
     _batch_parity(g, d, 128, [(0, 0), (2, 5)], ns=5 if d <= 128 else 3)
@@ -464,6 +464,10 @@ def test_bf16_batches_match_oracle_at_rounded_rows():
                        tuning=dict(chunk=32, max_chunks=64, light=0, unroll=1))
     g = api.Graph.sbm(3000, 6, 0.01, 0.001, seed=5)
     _bf16_batch_parity(g, 64, 512, [(0, 0), (3, 5)])
+    # widths without a compile-time lane count (G = 32, 64 lanes per row)
+    g = api.Graph.sbm(700, 7, 0.05, 0.004, seed=3)
+    _bf16_batch_parity(g, 256, 128, [(0, 0), (2, 5)])
+    _bf16_batch_parity(g, 512, 128, [(1, 3)])
 
 
 def test_bf16_training_tracks_bf16_oracle():

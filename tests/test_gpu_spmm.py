@@ -40,7 +40,7 @@ def _spmm(g, X, B, lr, gather=api.FP32, quirks=3, tuning=None, iterations=1):
         return ctx.spmm(iterations, B, lr)
 
 
-@pytest.mark.parametrize("d", [128, 64, 32, 256])
+@pytest.mark.parametrize("d", [128, 64, 32, 256, 512])
 def test_spmm_matches_oracle_sbm(d):
     g = api.Graph.sbm(6000, 12, 0.01, 0.0004, seed=5)
     X = api.init_uniform(g.n, d, -0.5, 0.5, seed=2)
