@@ -62,7 +62,7 @@ def traffic_from_profile(name: str):
 
 def cpu_baseline(g, lists, d, B, ns, lr, budget_s: float, max_epochs: int):
     sys.path.insert(0, str(ROOT / "tests"))
-    import _oracle as orc  # the checker, timed as the CPU baseline
+    import _oracle as orc  # the oracle, timed as the CPU baseline
     from distembed_amd import api
     threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
     X = api.init_uniform(g.n, d, -0.5, 0.5, seed=0).astype(np.float64)
@@ -122,7 +122,7 @@ def spmm_form(api, args) -> dict:
     res["bf16_speedup"] = res["fp32"]["ms_per_product"] / res["bf16"]["ms_per_product"]
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "tests"))
-        import _oracle as orc  # the checker, timed as the CPU baseline
+        import _oracle as orc  # the oracle, timed as the CPU baseline
         threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or (os.cpu_count() or 1)
         X64 = X.astype(np.float64)
         t1 = time.perf_counter()
@@ -225,7 +225,7 @@ def main() -> int:
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path "
+        print("bench.py runs on one GPU only: the library has no multi-GPU path "
               "(the halo exchange across GPUs is a design item); use --gpus 1", file=sys.stderr)
         return 2
 

@@ -1,7 +1,8 @@
 // ORACLE -- test infrastructure only. CPU restatement of the reference's
 // world-size-1 force-embedding path, used by tests/, by
-// __graft_entry__.smoke() and by bench.py's cpu_baseline leg as the checker
-// and as the CPU baseline. Never linked into, or called by, the product.
+// __graft_entry__.smoke() and by bench.py's cpu_baseline leg, to check the
+// GPU results and as the CPU baseline. Never linked into, or called by, the
+// product.
 //
 // Follows (all paths relative to the reference root):
 //   cpp/algo/algo.hpp:74-192     driver loop: batches, last batch, seeds i+j,

@@ -1,7 +1,7 @@
 """ctypes wrapper of the oracle (oracle/liboracle.so) -- test infrastructure.
 
 Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it,
-as the checker; the product never does.
+to check the GPU results; the product never does.
 """
 from __future__ import annotations
 
