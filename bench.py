@@ -20,7 +20,9 @@ around every launch is reported beside it; those event nodes add about
 2 us per launch. traffic: HBM bytes per launch from the
 committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when present.
 bf16_form: the same workload with BF16 gathers (FP32 master and
-accumulation), its speed-up and its final loss against the FP32 run.
+accumulation), its speed-up and its final loss against the FP32 run, and
+against an FP32 run with the same tiling (the tiling alone moves the final
+loss of these chaotic runs; that spread is reported too).
 cpu_baseline: the oracle's FP64 restatement of the reference loop (OpenMP,
 schedule(static)) on a bounded sample of the same workload.
 
@@ -330,6 +332,15 @@ def main() -> int:
             "loss_rel_diff_vs_fp32": (b16["loss"] - f32["loss"]) / f32["loss"],
             "tuning": b16["tuning"],
         }
+        # R-MAT training at this rate is chaotic: the summation order alone
+        # (the tiling) moves the final loss. An FP32 run with the BF16 run's
+        # tiling gives the like-for-like comparison and the FP32 spread.
+        f32b = run_form(api, g, X0, lists, api.FP32, args, b16["tuning"])
+        out["bf16_form"]["loss_rel_diff_vs_fp32_same_tiling"] = (
+            (b16["loss"] - f32b["loss"]) / f32b["loss"])
+        out["bf16_form"]["fp32_loss_rel_diff_between_tilings"] = (
+            (f32b["loss"] - f32["loss"]) / f32["loss"])
+        log(f"fp32 with the bf16 tiling: loss {f32b['loss']:.6e}")
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, lists, d, B, ns, lr, args.cpu_budget,
                                            args.cpu_epochs)
