@@ -40,12 +40,25 @@
 #include <string>
 #include <vector>
 
+#include <unistd.h>
+
 namespace distembed_amd {
 
 namespace detail {
 inline void check(fe_status st, const fe_ctx *ctx, const char *what) {
   if (st != FE_OK)
     throw std::runtime_error(std::string(what) + ": " + fe_last_error(ctx));
+}
+
+// Resident set size of this process in MB (/proc/self/statm), 0 if unknown.
+inline double resident_mb() {
+  FILE *f = std::fopen("/proc/self/statm", "r");
+  if (!f) return 0.0;
+  long pages = 0, resident = 0;
+  const int got = std::fscanf(f, "%ld %ld", &pages, &resident);
+  std::fclose(f);
+  if (got != 2) return 0.0;
+  return (double)resident * (double)sysconf(_SC_PAGESIZE) / (1024.0 * 1024.0);
 }
 } // namespace detail
 
@@ -201,7 +214,12 @@ public:
   void set_quirks(uint32_t q) { quirks = q; }
   fe_ctx *context() { return ctx; }
   // "Total Time" (ms, as cpp/algo/algo.hpp:75,256-261 records it),
-  // "Train Time" (ms, device part), "Edges+Negatives/s".
+  // "Memory usage" (host resident set in MB at the end of the call; the
+  // reference averages it over batches, cpp/core/common.cpp:134-158),
+  // "Train Time" (ms, device part), "Edges+Negatives/s" and "Achieved GB/s"
+  // (the force step's algorithmic bytes: gathered rows and column ids, row
+  // pointers, own rows read and new rows written, the shadow rows with BF16
+  // gathers and the negative rows, per epoch, over Train Time).
   const std::map<std::string, double> &perf_stats() const { return stats; }
 
   // iterations epochs of batch_size-row minibatches with ns negatives per
@@ -244,8 +262,16 @@ public:
                              ? (double)edges * iterations + negs0 + (double)negs1 * (iterations - 1)
                              : 0.0;
     stats["Total Time"] = std::chrono::duration<double, std::milli>(t3 - t0).count();
+    stats["Memory usage"] = detail::resident_mb();
     stats["Train Time"] = train_ms;
     stats["Edges+Negatives/s"] = train_ms > 0 ? units / (train_ms * 1e-3) : 0.0;
+    const double s_g = gather_dtype == FE_GATHER_BF16 ? 2.0 : 4.0;
+    const double batches = (double)((n + batch_size - 1) / batch_size);
+    const double epoch_bytes = (double)edges * (d * s_g + 4) + (double)(n + 1) * 8 +
+                               2.0 * n * d * 4 + batches * ns * d * s_g +
+                               (gather_dtype == FE_GATHER_BF16 ? (double)n * d * 2 : 0.0);
+    stats["Achieved GB/s"] =
+        train_ms > 0 ? epoch_bytes * iterations / (train_ms * 1e-3) / 1e9 : 0.0;
   }
 };
 

@@ -1,6 +1,7 @@
 """The dist_embed driver (C++ mirror of the reference API over the C ABI) on
 the GPU: its embedding.txt against the oracle run on the same graph, X0 and
 negative lists."""
+import json
 import subprocess
 
 import numpy as np
@@ -46,6 +47,9 @@ def test_driver_sbm_matches_oracle(tmp_path):
     np.testing.assert_allclose(got, want, rtol=0, atol=5e-6 + 1e-4 * np.abs(want).max())
     perf = (tmp_path / "perf_output").read_text()
     assert '"algo": "Embedding"' in perf and '"Total Time"' in perf and perf.rstrip().endswith(",")
+    rec = json.loads(perf.rstrip().rstrip(","))
+    st = rec["perf_stats"]
+    assert st["Memory usage"] > 0 and st["Achieved GB/s"] > 0 and st["Edges+Negatives/s"] > 0
 
 
 def test_driver_mtx_input_equals_generated_graph(tmp_path):
