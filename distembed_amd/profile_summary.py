@@ -8,10 +8,10 @@ trace: per-kernel launch count, total and average duration (from the
 --kernel-trace --stats run), plus the force kernel's per-launch average
 (only the instantiations whose name matches <regex>, when given: e.g.
 "force_kernel1?<\d+, 1," for the BF16-gather ones).
-pmc: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the force kernel,
-scaled by what the same counters read for a cold 1 GiB device-to-device
-copy (copyprobe), and the resulting HBM bytes per force launch
-(hbm_bytes_per_launch).
+pmc: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the force kernel (or
+of the kernels <regex> selects, e.g. "spmm_kernel<\d+, 0,"), scaled by what
+the same counters read for a cold 1 GiB device-to-device copy (copyprobe),
+and the resulting HBM bytes per launch (hbm_bytes_per_launch).
 """
 from __future__ import annotations
 
@@ -79,7 +79,8 @@ def pmc(fetch_csv: str, write_csv: str, out: str, command: str, cal_fetch_csv: s
     tot_f = tot_w = 0.0
     tot_n = 0
     for k in sorted(fetch):
-        if "force_kernel" not in k or (match and not re.search(match, k)):
+        # the kernels the regex selects (default: every force-kernel instantiation)
+        if not (re.search(match, k) if match else "force_kernel" in k):
             continue
         f = statistics.mean(fetch[k].values())
         w = statistics.mean(write.get(k, {0: 0.0}).values())
@@ -92,8 +93,8 @@ def pmc(fetch_csv: str, write_csv: str, out: str, command: str, cal_fetch_csv: s
         lines.append(f"{k}: dispatches={n} fetch_KiB={f:.1f} write_KiB={w:.1f} "
                      f"read_bytes={fb:.4e} write_bytes={wb:.4e}")
     if tot_n:
-        # every instantiation of the force kernel (batches use different ones)
-        lines.append(f"all force launches: dispatches={tot_n}")
+        # every selected instantiation (the force step's batches use several)
+        lines.append(f"all selected launches: dispatches={tot_n}")
         lines.append(f"hbm_bytes_per_launch = {(tot_f + tot_w) / tot_n:.6e}")
         lines.append(f"hbm_read_bytes_per_launch = {tot_f / tot_n:.6e}")
         lines.append(f"hbm_write_bytes_per_launch = {tot_w / tot_n:.6e}")
