@@ -170,8 +170,16 @@ template <size_t DIM> int run(const Options &o, SpMat<double> *mat) {
     dense_mat->init_seed = o.init_seed;
     dense_mat->reinitialize();
   }
+  // the receiver matrix holds the transposed CSR (cpp/dist_embed.cpp:197-200);
+  // with -col_major 1 the step reads it (cpp/algo/algo.hpp:131-133)
+  SpMat<double> receiver(grid.get());
+  SpMat<double> *recv = mat;
+  if (o.col_major) {
+    mat->transpose_into(receiver);
+    recv = &receiver;
+  }
   auto embedding_algo = std::unique_ptr<EmbeddingAlgo<INDEX_TYPE, VALUE_TYPE, DIM>>(
-      new EmbeddingAlgo<INDEX_TYPE, VALUE_TYPE, DIM>(mat, mat, mat, dense_mat.get(), grid.get(),
+      new EmbeddingAlgo<INDEX_TYPE, VALUE_TYPE, DIM>(mat, recv, mat, dense_mat.get(), grid.get(),
                                                      o.alpha, o.beta, 5, -5, o.col_major,
                                                      o.sync_comm));
   embedding_algo->set_gather_dtype(o.gather);

@@ -424,7 +424,7 @@ static fe_status time_tiling(fe_ctx *c, int epochs, bool per_batch, double *ms_e
     *ms_epoch = sum;
   } else {
     float ms = 0.f;
-    hipEventElapsedTime(&ms, e0, e1);
+    if ((s = hip_ok(c, hipEventElapsedTime(&ms, e0, e1), "tiling time")) != FE_OK) return s;
     *ms_epoch = ms / epochs;
   }
   c->t_batch.clear();

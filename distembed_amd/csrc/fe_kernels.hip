@@ -19,11 +19,12 @@
 // flight per group, and the group accumulators are combined at the end with
 // a fixed xor tree. Heavy rows are split into chunks whose partial deltas
 // are summed in index order by whichever chunk finishes last (agent-scope
-// release/acquire around a per-row counter), so every result is
-// deterministic and independent of the launch shape. The batch's negative
-// rows are staged once per block in LDS. X holds one slot more than there
-// are batches: a fused launch writes its batch's new rows into the free slot,
-// which none of its threads reads, so one launch per minibatch suffices.
+// atomic partials, ordered against a per-row counter by publish_partial),
+// so every result is deterministic and independent of the launch shape.
+// The batch's negative rows are staged once per block in LDS. X holds one
+// slot more than there are batches: a fused launch writes its batch's new
+// rows into the free slot, which none of its threads reads, so one launch
+// per minibatch suffices.
 // No MFMA: the step is a sparse gather, not a dense contraction.
 #include <hip/hip_runtime.h>
 #include <algorithm>
@@ -110,8 +111,8 @@ __device__ __forceinline__ void store8_lds(float *p, const float (&o)[8]) {
 
 // Partials of heavy rows cross CUs (and XCDs, whose L2s are not coherent with
 // each other) inside one launch. They are written and read with agent-scope
-// relaxed atomics, which the hardware performs at the device-coherent level
-// (sc1), so no L2 writeback or invalidation is needed around them.
+// atomics (device-coherent sc1 accesses); publish_partial orders them
+// against the per-row counter.
 typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(1))) int gint;
 
@@ -133,8 +134,34 @@ __device__ __forceinline__ void load8_coherent(const float *base, int q, int G, 
   for (int k = 0; k < 8; ++k) o[k] = __hip_atomic_load(g + k * G, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// s_waitcnt vmcnt(0): this wave's stores have completed.
-__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+// Counts this wave's chunk partial of a heavy row (just stored by its lanes
+// with agent-scope atomic stores) and tells, in every lane, whether it was
+// the row's last; the last chunk then reads all partials with agent-scope
+// atomic loads. Agent-scope atomics are coherent device-wide by themselves
+// (one modification order per location across all XCDs); what the protocol
+// needs beyond that is order:
+//  * the partial before the count: s_waitcnt vmcnt(0) waits until every
+//    store of the wave has completed (on this target family vmcnt counts
+//    stores), which is the wait the compiler's own agent-scope release
+//    emits before the releasing access. Its other part, an L2 writeback,
+//    only publishes non-atomic data, which the consumer does not read; as a
+//    fence it made the force launch 3.3 times slower (123 us against 37.5 us
+//    per launch on R-MAT scale 20).
+//  * the count before the partials are read: the loads are issued only after
+//    the wave has the counter's old value (a data dependence through the
+//    shuffle and the branch); an acquire fence would add an L2 invalidation,
+//    again for non-atomic data only.
+// The "memory" clobbers keep the compiler from moving any memory access
+// across either point.
+__device__ __forceinline__ bool publish_partial(int *counter, int kparts, int lane) {
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+  int old = 0;
+  if (lane == 0)
+    old = __hip_atomic_fetch_add((gint *)counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  old = __shfl(old, 0);
+  asm volatile("" ::: "memory");
+  return old == kparts - 1;
+}
 
 // Two FP32 values to BF16, round to nearest even (one v_cvt_pk_bf16_f32;
 // subnormals are kept, a NaN stays a NaN); lo in bits 0-15.
@@ -743,13 +770,7 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
       if (!finish) {
         const int sid = (int)pre1.x, p0 = (int)pre1.y, kparts = (int)pre1.z;
         if (g == 0) store8_coherent(sd.P + (size_t)pslot * D, q, G, acc);
-        wait_stores();
-        int old = 0;
-        if (lane == 0)
-          old = __hip_atomic_fetch_add((gint *)(sd.counters + sid), 1, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-        old = __shfl(old, 0);
-        if (old == kparts - 1) {
+        if (publish_partial(sd.counters + sid, kparts, lane)) {
 #pragma unroll
           for (int k = 0; k < 8; ++k) acc[k] = 0.f;
           sum_partials<U>(sd, p0, kparts, NG, g, q, acc);
@@ -979,13 +1000,7 @@ __global__ __launch_bounds__(512) void spmm_kernel(const StepDesc *__restrict__ 
       if (!finish) {
         const int sid = (int)it1.x, p0 = (int)it1.y, kparts = (int)it1.z;
         if (g == 0) store8_coherent(sd.P + (size_t)pslot * D, q, G, acc);
-        wait_stores();
-        int old = 0;
-        if (lane == 0)
-          old = __hip_atomic_fetch_add((gint *)(sd.counters + sid), 1, __ATOMIC_RELAXED,
-                                       __HIP_MEMORY_SCOPE_AGENT);
-        old = __shfl(old, 0);
-        if (old == kparts - 1) {
+        if (publish_partial(sd.counters + sid, kparts, lane)) {
 #pragma unroll
           for (int k = 0; k < 8; ++k) acc[k] = 0.f;
           sum_partials<U>(sd, p0, kparts, NG, g, q, acc);

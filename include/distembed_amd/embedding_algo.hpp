@@ -17,8 +17,14 @@
 //
 // Differences a caller sees:
 //   * One GPU per process: a grid with more than one rank is rejected
-//     (std::runtime_error); alpha, beta, col_major and sync_comm only steer
-//     the multi-rank exchange and are accepted and ignored.
+//     (std::runtime_error); alpha, beta and sync_comm only steer the
+//     multi-rank exchange and are accepted and ignored. col_major selects
+//     the CSR the step reads, as cpp/algo/algo.hpp:131-133 does: the
+//     receiver's (built transposed, cpp/dist_embed.cpp:197-200) or the
+//     native one. At world size 1 the kernel is the row-major one either
+//     way (cpp/algo/algo.hpp:184-186 passes col_major = false), so the
+//     skip of column N-1 applies in both cases; for a symmetric graph the
+//     two CSRs are equal.
 //   * The arithmetic runs on the GPU in FP32 (VALUE_TYPE = double values are
 //     converted on upload and download); set_gather_dtype(FE_GATHER_BF16)
 //     selects the BF16-shadow form.
@@ -32,12 +38,14 @@
 
 #include <force_embed.h>
 
+#include <atomic>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
 #include <map>
 #include <stdexcept>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include <unistd.h>
@@ -60,6 +68,41 @@ inline double resident_mb() {
   if (got != 2) return 0.0;
   return (double)resident * (double)sysconf(_SC_PAGESIZE) / (1024.0 * 1024.0);
 }
+
+// "Memory usage" as the reference records it: the resident set in whole MB
+// (cpp/core/common.cpp:55-70), summed after every batch and divided by the
+// number of samples (cpp/algo/algo.hpp:256-259). The batches run on the
+// device inside one call here, so a helper thread takes the samples, one
+// per millisecond while the run lasts and one at its end.
+class ResidentSampler {
+public:
+  void start() {
+    run_.store(true);
+    th_ = std::thread([this] {
+      while (run_.load()) {
+        add();
+        std::this_thread::sleep_for(std::chrono::milliseconds(1));
+      }
+    });
+  }
+  double stop() {
+    if (th_.joinable()) {
+      run_.store(false);
+      th_.join();
+    }
+    add();
+    return (double)(sum_ / count_);
+  }
+
+private:
+  void add() {
+    sum_ += (uint64_t)resident_mb();
+    ++count_;
+  }
+  std::atomic<bool> run_{false};
+  std::thread th_;
+  uint64_t sum_ = 0, count_ = 0;
+};
 } // namespace detail
 
 namespace net {
@@ -104,6 +147,25 @@ public:
     gNNz = (uint64_t)rp[n];
     row_ptr.assign(rp, rp + n + 1);
     col_idx.assign(ci, ci + rp[n]);
+  }
+  // This matrix's CSR transposed into `out` (columns ascending per row), as
+  // the reference's receiver matrix holds it (SpMat(..., transpose = true),
+  // cpp/dist_embed.cpp:197-200; cpp/core/csr_local.hpp:60-66).
+  void transpose_into(SpMat &out) const {
+    const int64_t n = (int64_t)gRows;
+    out.gRows = out.gCols = gRows;
+    out.gNNz = gNNz;
+    out.batch_size = batch_size;
+    out.proc_row_width = proc_row_width;
+    out.proc_col_width = proc_col_width;
+    out.row_ptr.assign((size_t)n + 1, 0);
+    for (int32_t c : col_idx) ++out.row_ptr[(size_t)c + 1];
+    for (int64_t r = 0; r < n; ++r) out.row_ptr[r + 1] += out.row_ptr[r];
+    out.col_idx.assign(col_idx.size(), 0);
+    std::vector<int64_t> next(out.row_ptr.begin(), out.row_ptr.end() - 1);
+    for (int64_t r = 0; r < n; ++r) // rows ascending: each new row's columns come out sorted
+      for (int64_t e = row_ptr[r]; e < row_ptr[r + 1]; ++e)
+        out.col_idx[(size_t)next[col_idx[e]]++] = (int32_t)r;
   }
   // The CSR is built when the matrix is read (cpp/core/sparse_mat.hpp:405-415).
   void initialize_CSR_blocks(bool = true) {}
@@ -214,8 +276,8 @@ public:
   void set_quirks(uint32_t q) { quirks = q; }
   fe_ctx *context() { return ctx; }
   // "Total Time" (ms, as cpp/algo/algo.hpp:75,256-261 records it),
-  // "Memory usage" (host resident set in MB at the end of the call; the
-  // reference averages it over batches, cpp/core/common.cpp:134-158),
+  // "Memory usage" (mean host resident set in whole MB while the batches
+  // run, see detail::ResidentSampler),
   // "Train Time" (ms, device part), "Edges+Negatives/s" and "Achieved GB/s"
   // (the force step's algorithmic bytes: gathered rows and column ids, row
   // pointers, own rows read and new rows written, the shadow rows with BF16
@@ -233,14 +295,15 @@ public:
                                std::to_string(grid->world_size) + " ranks is not supported");
     const int64_t n = (int64_t)dense_local->rows;
     const int d = (int)embedding_dim;
-    if ((int64_t)sp_local_native->gRows != n || sp_local_native->row_ptr.size() != (size_t)n + 1)
+    // the CSR the reference's step reads (cpp/algo/algo.hpp:131-133)
+    const SpMat<VALUE_TYPE> *csr = col_major ? sp_local_receiver : sp_local_native;
+    if (!csr || (int64_t)csr->gRows != n || csr->row_ptr.size() != (size_t)n + 1)
       throw std::runtime_error("algo_force2_vec_ns: CSR rows differ from the embedding rows");
     if (!ctx) {
       detail::check(fe_create(&ctx, grid ? grid->device : 0, n, d, gather_dtype, quirks),
                     nullptr, "fe_create");
-      detail::check(fe_upload_csr(ctx, sp_local_native->row_ptr.data(),
-                                  sp_local_native->col_idx.data(),
-                                  (int64_t)sp_local_native->col_idx.size()),
+      detail::check(fe_upload_csr(ctx, csr->row_ptr.data(), csr->col_idx.data(),
+                                  (int64_t)csr->col_idx.size()),
                     ctx, "fe_upload_csr");
     }
     std::vector<float> x((size_t)n * d);
@@ -249,8 +312,12 @@ public:
     detail::check(fe_configure(ctx, batch_size, (float)lr, (float)MAX_BOUND), ctx,
                   "fe_configure");
     const auto t1 = std::chrono::steady_clock::now();
-    detail::check(fe_train(ctx, iterations, batch_size, ns, (float)lr, 1), ctx, "fe_train");
-    detail::check(fe_synchronize(ctx), ctx, "fe_synchronize");
+    detail::ResidentSampler mem;
+    mem.start();
+    fe_status st = fe_train(ctx, iterations, batch_size, ns, (float)lr, 1);
+    if (st == FE_OK) st = fe_synchronize(ctx);
+    const double mem_mb = mem.stop();
+    detail::check(st, ctx, "fe_train");
     const auto t2 = std::chrono::steady_clock::now();
     detail::check(fe_download_x(ctx, x.data()), ctx, "fe_download_x");
     for (size_t i = 0; i < x.size(); ++i) dense_local->nCoordinates[i] = (VALUE_TYPE)x[i];
@@ -262,7 +329,7 @@ public:
                              ? (double)edges * iterations + negs0 + (double)negs1 * (iterations - 1)
                              : 0.0;
     stats["Total Time"] = std::chrono::duration<double, std::milli>(t3 - t0).count();
-    stats["Memory usage"] = detail::resident_mb();
+    stats["Memory usage"] = mem_mb;
     stats["Train Time"] = train_ms;
     stats["Edges+Negatives/s"] = train_ms > 0 ? units / (train_ms * 1e-3) : 0.0;
     const double s_g = gather_dtype == FE_GATHER_BF16 ? 2.0 : 4.0;

@@ -87,3 +87,45 @@ def test_driver_spmm_runs_one_record_per_iteration(tmp_path):
     assert perf.count('"Local SpMM"') == 2 and perf.count('"Edges/s"') == 2
     assert '"batch_size": 900' in perf and perf.rstrip().endswith(",")
     assert not (tmp_path / "embedding.txt").exists()
+
+
+def test_driver_col_major_reads_the_transposed_csr(tmp_path):
+    """-col_major 1 reads the receiver's CSR, which the reference builds
+    transposed (cpp/algo/algo.hpp:131-133, cpp/dist_embed.cpp:197-200); at
+    world size 1 the kernel stays row-major (:184-186), so the skip of column
+    N-1 applies either way. A directed ("general") graph with edges into
+    N-1 tells the two CSRs apart."""
+    import types
+    n, d, B, T = 500, 32, 64, 3
+    rng = np.random.default_rng(4)
+    pairs = {(int(a), int(b)) for a, b in rng.integers(0, n, (3000, 2)) if a != b}
+    pairs |= {(k, n - 1) for k in range(0, n - 1, 7)}  # edges into the last vertex
+    pairs = sorted(pairs)
+    u = np.array([p[0] for p in pairs])
+    v = np.array([p[1] for p in pairs])
+    mtx = tmp_path / "g.mtx"
+    mtx.write_text("%%MatrixMarket matrix coordinate pattern general\n"
+                   f"{n} {n} {len(pairs)}\n" + "\n".join(f"{a + 1} {b + 1}" for a, b in pairs)
+                   + "\n")
+
+    def csr(rows, cols):
+        order = np.lexsort((cols, rows))
+        rp = np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))]).astype(np.int64)
+        return types.SimpleNamespace(n=n, row_ptr=rp, col_idx=cols[order].astype(np.int32))
+
+    native, transposed = csr(u, v), csr(v, u)
+    g_file = api.Graph.read_mtx(mtx)
+    assert np.array_equal(g_file.row_ptr, native.row_ptr)
+    assert np.array_equal(g_file.col_idx, native.col_idx)
+    batches = -(-n // B)
+    lists = api.reference_negatives(n, T + batches - 1, 5)[0]
+    for flag, g in (("0", native), ("1", transposed)):
+        got, _ = _run(tmp_path, "-input", str(mtx), "-dim", str(d), "-batch", str(B), "-iter",
+                      str(T), "-col_major", flag)
+        want = api.init_uniform(n, d, -1.0, 1.0, seed=0)
+        orc.train(g, want, B, T, lists, quirks=3)
+        np.testing.assert_allclose(got, want, rtol=0, atol=5e-6 + 1e-4 * np.abs(want).max(),
+                                   err_msg=f"-col_major {flag}")
+    other = api.init_uniform(n, d, -1.0, 1.0, seed=0)
+    orc.train(native, other, B, T, lists, quirks=3)
+    assert np.abs(got - other).max() > 1e-3  # the two CSRs give different embeddings

@@ -12,6 +12,7 @@ import time
 import numpy as np
 import pytest
 
+import _oracle as orc
 from distembed_amd import api
 
 pytestmark = pytest.mark.gpu
@@ -75,6 +76,40 @@ def test_engine_job_equals_single_graph_context():
         ctx.train(s["iterations"], s["batch_size"], 5, 0.02)
         xc = ctx.download_x()
     np.testing.assert_array_equal(xe, xc)
+
+
+def test_every_job_of_mixed_rounds_matches_context_and_oracle():
+    """The four job kinds (FP32 d=128, BF16 d=64, FP32 d=32, B=333 with the
+    multiply-high slot division) share rounds whose launches mix FP32 and
+    BF16 gathers and lane counts (the kernel's run-time dtype and width
+    form). Each result equals the single-graph context bit for bit and tracks
+    the oracle's run: FP32 within 1e-4 of max|X| everywhere and 1e-5
+    normwise plus twice the FP32 oracle's own drift from the FP64 oracle;
+    the BF16 form against the oracle's BF16 form within 1e-5 normwise and
+    1e-3 of max|X|."""
+    specs = _jobs()
+    with api.Engine(2 * GiB) as e:
+        ids = [_submit(e, s) for s in specs]
+        mixed = _run_all(e, ids)
+    for k, (s, xe) in enumerate(zip(specs, mixed)):
+        g, x0, B, T = s["g"], s["x0"], s["batch_size"], s["iterations"]
+        with api.ForceEmbed(g.n, x0.shape[1], gather=s["gather"]) as ctx:
+            ctx.upload_csr(g)
+            ctx.upload_x(x0)
+            ctx.train(T, B, 5, 0.02)
+            np.testing.assert_array_equal(xe, ctx.download_x(), err_msg=f"job {k}")
+        lists = api.reference_negatives(g.n, T + -(-g.n // B) - 1, 5)[0]
+        xr = x0.copy()
+        orc.train(g, xr, B, T, lists, bf16=(s["gather"] == api.BF16))
+        diff = xe.astype(np.float64) - xr
+        if s["gather"] == api.BF16:
+            assert np.linalg.norm(diff) <= 1e-5 * np.linalg.norm(xr), k
+            assert np.abs(diff).max() <= 1e-3 * np.abs(xr).max(), k
+        else:
+            x64 = x0.astype(np.float64)
+            orc.train(g, x64, B, T, lists)
+            assert np.abs(diff).max() <= 1e-4 * np.abs(xr).max(), k
+            assert np.linalg.norm(diff) <= 1e-5 * np.linalg.norm(xr) + 2 * np.linalg.norm(xr - x64), k
 
 
 def test_admission_by_budget_and_reuse():

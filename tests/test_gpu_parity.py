@@ -1,82 +1,19 @@
 """GPU parity of the HIP force step against the oracle (through the C ABI).
 
-Tolerance (FP32 path), per batch, on identical X and identical negative lists:
-  * normwise: ||delta_gpu - delta_32|| <= 1e-5 ||delta_32||;
-  * elementwise on the updated rows X' = X + delta:
-      |X'_gpu - X'_32| <= 1e-5 (|X'_32| + rms(X'_32)) + 2 |X'_32 - X'_64|,
-    where X'_32 / X'_64 come from the FP32 / FP64 oracle. The last term
-    admits the FP32 reference's own rounding error: it sums each component
-    serially in float (64 701 terms for the heaviest R-MAT-20 row), while the
-    GPU sums short interleaved runs and combines them in a fixed tree, so on
-    hub rows it sits closer to the FP64 result than the FP32 reference does.
-The GPU reduces |v|^2 with a DPP tree, so results agree to rounding, not
-bitwise. Every GPU result is bitwise reproducible.
+The per-batch bar and its reasons are stated in tests/_parity.py: normwise
+1e-5 on the delta, and elementwise 1e-5 relative plus twice the FP32
+oracle's own distance from the FP64 oracle.
 """
 import numpy as np
 import pytest
 
 import _oracle as orc
+from _parity import (RTOL, batch_parity as _batch_parity, bf16_batch_parity,
+                     check_rows as _check_rows, clamp_fractions, make_ctx as _ctx,
+                     rows_of as _rows)
 from distembed_amd import api
 
 pytestmark = pytest.mark.gpu
-
-RTOL = 1e-5
-
-
-def _ctx(g, d, X, B, lists, gather=api.FP32, quirks=3, lr=0.02, tuning=None):
-    ctx = api.ForceEmbed(g.n, d, gather=gather, quirks=quirks)
-    ctx.upload_csr(g)
-    ctx.upload_x(X)
-    if lists is not None:
-        ctx.set_negatives(lists)
-    if tuning:
-        ctx.set_tuning(**tuning)
-    ctx.configure(B, lr, 5.0)
-    return ctx
-
-
-def _rows(n, B, j):
-    return min((j + 1) * B, n) - j * B
-
-
-def _check_rows(new_gpu, new_ref, delta_gpu, delta_ref, new_ref64, what=""):
-    new_ref = new_ref.astype(np.float64)
-    scale = float(np.sqrt(np.mean(new_ref ** 2)))
-    err = np.abs(new_gpu.astype(np.float64) - new_ref)
-    bound = RTOL * (np.abs(new_ref) + scale) + 2.0 * np.abs(new_ref - new_ref64)
-    assert np.all(err <= bound), f"{what}: max excess {(err - bound).max():.3e}"
-    num = np.linalg.norm((delta_gpu - delta_ref).astype(np.float64))
-    den = np.linalg.norm(delta_ref.astype(np.float64))
-    assert num <= RTOL * den + 1e-30, f"{what}: normwise delta error {num / den:.3e}"
-
-
-def _batch_parity(g, d, B, steps, ns=5, quirks=3, tuning=None, seed=0, lr=0.02):
-    X = api.init_uniform(g.n, d, -0.5, 0.5, seed=seed)
-    batches = -(-g.n // B)
-    n_lists = max(e for e, _ in steps) + batches
-    lists = api.reference_negatives(g.n, n_lists, ns)[0] if ns else None
-    ctx = _ctx(g, d, X, B, lists, quirks=quirks, tuning=tuning, lr=lr)
-    try:
-        for (epoch, j) in steps:
-            rows = _rows(g.n, B, j)
-            dg = ctx.step_delta(epoch, j, rows)
-            dr = orc.step(g, X.copy(), B, epoch, j, lists, lr=lr, quirks=quirks, apply=False,
-                          want_delta=True)
-            X64 = X.astype(np.float64)
-            dr64 = orc.step(g, X64.copy(), B, epoch, j, lists, lr=lr, quirks=quirks,
-                            apply=False, want_delta=True, dtype=np.float64)
-            r0 = j * B
-            _check_rows(X[r0:r0 + rows] + dg, X[r0:r0 + rows] + dr, dg, dr,
-                        X64[r0:r0 + rows] + dr64, f"d={d} B={B} epoch={epoch} batch={j}")
-            # the applied step writes exactly X + delta into the batch rows
-            ctx.step(epoch, j)
-            Xg = ctx.download_x()
-            np.testing.assert_array_equal(Xg[r0:r0 + rows], X[r0:r0 + rows] + dg)
-            np.testing.assert_array_equal(np.delete(Xg, np.s_[r0:r0 + rows], 0),
-                                          np.delete(X, np.s_[r0:r0 + rows], 0))
-            ctx.upload_x(X)
-    finally:
-        ctx.close()
 
 
 @pytest.mark.parametrize("d", [128, 64, 32])
@@ -122,6 +59,112 @@ def test_isolated_and_empty_batches():
     n = 300
     g = api.Graph.from_edges(n, np.array([0, 5]), np.array([1, 6]))  # nearly all rows empty
     _batch_parity(g, 64, 64, [(0, 0), (0, 4), (2, 2)])
+
+
+# ---- scale() (cpp/algo/algo.hpp:65-72, applied at :575-579 and :642-645) -----------
+
+def _rep_rows(n, B, epoch, j, quirks=3):
+    """considering_batch_size at (epoch, batch) (cpp/algo/algo.hpp:135,167-169)."""
+    batches = -(-n // B)
+    last = n - B * (batches - 1)
+    return last if (j == batches - 1 or (epoch > 0 and quirks & 1)) else B
+
+
+def _big_components_x(n, d, seed):
+    """Rows of small components plus two large ones in [-1, 1]: distances
+    near 1, where the attractive |v_k d1| = 2|v_k|/(1+s) approaches its
+    maximum of 1 and the repulsive one exceeds it."""
+    rng = np.random.default_rng(seed)
+    X = rng.uniform(-0.01, 0.01, (n, d)).astype(np.float32)
+    k = rng.integers(0, d, (n, 2))
+    X[np.arange(n)[:, None], k] = rng.uniform(-1, 1, (n, 2)).astype(np.float32)
+    return X
+
+
+@pytest.mark.parametrize("bound", [0.5, 1.0])
+def test_clamp_binds_below_two(bound):
+    # a bound below 2 takes the exact attractive expressions (a bound >= 2
+    # can never bind: 2|v_k|/(1+s) <= 1); at 0.5 the attractive clamp binds,
+    # at 1.0 only the repulsive one can
+    g = api.Graph.sbm(2000, 10, 18 / 199, 2 / 1800, seed=0)
+    d, B = 64, 256
+    X = _big_components_x(g.n, d, seed=11)
+    lists = api.reference_negatives(g.n, 20, 5)[0]
+    steps = [(0, 0), (0, 3), (1, 7)]
+    for (e, j) in steps:
+        fa, fr = clamp_fractions(g, X, B, j, lists[e + j], _rep_rows(g.n, B, e, j), bound)
+        assert fr > 1e-3, (bound, e, j, fr)
+        if bound < 1.0:
+            assert fa > 0.01, (bound, e, j, fa)
+        else:
+            assert fa == 0.0
+    _batch_parity(g, d, B, steps, X=X, bound=bound, lists=lists)
+
+
+def test_clamp_binding_and_free_rows_in_one_wave():
+    # every third row sits in a tight cluster; each list holds two cluster
+    # rows, so the repulsive clamp binds for cluster rows (s ~ 1e-5) and not
+    # for the spread rows sharing their waves (med3 and single-FMA forms in
+    # one launch), at the reference's bound 5
+    g = api.Graph.sbm(3000, 12, 0.01, 0.0005, seed=6)
+    d, B = 128, 512
+    rng = np.random.default_rng(3)
+    X = rng.uniform(-0.5, 0.5, (g.n, d)).astype(np.float32)
+    cl = np.arange(0, g.n, 3)
+    X[cl] = rng.uniform(-2e-4, 2e-4, (cl.size, d)).astype(np.float32)
+    n_lists = 12
+    lists = np.stack([[cl[(7 * l) % cl.size], cl[(7 * l + 3) % cl.size], 1 + 3 * l, 2 + 3 * l,
+                       3 * l + 4] for l in range(n_lists)]).astype(np.int32)
+    steps = [(0, 0), (0, 4), (2, 5)]
+    for (e, j) in steps:
+        _, fr = clamp_fractions(g, X, B, j, lists[e + j], _rep_rows(g.n, B, e, j), 5.0)
+        assert 0.05 < fr < 0.95, (e, j, fr)
+    _batch_parity(g, d, B, steps, X=X, bound=5.0, lists=lists)
+    _batch_parity(g, d, B, steps, X=X, bound=5.0, lists=lists,
+                  tuning=dict(light=0, chunk=16, max_chunks=4))
+
+
+def test_non_finite_rows_follow_the_reference():
+    # scale() passes a NaN through; an infinite component makes s = inf, so
+    # d1 = -0 (attractive) or 0 (repulsive) and v_k * d1 is NaN where v_k is
+    # infinite and +-0 elsewhere; a row of 1e20 overflows s in FP32 while
+    # every component stays finite. The GPU must give the FP32 oracle's NaNs
+    # and infinities at the same places and meet the bar elsewhere.
+    g = api.Graph.sbm(1200, 6, 0.02, 0.001, seed=2)
+    d, B = 64, 256
+    X = api.init_uniform(g.n, d, seed=4)
+    nan_row, inf_row, big_row = 40, 300, 700
+    X[nan_row] = np.nan
+    X[inf_row, 5] = np.inf
+    X[big_row] = 1e20
+    lists = api.reference_negatives(g.n, 12, 5)[0]
+    lists[3, 1] = inf_row   # epoch 1, batch 2: every repelled row meets +inf
+    lists[4, 2] = big_row   # epoch 1, batch 3: s overflows for every repelled row
+    lists[lists == nan_row] = nan_row + 1
+    ctx = _ctx(g, d, X, B, lists)
+    X64 = X.astype(np.float64)
+    try:
+        for (e, j) in ((0, 0), (0, 1), (0, 2), (1, 2), (1, 3)):
+            rows = _rows(g.n, B, j)
+            r0 = j * B
+            dg = ctx.step_delta(e, j, rows)
+            dr = orc.step(g, X, B, e, j, lists, apply=False, want_delta=True)
+            dr64 = orc.step(g, X64, B, e, j, lists, apply=False, want_delta=True,
+                            dtype=np.float64)
+            ng, nr = X[r0:r0 + rows] + dg, X[r0:r0 + rows] + dr
+            assert np.array_equal(np.isnan(ng), np.isnan(nr)), (e, j)
+            assert np.array_equal(np.isposinf(ng), np.isposinf(nr)), (e, j)
+            assert np.array_equal(np.isneginf(ng), np.isneginf(nr)), (e, j)
+            fin = np.isfinite(nr)
+            n64 = X64[r0:r0 + rows] + dr64
+            ref, got = nr[fin].astype(np.float64), ng[fin].astype(np.float64)
+            scale = float(np.sqrt(np.mean(ref ** 2)))
+            bound = RTOL * (np.abs(ref) + scale) + 2.0 * np.abs(ref - n64[fin])
+            assert np.all(np.abs(got - ref) <= bound), (e, j)
+        # the cases occurred: NaN rows, a NaN component from the infinity
+        assert np.isnan(ctx.step_delta(1, 2, B)).any()
+    finally:
+        ctx.close()
 
 
 def test_tuning_does_not_change_results():
@@ -406,27 +449,6 @@ def test_bf16_final_loss_within_one_percent_of_fp32():
     assert abs(l16 - l32) <= 0.01 * abs(l32), (l16, l32)
 
 
-def _bf16_batch_parity(g, d, B, steps, tuning=None):
-    X = api.init_uniform(g.n, d, -0.5, 0.5, seed=1)
-    batches = -(-g.n // B)
-    lists = api.reference_negatives(g.n, max(e for e, _ in steps) + batches, 5)[0]
-    ctx = _ctx(g, d, X, B, lists, gather=api.BF16, tuning=tuning)
-    try:
-        Xr = orc.bf16_round(X)
-        for (epoch, j) in steps:
-            rows = _rows(g.n, B, j)
-            dg = ctx.step_delta(epoch, j, rows)
-            dr = orc.step(g, Xr.copy(), B, epoch, j, lists, apply=False, want_delta=True)
-            dr64 = orc.step(g, Xr.astype(np.float64), B, epoch, j, lists, apply=False,
-                            want_delta=True, dtype=np.float64)
-            r0 = j * B
-            X64 = X[r0:r0 + rows].astype(np.float64)
-            _check_rows(X[r0:r0 + rows] + dg, X[r0:r0 + rows] + dr, dg, dr, X64 + dr64,
-                        f"bf16 d={d} B={B} epoch={epoch} batch={j}")
-    finally:
-        ctx.close()
-
-
 def test_device_bf16_rounding_matches_oracle():
     """The shadow's rounding on the device (fe_round_bf16, the conversion
     every BF16 launch applies) equals the oracle's round-to-nearest-even bit
@@ -459,15 +481,15 @@ def test_bf16_batches_match_oracle_at_rounded_rows():
     # (every gather and the row's own distance), so on identical X they meet
     # the FP32 per-batch bound against the oracle run on the rounded matrix
     g = api.Graph.rmat(14, 16, seed=3)
-    _bf16_batch_parity(g, 128, 2048, [(0, 0), (0, 3), (2, 7)])
-    _bf16_batch_parity(g, 128, 2048, [(1, 0)],
+    bf16_batch_parity(g, 128, 2048, [(0, 0), (0, 3), (2, 7)])
+    bf16_batch_parity(g, 128, 2048, [(1, 0)],
                        tuning=dict(chunk=32, max_chunks=64, light=0, unroll=1))
     g = api.Graph.sbm(3000, 6, 0.01, 0.001, seed=5)
-    _bf16_batch_parity(g, 64, 512, [(0, 0), (3, 5)])
+    bf16_batch_parity(g, 64, 512, [(0, 0), (3, 5)])
     # widths without a compile-time lane count (G = 32, 64 lanes per row)
     g = api.Graph.sbm(700, 7, 0.05, 0.004, seed=3)
-    _bf16_batch_parity(g, 256, 128, [(0, 0), (2, 5)])
-    _bf16_batch_parity(g, 512, 128, [(1, 3)])
+    bf16_batch_parity(g, 256, 128, [(0, 0), (2, 5)])
+    bf16_batch_parity(g, 512, 128, [(1, 3)])
 
 
 def test_bf16_training_tracks_bf16_oracle():

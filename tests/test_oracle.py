@@ -313,6 +313,48 @@ def test_loss_closed_form():
     assert a3 == pytest.approx(math.log(2.0), rel=1e-15)  # entry 0 -> 1 skipped
 
 
+def test_repulsive_loss_at_a_self_pair():
+    # one vertex: every evaluation pair is the row with itself, s = 0, and
+    # each term is log(1/eps)/(1-eps) with eps = 1e-6, i.e. 6 ln(10)/(1-1e-6)
+    g = _graph_from_pairs(1, [])
+    X = np.array([[0.3, -0.2, 0.7]])
+    a, r = orc.loss(g, X, quirks=0, seed=5, eval_ns=4)
+    assert a == 0.0
+    assert r == pytest.approx(4 * 6 * math.log(10.0) / (1 - 1e-6), rel=1e-14)
+
+
+def test_repulsive_loss_gradient_is_the_repulsive_step():
+    # an independent route to the repulsive loss term: its gradient by
+    # central differences of the oracle's loss must equal the repulsive
+    # update of the step (itself pinned against torch autograd above), with
+    # lr = 1 and no clamp. A pair (r, m) moves x_r by the step of row r
+    # against negative m and pulls x_m by the opposite amount.
+    n, d, seed, eval_ns = 7, 4, 3, 2
+    g = _graph_from_pairs(n, [])  # no edges: the loss is its repulsive part
+    rng = np.random.default_rng(1)
+    X = rng.uniform(-0.6, 0.6, (n, d))
+    pairs = [(r, orc.eval_negative(seed, r, k, n)) for r in range(n) for k in range(eval_ns)]
+    want = np.zeros_like(X)
+    for (r, m) in pairs:
+        lists = np.zeros((n, 1), dtype=np.int32)
+        lists[r, 0] = m
+        delta = orc.step(g, X.copy(), 1, 0, r, lists, lr=1.0, bound=1e30, quirks=0,
+                         apply=False, want_delta=True, dtype=np.float64)[0]
+        want[r] -= delta  # dL/dx_r = -(step of row r)
+        want[m] += delta  # dL/dx_m = +(step of row r)
+    h = 1e-6
+    got = np.zeros_like(X)
+    for r in range(n):
+        for k in range(d):
+            xp, xm = X.copy(), X.copy()
+            xp[r, k] += h
+            xm[r, k] -= h
+            got[r, k] = (orc.loss(g, xp, quirks=0, seed=seed, eval_ns=eval_ns)[1]
+                         - orc.loss(g, xm, quirks=0, seed=seed, eval_ns=eval_ns)[1]) / (2 * h)
+    assert np.abs(want).max() > 0.1
+    np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6 * np.abs(want).max())
+
+
 def test_eval_negative_same_in_library_and_oracle():
     for (seed, r, k, n) in [(0, 0, 0, 10), (5, 123456, 3, 1_048_576), (2**63, 7, 9, 4_194_304)]:
         assert api.eval_negative(seed, r, k, n) == orc.eval_negative(seed, r, k, n)
