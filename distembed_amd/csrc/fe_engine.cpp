@@ -371,11 +371,10 @@ extern "C" fe_status fe_engine_submit(fe_engine *e, const fe_job_desc *d, int *j
   Job &j = ej->job;
   j.allocator = e->arena.get();
   j.tun = tun;
-  s = job_init(j, d->n, d->d, d->gather_dtype, d->quirks);
+  // X at its full slot size from the start: no second copy while growing
+  s = job_init(j, d->n, d->d, d->gather_dtype, d->quirks, d->batch_size);
   j.pending_plan = std::move(sizing.plan);
   j.pending_B = d->batch_size;
-  // X at its full slot size from the start: no second copy while growing
-  if (s == FE_OK) s = job_reserve_slots(j, d->batch_size);
   if (s == FE_OK) s = job_upload_csr(j, d->row_ptr, d->col_idx, d->nnz, e->upload);
   if (s == FE_OK) s = job_upload_x(j, d->x0, e->upload);
   if (s == FE_OK) s = job_configure(j, d->batch_size, d->lr, d->max_bound);

@@ -137,19 +137,6 @@ fe_status job_normalize_layout(Job &j, hipStream_t st) {
   return FE_OK;
 }
 
-fe_status job_reserve_slots(Job &j, int B) {
-  if (B <= 0) {
-    j.err = "batch size must be positive";
-    return FE_ERR_INVALID_ARG;
-  }
-  const int64_t batches = (j.n + B - 1) / B;
-  const size_t need = (size_t)(batches + 1) * B * j.d;
-  fe_status s = ensure_cap(j, j.X, j.cap_X, need, "X slots");
-  if (s == FE_OK && j.dtype == FE_GATHER_BF16) s = ensure_cap(j, j.Xs, j.cap_Xs, need, "shadow slots");
-  if (s == FE_OK) j.x_ready = false;
-  return s;
-}
-
 void job_advance_layout(Job &j, int epochs) {
   if (j.lay.P <= 0 || epochs <= 0) return;
   j.rot = (int)(((int64_t)j.rot + epochs) % j.lay.P);
@@ -180,7 +167,7 @@ hipError_t record_event(hipEvent_t ev, hipStream_t st) {
   return hipEventRecord(ev, st);
 }
 
-fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks) {
+fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks, int slot_B) {
   if (n <= 0 || n >= 2147483647LL) {
     j.err = "n must be in [1, 2^31-1)";
     return FE_ERR_INVALID_ARG;
@@ -203,10 +190,14 @@ fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks) {
   j.dtype = dtype;
   j.quirks = quirks;
   j.c_limit = (quirks & FE_QUIRK_SKIP_LAST_COL) ? n - 1 : n;
-  fe_status s = ensure_cap(j, j.X, j.cap_X, (size_t)n * d, "X");
+  // one allocation at the final size: replacing a smaller block would leave
+  // a hole in an engine's first-fit arena, and a job sized to the budget
+  // would then never fit
+  const int64_t rows = slot_B > 0 ? ((n + slot_B - 1) / slot_B + 1) * (int64_t)slot_B : n;
+  fe_status s = ensure_cap(j, j.X, j.cap_X, (size_t)rows * d, "X");
   if (s != FE_OK) return s;
   if (dtype == FE_GATHER_BF16) {
-    s = ensure_cap(j, j.Xs, j.cap_Xs, (size_t)n * d, "BF16 shadow");
+    s = ensure_cap(j, j.Xs, j.cap_Xs, (size_t)rows * d, "BF16 shadow");
     if (s != FE_OK) return s;
   }
   return ensure_cap(j, j.epoch, j.cap_epoch, 4, "epoch state");

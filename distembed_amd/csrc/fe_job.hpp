@@ -143,7 +143,9 @@ struct Job {
 };
 
 // All return FE_OK or an error with job.err set. `st` is the stream used.
-fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks);
+// slot_B > 0: X (and the shadow) get every slot of the layout of batch size
+// slot_B at once, so a fused run later needs no second allocation.
+fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks, int slot_B = 0);
 void job_free(Job &j);
 fe_status job_upload_csr(Job &j, const int64_t *row_ptr, const int32_t *col, int64_t nnz,
                          hipStream_t st);
@@ -162,9 +164,6 @@ fe_status job_set_cuts(Job &j, const std::vector<Cut> &cuts);
 // shadow) must hold every slot of the layout (fused runs); otherwise only
 // the n rows of the layout at rest are needed.
 fe_status job_prepare(Job &j, hipStream_t st, bool slots = false);
-// Sizes X (and the shadow) for the slot layout of batch size B before any
-// rows are uploaded (their contents are not kept).
-fe_status job_reserve_slots(Job &j, int B);
 // Layout at rest after a fused run of `epochs` epochs.
 void job_advance_layout(Job &j, int epochs);
 // Moves the rows back to rotation 0 (row c at position c).

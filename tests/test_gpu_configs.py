@@ -200,7 +200,9 @@ def test_serving_poisson_jobs_equal_alone():
                             if ex.status != 4:  # only over-budget is retried
                                 raise
                             tries += 1
-                            assert tries < 200000, "admission never succeeded"
+                            if tries % 5000 == 0:
+                                print(f"serving: job {i} still waiting for memory", flush=True)
+                            assert tries < 30000, f"job {i} was never admitted"
                             time.sleep(0.001)
                     if tries:
                         waited.append(i)
@@ -224,7 +226,7 @@ def test_serving_poisson_jobs_equal_alone():
         ts, tc = threading.Thread(target=submitter), threading.Thread(target=collector)
         ts.start()
         tc.start()
-        deadline = time.time() + 300
+        deadline = time.time() + 120
         while not errors:
             assert time.time() < deadline, "serving loop did not finish"
             with lock:
