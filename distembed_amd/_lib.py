@@ -82,6 +82,7 @@ SIGNATURES = {
     "fe_get_tuning": (C.c_int, [_P, C.POINTER(fe_tuning)]),
     "fe_step": (C.c_int, [_P, C.c_int, C.c_int]),
     "fe_step_delta": (C.c_int, [_P, C.c_int, C.c_int, _FP]),
+    "fe_step_delta_noalloc": (C.c_int, [_P, C.c_int, C.c_int, C.c_void_p, C.c_void_p]),
     "fe_train": (C.c_int, [_P, C.c_int, C.c_int, C.c_int, C.c_float, C.c_int]),
     "fe_synchronize": (C.c_int, [_P]),
     "fe_loss": (C.c_int, [_P, C.c_uint64, C.c_int, _DP, _DP]),

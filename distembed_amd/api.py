@@ -209,6 +209,13 @@ class ForceEmbed:
         self._check(self._lib.fe_step_delta(self._ctx, epoch, batch, _ptr(out, C.c_float)))
         return out
 
+    def step_delta_noalloc(self, epoch: int, batch: int, delta_dev: int, stream: int = 0) -> None:
+        """fe_step_delta_noalloc: the batch's delta into device memory at
+        address delta_dev, enqueued on `stream` (0: the context's stream);
+        returns without waiting."""
+        self._check(self._lib.fe_step_delta_noalloc(self._ctx, epoch, batch, C.c_void_p(delta_dev),
+                                                    C.c_void_p(stream) if stream else None))
+
     def train(self, iterations: int, batch_size: int, ns: int, lr: float,
               use_graph: bool = True) -> None:
         self._check(self._lib.fe_train(self._ctx, iterations, batch_size, ns, lr,

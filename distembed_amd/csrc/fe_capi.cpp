@@ -30,6 +30,10 @@ struct fe_ctx {
   // per-batch force times of replayed epochs (event pair p = batch p)
   std::vector<double> t_batch;
   std::vector<int64_t> n_batch;
+  // this device's address of the static scratch array of
+  // fe_step_delta_noalloc (looked up at creation), or the lookup's failure
+  void *scratch = nullptr;
+  std::string scratch_err;
   std::string err;
 };
 
@@ -89,6 +93,15 @@ extern "C" fe_status fe_create(fe_ctx **out, int device, int64_t n, int d, int g
   fe_status s = set_device(c);
   if (s == FE_OK) s = hip_ok(c, hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking), "hipStreamCreate");
   if (s == FE_OK) s = from_job(c, job_init(c->job, n, d, gather_dtype, quirks));
+  if (s == FE_OK) {
+    // once per device (cached): the scratch array of fe_step_delta_noalloc
+    const hipError_t se = scratch_address(device, &c->scratch);
+    if (se != hipSuccess) {
+      (void)hipGetLastError();
+      c->scratch = nullptr;
+      c->scratch_err = std::string("scratch array address: ") + hipGetErrorString(se);
+    }
+  }
   if (s != FE_OK) {
     fe::set_host_error("fe_create: " + c->err);
     job_free(c->job);
@@ -274,6 +287,50 @@ extern "C" fe_status fe_step_delta(fe_ctx *c, int epoch, int batch, float *delta
                   "delta copy")) != FE_OK)
     return s;
   return hip_ok(c, hipStreamSynchronize(c->stream), "fe_step_delta sync");
+}
+
+extern "C" fe_status fe_step_delta_noalloc(fe_ctx *c, int epoch, int batch, float *delta_dev,
+                                           void *stream) {
+  CTX_GUARD(c);
+  if (!delta_dev) return fail(c, FE_ERR_INVALID_ARG, "fe_step_delta_noalloc: NULL delta_dev");
+  if (!c->scratch) return fail(c, FE_ERR_HIP, "fe_step_delta_noalloc: " + c->scratch_err);
+  Job &j = c->job;
+  // preparing the plan or the descriptors would allocate: a prepared
+  // context is required
+  if (j.nnz < 0 || !j.x_ready || j.B <= 0 || j.plan_dirty || j.steps_dirty || !j.edesc)
+    return fail(c, FE_ERR_STATE,
+                "fe_step_delta_noalloc: the context is not prepared (one fe_step, fe_step_delta "
+                "or fe_train call after the last change of the CSR, X, negative lists, batch size "
+                "or tuning prepares it)");
+  fe_status s = check_batch(c, epoch, batch);
+  if (s != FE_OK) return s;
+  const int64_t pbytes = (int64_t)j.plan.pslots[batch] * j.d * (int64_t)sizeof(float);
+  const int64_t nctr = j.plan.split_off[batch + 1] - j.plan.split_off[batch];
+  if (pbytes > (int64_t)kScratchPartialBytes || nctr > kScratchCounters)
+    return fail(c, FE_ERR_INVALID_ARG,
+                "fe_step_delta_noalloc: batch " + std::to_string(batch) + " needs " +
+                    std::to_string(pbytes) + " bytes of heavy-row partials and " +
+                    std::to_string(nctr) + " split counters; the scratch array holds " +
+                    std::to_string(kScratchPartialBytes) + " bytes and " +
+                    std::to_string(kScratchCounters) +
+                    " counters (use fe_step_delta, or a larger chunk / smaller max_chunks)");
+  hipStream_t st = stream ? static_cast<hipStream_t>(stream) : c->stream;
+  // the unfused delta descriptor, with the partials and counters in scratch
+  StepDesc sd = j.host_desc[(size_t)j.plan.batches * DESC_DELTA + batch];
+  sd.S = delta_dev;
+  sd.P = static_cast<float *>(c->scratch);
+  sd.counters = reinterpret_cast<int *>(static_cast<char *>(c->scratch) + kScratchPartialBytes);
+  // a previous call may have stopped between a counter's first and last
+  // chunk: every counter starts at zero
+  if ((s = hip_ok(c, launch_zero_ints(sd.counters, (int)nctr, st), "scratch counters")) != FE_OK)
+    return s;
+  if ((s = from_job(c, job_set_epoch(j, epoch, st))) != FE_OK) return s;
+  const Shape sh = job_shape(j, batch);
+  return hip_ok(c,
+                launch_force1(sd, j.force_blocks[batch], sh.waves_per_block, sh.unroll,
+                              force_lds_bytes(j.ns, j.d), j.dtype == FE_GATHER_BF16 ? 1 : 0, j.G,
+                              st),
+                "fe_step_delta_noalloc: force kernel launch");
 }
 
 // One epoch of launches on the context stream: the epoch tick, then one

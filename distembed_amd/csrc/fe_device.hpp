@@ -112,4 +112,12 @@ struct alignas(16) LossDesc {
 constexpr int kMaxLdsFloats = 8192; // staged negative rows: ns * D <= 8192
 constexpr int kMaxNs = 64;
 
+// Scratch array of the step that allocates nothing (fe_step_delta_noalloc):
+// kScratchPartialBytes of heavy-row partials, then kScratchCounters split
+// counters.
+constexpr size_t kScratchPartialBytes = (size_t)15 << 20;
+constexpr int kScratchCounters = 1 << 18;
+constexpr size_t kScratchBytes = kScratchPartialBytes + (size_t)kScratchCounters * sizeof(int);
+constexpr int kScratchDevices = 64;
+
 } // namespace fe

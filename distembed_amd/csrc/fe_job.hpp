@@ -36,6 +36,10 @@ hipError_t launch_loss_edges(const LossDesc &ld, double *block_out, int n_blocks
 hipError_t launch_loss_negs(const LossDesc &ld, double *block_out, int n_blocks,
                             int pairs_per_group, hipStream_t st);
 hipError_t launch_reduce(const double *in, int n, double *out, hipStream_t st);
+hipError_t launch_zero_ints(int *p, int n, hipStream_t st);
+// This device's address of the static scratch array (kScratchBytes), looked
+// up once per device; `device` must be the current device.
+hipError_t scratch_address(int device, void **out);
 
 // Defaults: the autotuner's choice for R-MAT scale 20 at d = 128 on MI355X.
 struct Tuning {

@@ -28,6 +28,7 @@
 // No MFMA: the step is a sparse gather, not a dense contraction.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <mutex>
 #include "fe_device.hpp"
 
 namespace fe {
@@ -1249,6 +1250,45 @@ __global__ __launch_bounds__(1024) void reduce_kernel(const double *__restrict__
   if (threadIdx.x == 0) *out = part[0];
 }
 
+// Zeroes n ints (a kernel, not a memset node, so the fill can sit inside a
+// captured HIP graph).
+__global__ void zero_ints_kernel(int *__restrict__ p, int n) {
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0;
+}
+
+// ---- scratch of the entry point that allocates nothing -----------------------
+// fe_step_delta_noalloc keeps a step's heavy-row partials and split counters
+// in this array instead of buffers of the context. Every device the library
+// runs on holds its own copy for the life of the process. Host code never
+// passes the array's name to a kernel: scratch_address() looks up this
+// device's address of it once (hipGetSymbolAddress) and every later use
+// goes through that address.
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kScratchDevices] = {};
+} // namespace
+
+hipError_t scratch_address(int device, void **out) {
+  *out = nullptr;
+  if (device < 0 || device >= kScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice; // the caller makes `device` current
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    if (!p) return hipErrorInvalidSymbol;
+    g_scratch_ptr[device] = p;
+  }
+  *out = g_scratch_ptr[device];
+  return hipSuccess;
+}
+
 // ---- launch wrappers -------------------------------------------------------
 
 // lds_bytes: staged negative rows and sub-range sums (force_lds_bytes);
@@ -1381,6 +1421,13 @@ hipError_t launch_loss_negs(const LossDesc &ld, double *block_out, int n_blocks,
 
 hipError_t launch_reduce(const double *in, int n, double *out, hipStream_t st) {
   hipLaunchKernelGGL(reduce_kernel, dim3(1), dim3(1024), 0, st, in, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_zero_ints(int *p, int n, hipStream_t st) {
+  if (n <= 0) return hipSuccess;
+  const int blocks = std::max(1, std::min((n + 255) / 256, 1024));
+  hipLaunchKernelGGL(zero_ints_kernel, dim3(blocks), dim3(256), 0, st, p, n);
   return hipGetLastError();
 }
 

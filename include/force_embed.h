@@ -126,6 +126,24 @@ fe_status fe_step(fe_ctx *ctx, int epoch, int batch);
 /* Same forces as fe_step but X is left unchanged: the rows x d delta of the
  * batch is copied to delta_out (host). rows = min(B, n - batch*B).        */
 fe_status fe_step_delta(fe_ctx *ctx, int epoch, int batch, float *delta_out);
+/* The step of fe_step_delta without any allocation. No counterpart in the
+ * reference: it is for callers that cannot allocate device memory -- code
+ * being captured into a HIP graph, tight serving loops, harnesses that forbid
+ * hipMalloc. The rows x d FP32 delta goes to device memory at delta_dev
+ * (rows = min(B, n - batch*B)); the launches are enqueued on `stream` (a
+ * hipStream_t; NULL: the context's stream) and the call returns without
+ * waiting for them. The heavy-row partials and split counters live in one
+ * static device array per GPU, held until the process ends: at most
+ * 15 MiB of partials (the batch's partial slots x d x 4 bytes) and 262144
+ * counters; a batch needing more gets FE_ERR_INVALID_ARG. Every call on a
+ * GPU shares that array, so calls whose launches overlap on one GPU (other
+ * contexts or streams included) race; callers order them. The context must
+ * be prepared: one fe_step, fe_step_delta or fe_train call after the last
+ * change of the CSR, X, negative lists, batch size or tuning prepares it;
+ * otherwise FE_ERR_STATE. The call launches three kernels (counter fill,
+ * epoch state, force) and nothing else.                                    */
+fe_status fe_step_delta_noalloc(fe_ctx *ctx, int epoch, int batch, float *delta_dev,
+                                void *stream);
 /* algo_force2_vec_ns(iterations, batch_size, ns, lr) (cpp/algo/algo.hpp:74).
  * Uses the lists set with fe_set_negatives when they cover
  * iterations + batches - 1 lists with `ns` ids each; otherwise generates the
