@@ -25,6 +25,15 @@ against an FP32 run with the same tiling (the tiling alone moves the final
 loss of these chaotic runs; that spread is reported too).
 cpu_baseline: the oracle's FP64 restatement of the reference loop (OpenMP,
 schedule(static)) on a bounded sample of the same workload.
+precision: R-MAT scale 20 and 22 at 5 and 10 negatives, FP32 and BF16
+(each autotuned): epoch times, BF16 speed-ups, final losses (also like for
+like, at the BF16 run's tiling) and per-step gradient errors of the BF16
+form against the FP32 kernel.
+serving: Poisson arrivals of R-MAT embedding jobs (2e4-2e7 edges, d from
+{32, 64, 128}, 50 epochs) at 4, 8 and 16 jobs/s under an 8 GiB budget:
+p50/p99 job latency (arrival to result on the host) and edges+negatives/s.
+batching: the 512-graph SBM pool (d=64, B=1024, 50 epochs) in captured
+rounds, FP32 and BF16, tiles from the library's engine autotuner.
 
 --form precision | batching | serving runs the other workloads instead
 (distembed_amd/bench_forms.py) and prints their results as one JSON line;
@@ -60,6 +69,15 @@ def traffic_from_profile(name: str):
         return None
     m = re.search(r"hbm_bytes_per_launch\s*[:=]\s*([0-9.eE+]+)", p.read_text())
     return float(m.group(1)) if m else None
+
+
+def load_tuning(arg: str):
+    """--tuning value: JSON text, @path of a JSON file, or empty (None)."""
+    if not arg:
+        return None
+    if arg.startswith("@"):
+        return json.loads(Path(arg[1:]).read_text())
+    return json.loads(arg)
 
 
 def cpu_baseline(g, lists, d, B, ns, lr, budget_s: float, max_epochs: int):
@@ -214,7 +232,11 @@ def main() -> int:
     ap.add_argument("--tuning", type=str, default="",
                     help='json {"tuning": {fe_tuning fields}, "per_batch": [[unroll, '
                          'waves_per_block, items_per_wave, chunk, max_chunks, light] per '
-                         'batch]} (as reported in config.tuning); skips the autotuner')
+                         'batch]} (as reported in config.tuning), or @file holding it; '
+                         'skips the autotuner')
+    ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-batching", action="store_true")
     ap.add_argument("--bf16-tuning", type=str, default="",
                     help="as --tuning, for the BF16 pass (skips its autotuner)")
     ap.add_argument("--form", choices=["plain", "precision", "batching", "serving", "spmm"],
@@ -260,7 +282,7 @@ def main() -> int:
     X0 = api.init_uniform(g.n, d, -0.5, 0.5, seed=0)
     epochs_needed = max(args.steps, args.warmup, args.cpu_epochs, 3)
     lists, hits = api.reference_negatives(g.n, epochs_needed + batches - 1, ns)
-    tuning = json.loads(args.tuning) if args.tuning else None
+    tuning = load_tuning(args.tuning)
 
     f32 = run_form(api, g, X0, lists, api.FP32, args, tuning)
     kt, kr = f32["kt"], f32["kt_region"]
@@ -313,8 +335,7 @@ def main() -> int:
         "loss_fp64_after_timed_epochs": f32["loss"],
     }
     if not args.no_bf16:
-        b16 = run_form(api, g, X0, lists, api.BF16, args,
-                       json.loads(args.bf16_tuning) if args.bf16_tuning else None)
+        b16 = run_form(api, g, X0, lists, api.BF16, args, load_tuning(args.bf16_tuning))
         kb = b16["kt_region"]
         log(f"bf16: {args.steps} epochs in {b16['dt']:.3f} s; force {kb['force_ms'] * 1e3:.1f} us "
             f"per launch over the timed region, {b16['kt']['force_ms'] * 1e3:.1f} us with an event "
@@ -344,6 +365,24 @@ def main() -> int:
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, lists, d, B, ns, lr, args.cpu_budget,
                                            args.cpu_epochs)
+    del g
+    from distembed_amd import bench_forms as bf
+    if not args.no_precision:
+        out["precision"] = bf.precision(args.steps, args.warmup)
+    if not args.no_serving:
+        sv = bf.serving((4.0, 8.0, 16.0), 48, 50, budget_gib=8.0)
+        out["serving"] = {k: sv[k] for k in ("epochs", "batch", "ns", "budget_gib")}
+        out["serving"]["levels"] = [{k: lv[k] for k in ("offered_qps", "jobs", "achieved_qps",
+                                                        "p50_ms", "p99_ms", "units_per_s")}
+                                    for lv in sv["levels"]]
+    if not args.no_batching:
+        bt = bf.batching(50, count=512, budget_gib=48.0, tune_every=1)
+        out["batching"] = {k: bt[k] for k in ("graphs", "vertices", "csr_entries", "d", "batch",
+                                              "ns", "epochs", "units", "tuned_on",
+                                              "bf16_speedup")}
+        for k in ("fp32", "bf16"):
+            out["batching"][k] = {f: bt[k][f] for f in ("units_per_s", "exec_s", "rounds",
+                                                       "tuning")}
     print(json.dumps(out), flush=True)
     return 0
 

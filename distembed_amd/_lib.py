@@ -119,6 +119,9 @@ SIGNATURES = {
     "fe_engine_collect": (C.c_int, [_P, C.c_int, _FP]),
     "fe_engine_set_tuning": (C.c_int, [_P, C.POINTER(fe_tuning)]),
     "fe_engine_stats": (C.c_int, [_P, _I64P, _I64P, _I64P]),
+    "fe_engine_autotune": (C.c_int, [_P, C.POINTER(fe_job_desc), C.c_int, C.c_int,
+                                     C.POINTER(fe_tuning), _DP, C.c_int, C.POINTER(C.c_int),
+                                     C.POINTER(fe_tuning), _DP]),
 }
 
 _lib = None

@@ -385,6 +385,29 @@ class Engine:
         t = _tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light)
         self._check(self._lib.fe_engine_set_tuning(self._e, C.byref(t)))
 
+    def autotune(self, jobs, epochs: int = 2):
+        """fe_engine_autotune over `jobs`, a list of (graph, x0, batch_size,
+        kwargs of submit): returns (best tuning dict, best ms, [(tuning dict,
+        ms), ...]) and keeps the best as the engine's tuning."""
+        descs, keep = [], []
+        for (g, x0, B, kw) in jobs:
+            d, k = self._desc(g, x0, B, epochs, **kw)
+            descs.append(d)
+            keep.append(k)
+        arr = (_lib.fe_job_desc * len(descs))(*descs)
+        cap = 64
+        res = (fe_tuning * cap)()
+        ms = (C.c_double * cap)()
+        n = C.c_int(0)
+        best = fe_tuning()
+        best_ms = C.c_double(0)
+        self._check(self._lib.fe_engine_autotune(self._e, arr, len(descs), epochs, res, ms, cap,
+                                                 C.byref(n), C.byref(best), C.byref(best_ms)))
+        del keep
+        as_dict = lambda t: {k: getattr(t, k) for k, _ in fe_tuning._fields_}
+        return as_dict(best), best_ms.value, [(as_dict(res[i]), ms[i])
+                                              for i in range(min(n.value, cap))]
+
     def stats(self):
         b, r, u = C.c_int64(), C.c_int64(), C.c_int64()
         self._check(self._lib.fe_engine_stats(self._e, C.byref(b), C.byref(r), C.byref(u)))

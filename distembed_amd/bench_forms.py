@@ -1,13 +1,15 @@
 """Workloads of the precision, batching and serving forms (bench.py --form).
 
 precision: R-MAT scale 20 and 22, d=128, FP32 and BF16-shadow gathers,
-  5 and 10 negatives: epoch time, BF16 speed-up and final loss of each.
+  5 and 10 negatives: epoch time, BF16 speed-up, final loss of each (also
+  against an FP32 run with the BF16 run's tiling) and the per-step gradient
+  error of the BF16 form against the FP32 kernel.
 batching: 512 independent SBM graphs (n log-uniform in [1e3, 2e5], 4-32
   blocks, average degree uniform in [8, 40] with 90 % of it inside the
   block), d=64, B=1024, 5 negatives, 50 epochs, FP32 and BF16; all graphs
   resident, advanced together round by round, the rounds captured in HIP
-  graphs (fe_engine_run_graph), tiles chosen by an empirical search on the
-  mix itself (tune_engine).
+  graphs (fe_engine_run_graph), tiles chosen on the mix itself by the
+  library's engine autotuner (fe_engine_autotune).
 serving: Poisson arrivals of R-MAT jobs (edge count log-uniform in
   [2e4, 2e7], d from {32, 64, 128}), 50 epochs, 5 negatives, admitted under
   an HBM budget and interleaved round by round (fe_engine_run); submission
@@ -46,46 +48,107 @@ def _units(g: api.Graph, B: int, ns: int, epochs: int, quirks: int = 3) -> int:
 
 # ---- precision form --------------------------------------------------------
 
-def precision(steps: int, warmup: int) -> dict:
+def _train_timed(g, X0, lists, gather, B, ns, lr, steps, warmup, tuning=None):
+    """Autotune (unless a tuning is given), warm up, time `steps` epochs from
+    X0; returns ms/epoch, units/s, the FP64 loss at the end and the tuning."""
+    with api.ForceEmbed(g.n, X0.shape[1], gather=gather) as ctx:
+        ctx.upload_csr(g)
+        ctx.upload_x(X0)
+        ctx.set_negatives(lists)
+        ctx.configure(B, lr, 5.0)
+        if tuning is None:
+            ctx.autotune(epochs=2, search_chunking=True)
+        else:
+            ctx.set_tuning(**tuning["tuning"])
+            ctx.set_batch_tunings(tuning["per_batch"])
+        tun = {"tuning": ctx.get_tuning(),
+               "per_batch": [[t[k] for k in ("unroll", "waves_per_block", "items_per_wave",
+                                              "chunk", "max_chunks", "light")]
+                             for t in ctx.batch_tunings()]}
+        if warmup:
+            ctx.upload_x(X0)
+            ctx.train(warmup, B, ns, lr)
+        ctx.upload_x(X0)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        ctx.train(steps, B, ns, lr)
+        ctx.synchronize()
+        dt = time.perf_counter() - t1
+        a, r = ctx.loss(0, 5)
+        e, n0, n1 = ctx.epoch_units(ns)
+    units = steps * e + n0 + (steps - 1) * n1
+    return {"ms_per_epoch": dt / steps * 1e3, "units_per_s": units / dt, "loss": a + r,
+            "tuning": tun}
+
+
+def _gradient_error(g, X0, lists, B, ns, lr, batches):
+    """||delta_bf16 - delta_fp32|| / ||delta_fp32|| of single steps on the
+    same X and lists (the BF16 form against the plain FP32 kernel)."""
     out = []
-    for scale in (20, 22):
-        t0 = time.perf_counter()
-        g = api.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=0)
-        log(f"precision: R-MAT {scale}: n={g.n} nnz={g.nnz} ({time.perf_counter() - t0:.1f} s)")
-        d, B, lr = 128, 16384, 0.02
+    ctxs = []
+    try:
+        for gather in (api.FP32, api.BF16):
+            ctx = api.ForceEmbed(g.n, X0.shape[1], gather=gather)
+            ctxs.append(ctx)
+            ctx.upload_csr(g)
+            ctx.upload_x(X0)
+            ctx.set_negatives(lists)
+            ctx.configure(B, lr, 5.0)
+        for (e, j) in batches:
+            rows = min((j + 1) * B, g.n) - j * B
+            d32 = ctxs[0].step_delta(e, j, rows).astype(np.float64)
+            d16 = ctxs[1].step_delta(e, j, rows).astype(np.float64)
+            out.append({"epoch": e, "batch": j,
+                        "rel_err": float(np.linalg.norm(d16 - d32) / np.linalg.norm(d32))})
+    finally:
+        for ctx in ctxs:
+            ctx.close()
+    return out
+
+
+def precision(steps: int, warmup: int, cases=((20, 5), (20, 10), (22, 5), (22, 10))) -> dict:
+    """R-MAT at each (scale, ns) of `cases`, d=128, B=16384: FP32 and BF16
+    epoch times (each autotuned), the BF16 speed-up, the final FP64 loss of
+    both and of an FP32 run with the BF16 run's tiling (the tiling alone,
+    i.e. the summation order, moves the final loss of these chaotic runs),
+    and the per-step gradient error of the BF16 form against the FP32 kernel
+    on the first, second and last batch."""
+    out = []
+    graphs = {}
+    d, B, lr = 128, 16384, 0.02
+    for scale, ns in cases:
+        if scale not in graphs:
+            graphs.clear()
+            t0 = time.perf_counter()
+            graphs[scale] = api.Graph.rmat(scale, 16, 0.57, 0.19, 0.19, seed=0)
+            log(f"precision: R-MAT {scale}: n={graphs[scale].n} nnz={graphs[scale].nnz} "
+                f"({time.perf_counter() - t0:.1f} s)")
+        g = graphs[scale]
         X0 = api.init_uniform(g.n, d, -0.5, 0.5, seed=0)
         batches = -(-g.n // B)
-        for ns in (5, 10):
-            lists = api.reference_negatives(g.n, steps + warmup + batches, ns)[0]
-            row = {"graph": f"rmat{scale}", "n": g.n, "nnz": g.nnz, "ns": ns}
-            for name, gather in (("fp32", api.FP32), ("bf16", api.BF16)):
-                with api.ForceEmbed(g.n, d, gather=gather) as ctx:
-                    ctx.upload_csr(g)
-                    ctx.upload_x(X0)
-                    ctx.set_negatives(lists)
-                    ctx.configure(B, lr, 5.0)
-                    best, best_ms, _ = ctx.autotune(epochs=2, search_chunking=True)
-                    ctx.upload_x(X0)
-                    if warmup:
-                        ctx.train(warmup, B, ns, lr)
-                    ctx.upload_x(X0)
-                    ctx.synchronize()
-                    t1 = time.perf_counter()
-                    ctx.train(steps, B, ns, lr)
-                    ctx.synchronize()
-                    dt = time.perf_counter() - t1
-                    a, r = ctx.loss(0, 5)
-                    e, n0, n1 = ctx.epoch_units(ns)
-                units = steps * e + n0 + (steps - 1) * n1
-                row[name] = {"ms_per_epoch": dt / steps * 1e3, "units_per_s": units / dt,
-                             "loss": a + r, "tuning": best}
-                log(f"precision: rmat{scale} ns={ns} {name}: {dt / steps * 1e3:.3f} ms/epoch, "
-                    f"loss {a + r:.6e}")
-            row["bf16_speedup"] = row["fp32"]["ms_per_epoch"] / row["bf16"]["ms_per_epoch"]
-            row["bf16_loss_rel_diff"] = (row["bf16"]["loss"] - row["fp32"]["loss"]) / row["fp32"]["loss"]
-            out.append(row)
-        del g
-    return {"form": "precision", "epochs_timed": steps, "d": 128, "batch": 16384, "lr": 0.02,
+        lists = api.reference_negatives(g.n, steps + warmup + batches, ns)[0]
+        row = {"graph": f"rmat{scale}", "n": g.n, "nnz": g.nnz, "ns": ns}
+        row["fp32"] = _train_timed(g, X0, lists, api.FP32, B, ns, lr, steps, warmup)
+        row["bf16"] = _train_timed(g, X0, lists, api.BF16, B, ns, lr, steps, warmup)
+        same = _train_timed(g, X0, lists, api.FP32, B, ns, lr, steps, warmup,
+                            tuning=row["bf16"]["tuning"])
+        for k in ("fp32", "bf16"):
+            row[k].pop("tuning")
+        row["bf16_speedup"] = row["fp32"]["ms_per_epoch"] / row["bf16"]["ms_per_epoch"]
+        row["bf16_loss_rel_diff_vs_fp32"] = (row["bf16"]["loss"] - row["fp32"]["loss"]) / \
+            row["fp32"]["loss"]
+        row["bf16_loss_rel_diff_vs_fp32_same_tiling"] = (row["bf16"]["loss"] - same["loss"]) / \
+            same["loss"]
+        row["fp32_loss_rel_diff_between_tilings"] = (same["loss"] - row["fp32"]["loss"]) / \
+            row["fp32"]["loss"]
+        row["bf16_gradient_error_vs_fp32"] = _gradient_error(
+            g, X0, lists, B, ns, lr, [(0, 0), (0, 1), (0, batches - 1)])
+        log(f"precision: rmat{scale} ns={ns}: fp32 {row['fp32']['ms_per_epoch']:.3f} ms, bf16 "
+            f"{row['bf16']['ms_per_epoch']:.3f} ms ({row['bf16_speedup']:.3f}x), loss diff "
+            f"{row['bf16_loss_rel_diff_vs_fp32']:+.2%} ({row['bf16_loss_rel_diff_vs_fp32_same_tiling']:+.2%}"
+            f" same tiling)")
+        out.append(row)
+    return {"form": "precision", "epochs_timed": steps, "d": d, "batch": B, "lr": lr,
             "results": out}
 
 
@@ -109,29 +172,16 @@ def sbm_pool(count: int, seed: int = 0):
 
 
 def tune_engine(graphs, X0s, B: int, epochs: int, ns: int, gather: int, budget_gib: float):
-    """Empirical search of the engine's launch tiles on the mixed workload
-    itself: each candidate runs the given jobs through fe_engine_run_graph
-    and the fastest replay (HIP-event time) wins. A tile that suits one graph
-    alone can lose on rounds that mix hundreds of jobs of other sizes."""
-    cands = [dict(unroll=u, waves_per_block=w, items_per_wave=i, chunk=64, max_chunks=1024,
-                  light=li)
-             for u in (2, 4) for w in (4, 8) for i in (1, 2) for li in (8, 16)]
-    best, best_ms, table = None, float("inf"), []
-    for t in cands:
-        with api.Engine(int(budget_gib * 2 ** 30)) as eng:
-            eng.set_tuning(**t)
-            for g, x in zip(graphs, X0s):
-                eng.submit(g, x, B, epochs, ns=ns, gather=gather)
-            eng.run_graph()
-            ms = eng.graph_timing()["exec_ms"]
-        table.append((t, ms))
-        if ms < best_ms:
-            best, best_ms = t, ms
-    return best, best_ms, table
+    """The library's engine autotuner (fe_engine_autotune) on the given jobs:
+    each candidate tiling runs them through the captured-graph schedule and
+    the fastest replay wins."""
+    with api.Engine(int(budget_gib * 2 ** 30)) as eng:
+        return eng.autotune([(g, x, B, dict(ns=ns, gather=gather)) for g, x in zip(graphs, X0s)],
+                            epochs)
 
 
 def batching(epochs: int, count: int = 512, d: int = 64, B: int = 1024, ns: int = 5,
-             budget_gib: float = 96.0) -> dict:
+             budget_gib: float = 96.0, tune_every: int = 1) -> dict:
     t0 = time.perf_counter()
     graphs = sbm_pool(count)
     n_tot = sum(g.n for g in graphs)
@@ -140,13 +190,17 @@ def batching(epochs: int, count: int = 512, d: int = 64, B: int = 1024, ns: int 
         f"({time.perf_counter() - t0:.1f} s)")
     X0s = [api.init_uniform(g.n, d, -0.5, 0.5, seed=i) for i, g in enumerate(graphs)]
     units = sum(_units(g, B, ns, epochs) for g in graphs)
-    # tiles for this mix: searched on every 2nd graph for 2 epochs
+    # tiles for this mix: searched on every tune_every-th graph for 2 epochs (with
+    # every graph, the rounds of 2 epochs hold the same mix of active jobs as
+    # those of 50, in proportion: a job is active for epochs x its batches)
     res = {"form": "batching", "graphs": count, "vertices": n_tot, "csr_entries": nnz_tot,
-           "d": d, "batch": B, "ns": ns, "epochs": epochs, "units": units}
+           "d": d, "batch": B, "ns": ns, "epochs": epochs, "units": units,
+           "tuned_on": ("every graph" if tune_every == 1 else f"every {tune_every}th graph")
+           + ", 2 epochs"}
     for name, gather in (("fp32", api.FP32), ("bf16", api.BF16)):
         t0 = time.perf_counter()
-        best, best_ms, table = tune_engine(graphs[::2], X0s[::2], B, min(epochs, 2), ns, gather,
-                                           budget_gib)
+        best, best_ms, table = tune_engine(graphs[::tune_every], X0s[::tune_every], B,
+                                           min(epochs, 2), ns, gather, budget_gib)
         log(f"batching {name}: tiles {best} ({best_ms:.1f} ms on the sample; {len(table)} "
             f"candidates in {time.perf_counter() - t0:.1f} s)")
         with api.Engine(int(budget_gib * 2 ** 30)) as eng:

@@ -765,3 +765,109 @@ extern "C" fe_status fe_engine_graph_timing(const fe_engine *e, double *prepare_
   if (rounds) *rounds = e->graph_rounds;
   return FE_OK;
 }
+
+// Frees a job that has finished or failed without downloading its result.
+static void drop_job(fe_engine *e, int id) {
+  std::lock_guard<std::mutex> lk(e->mu);
+  if (id < 0 || id >= (int)e->jobs.size()) return;
+  EngineJob *ej = e->jobs[id].get();
+  if (ej->collected) return;
+  hipStreamSynchronize(e->stream);
+  hipStreamSynchronize(e->upload);
+  job_free(ej->job);
+  if (ej->uploaded) hipEventDestroy(ej->uploaded);
+  if (ej->done) hipEventDestroy(ej->done);
+  ej->uploaded = ej->done = nullptr;
+  ej->launched_all = true;
+  ej->collected = true;
+}
+
+// Makes finished jobs run all their epochs again from their current rows
+// (a later replay of the same work, for timing).
+static void rewind_jobs(fe_engine *e, const std::vector<int> &ids) {
+  std::lock_guard<std::mutex> lk(e->mu);
+  for (int id : ids) {
+    EngineJob *ej = e->jobs[id].get();
+    ej->epoch = 0;
+    ej->batch = 0;
+    ej->launched_all = ej->iterations == 0;
+  }
+}
+
+// Tiles of the batching form chosen by timing them on the workload itself:
+// a round mixes the batches of many jobs of different sizes, so a tiling
+// that suits one graph alone can lose on the mix. The jobs are submitted
+// once per cutting (light); every launch shape then replays all their
+// epochs through the captured-graph schedule from the rows the previous
+// replay left (the work items, and so the time, do not depend on the
+// values). The fastest replay (device time, HIP events) becomes the
+// engine's tuning.
+extern "C" fe_status fe_engine_autotune(fe_engine *e, const fe_job_desc *jobs, int n_jobs,
+                                        int epochs, fe_tuning *results, double *results_ms,
+                                        int max_results, int *n_results, fe_tuning *best,
+                                        double *best_ms) {
+  if (!e || !jobs || n_jobs <= 0 || epochs < 1)
+    return e ? efail(e, FE_ERR_INVALID_ARG, "fe_engine_autotune: invalid argument")
+             : FE_ERR_INVALID_ARG;
+  {
+    std::lock_guard<std::mutex> lk(e->mu);
+    for (auto &jp : e->jobs)
+      if (!jp->collected)
+        return efail(e, FE_ERR_STATE,
+                     "fe_engine_autotune: the engine holds jobs; collect them first");
+  }
+  const Tuning orig = engine_tuning(e);
+  std::vector<fe_job_desc> descs(jobs, jobs + n_jobs);
+  for (fe_job_desc &d : descs) d.iterations = epochs;
+  fe_status s = FE_OK;
+  double best_t = 1e300;
+  fe_tuning best_c{};
+  int n = 0;
+  std::vector<int> ids;
+  for (int light : {8, 16}) {
+    fe_tuning cut{64, 1024, orig.unroll, orig.waves_per_block, orig.items_per_wave, light};
+    if ((s = fe_engine_set_tuning(e, &cut)) != FE_OK) break;
+    ids.clear();
+    for (const fe_job_desc &d : descs) {
+      int id = -1;
+      if ((s = fe_engine_submit(e, &d, &id)) != FE_OK) break;
+      ids.push_back(id);
+    }
+    for (int u : {2, 4, 8}) {
+      for (int w : {4, 8}) {
+        for (int ipw : {1, 2}) {
+          if (s != FE_OK) break;
+          const fe_tuning c{64, 1024, u, w, ipw, light};
+          if ((s = fe_engine_set_tuning(e, &c)) != FE_OK) break;
+          rewind_jobs(e, ids);
+          if ((s = fe_engine_run_graph(e)) != FE_OK) break;
+          double ms = 0;
+          if ((s = fe_engine_graph_timing(e, nullptr, &ms, nullptr)) != FE_OK) break;
+          if (results && n < max_results) {
+            results[n] = c;
+            if (results_ms) results_ms[n] = ms;
+          }
+          ++n;
+          if (ms < best_t) {
+            best_t = ms;
+            best_c = c;
+          }
+        }
+      }
+    }
+    for (int id : ids) drop_job(e, id);
+    if (s != FE_OK) break;
+  }
+  if (s != FE_OK) {
+    std::string msg = g_engine_error;
+    std::lock_guard<std::mutex> lk(e->mu);
+    e->tun = orig;
+    g_engine_error = "fe_engine_autotune: " + msg;
+    return s;
+  }
+  if ((s = fe_engine_set_tuning(e, &best_c)) != FE_OK) return s;
+  if (n_results) *n_results = n;
+  if (best) *best = best_c;
+  if (best_ms) *best_ms = best_t;
+  return FE_OK;
+}

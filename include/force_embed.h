@@ -328,6 +328,19 @@ fe_status fe_engine_job_done(fe_engine *e, int job_id, int *done);
 fe_status fe_engine_collect(fe_engine *e, int job_id, float *x_out);
 /* Launch shape for later rounds and chunking for later submissions.        */
 fe_status fe_engine_set_tuning(fe_engine *e, const fe_tuning *t);
+/* Autotuner of the batching form's tiles (no counterpart in the reference):
+ * the n_jobs jobs (their iterations replaced by `epochs`) are submitted once
+ * per cutting -- light {8,16} at chunk 64, max_chunks 1024 -- and replayed
+ * through fe_engine_run_graph once per launch shape -- unroll {2,4,8} x
+ * waves per block {4,8} x items per wave {1,2}; the fastest replay (device
+ * time, HIP events) becomes the engine's tuning. The engine must hold no
+ * uncollected job; the trial jobs' results are discarded. Up to max_results
+ * candidates go to results / results_ms (replay ms); *best and *best_ms get
+ * the winner.                                                             */
+fe_status fe_engine_autotune(fe_engine *e, const fe_job_desc *jobs, int n_jobs,
+                             int epochs, fe_tuning *results, double *results_ms,
+                             int max_results, int *n_results, fe_tuning *best,
+                             double *best_ms);
 fe_status fe_engine_stats(const fe_engine *e, int64_t *bytes_in_use,
                           int64_t *rounds_run, int64_t *units_done);
 

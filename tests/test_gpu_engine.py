@@ -226,3 +226,29 @@ def test_submit_and_collect_on_other_threads():
     assert not errors, errors
     for k, a in enumerate(alone):
         np.testing.assert_array_equal(results[k], a)
+
+
+def test_engine_autotune_keeps_the_fastest_tiles():
+    """fe_engine_autotune times its 24 candidate tilings on the given jobs
+    through the captured-graph schedule, keeps the fastest as the engine's
+    tuning and leaves no job behind: a job run afterwards equals the same
+    job under an engine given that tuning explicitly."""
+    specs = _jobs()
+    with api.Engine(2 * GiB) as e:
+        best, best_ms, table = e.autotune(
+            [(s["g"], s["x0"], s["batch_size"], dict(ns=5, gather=s["gather"])) for s in specs],
+            epochs=1)
+        assert len(table) == 24
+        assert best_ms == min(ms for _, ms in table) and best_ms > 0
+        assert best in [t for t, _ in table]
+        assert e.stats()["bytes_in_use"] == 0
+        xa = _run_all(e, [_submit(e, specs[1])])[0]
+    with api.Engine(2 * GiB) as e:
+        e.set_tuning(**best)
+        xb = _run_all(e, [_submit(e, specs[1])])[0]
+    np.testing.assert_array_equal(xa, xb)
+    with api.Engine(2 * GiB) as e:
+        _submit(e, specs[2])
+        with pytest.raises(api.FeError) as err:  # jobs not collected yet
+            e.autotune([(specs[0]["g"], specs[0]["x0"], 512, {})], epochs=1)
+        assert err.value.status == 5
