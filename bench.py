@@ -6,8 +6,10 @@ duplicates and self-loops removed, natural vertex order), d=128, batch 16384,
 5 reference negatives per batch, lr 0.02 -- the one-GPU configuration the
 metric is quoted on. A step is one epoch: the 64 dependent minibatches of
 algo_force2_vec_ns over the whole graph, replayed as captured HIP graphs.
-Inputs are resident in HBM before the timed region starts; the launch
-tiling is chosen by the library's autotuner during warm-up.
+Inputs are resident in HBM before the timed region starts. The launch
+tiling is the per-batch tiling the library's autotuner (fe_autotune) chose
+for this workload, kept under profiles/ with the rocprofv3 summaries taken
+with it (config.tuning_source); --autotune searches afresh instead.
 
 value = (processed CSR entries + (row, negative) pairs) / second, FP32.
 roofline: the force kernel's algorithmic bytes per launch (its share of the
@@ -229,6 +231,8 @@ def main() -> int:
     ap.add_argument("--no-bf16", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--autotune", action="store_true",
+                    help="search the tiling with fe_autotune even where a kept tiling exists")
     ap.add_argument("--tuning", type=str, default="",
                     help='json {"tuning": {fe_tuning fields}, "per_batch": [[unroll, '
                          'waves_per_block, items_per_wave, chunk, max_chunks, light] per '
@@ -282,7 +286,18 @@ def main() -> int:
     X0 = api.init_uniform(g.n, d, -0.5, 0.5, seed=0)
     epochs_needed = max(args.steps, args.warmup, args.cpu_epochs, 3)
     lists, hits = api.reference_negatives(g.n, epochs_needed + batches - 1, ns)
+    # the autotuner's tiling for this workload, as profiled (profiles/)
+    kept = (args.scale, d, B, ns) == (20, 128, 16384, 5) and not args.autotune
+    kept32 = ROOT / "profiles" / "force_kernel_rmat20_d128_fp32_tuning.json"
+    kept16 = ROOT / "profiles" / "force_kernel_rmat20_d128_bf16_tuning.json"
+    if not args.tuning and kept and kept32.exists():
+        args.tuning = "@" + str(kept32)
+    if not args.bf16_tuning and kept and kept16.exists():
+        args.bf16_tuning = "@" + str(kept16)
     tuning = load_tuning(args.tuning)
+    def source(a):
+        return (str(Path(a[1:]).relative_to(ROOT)) if a.startswith("@") and
+                Path(a[1:]).resolve().is_relative_to(ROOT) else ("--tuning" if a else "fe_autotune"))
 
     f32 = run_form(api, g, X0, lists, api.FP32, args, tuning)
     kt, kr = f32["kt"], f32["kt_region"]
@@ -320,6 +335,7 @@ def main() -> int:
             "epochs_timed": args.steps, "step": f"one epoch ({batches} dependent minibatches)",
             "graph_replay": f32["use_graph"], "negative_draws_equal_to_n": hits,
             "tuning": f32["tuning"],
+            "tuning_source": source(args.tuning),
         },
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBPS, "unit": "GB/s",
@@ -352,6 +368,7 @@ def main() -> int:
             "loss_fp64_after_timed_epochs": b16["loss"],
             "loss_rel_diff_vs_fp32": (b16["loss"] - f32["loss"]) / f32["loss"],
             "tuning": b16["tuning"],
+            "tuning_source": source(args.bf16_tuning),
         }
         # R-MAT training at this rate is chaotic: the summation order alone
         # (the tiling) moves the final loss. An FP32 run with the BF16 run's
