@@ -3,6 +3,7 @@
 // embedding writer. Nothing here touches the GPU.
 #include <algorithm>
 #include <cerrno>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -10,6 +11,10 @@
 #include <new>
 #include <string>
 #include <vector>
+
+#include <omp.h>
+#include <sys/mman.h>
+#include <sys/stat.h>
 
 #include "force_embed.h"
 #include "fe_host.hpp"
@@ -370,6 +375,116 @@ extern "C" fe_status fe_init_uniform(float *x, int64_t count, float lo, float hi
 // kept as given. Values are ignored (the reference replaces them with the
 // column id, :64-75, and the embedding path never reads them). Self-loops
 // are kept: they add a zero force, as in the reference.
+namespace {
+
+// One MatrixMarket entry line "i j [value...]" at p (line end excluded):
+// 1-based i and j as sscanf("%lld %lld") reads them (leading blanks, an
+// optional sign, digits, blanks between the two). False if they do not parse.
+bool parse_entry(const char *p, const char *end, long long &i, long long &j) {
+  auto blank = [](char c) { return c == ' ' || c == '\t' || c == '\v' || c == '\f' || c == '\r'; };
+  auto number = [&](long long &out) {
+    while (p < end && blank(*p)) ++p;
+    bool neg = false;
+    if (p < end && (*p == '+' || *p == '-')) neg = *p++ == '-';
+    if (p >= end || *p < '0' || *p > '9') return false;
+    long long x = 0;
+    while (p < end && *p >= '0' && *p <= '9') {
+      if (x > (LLONG_MAX - 9) / 10) return false;
+      x = x * 10 + (*p++ - '0');
+    }
+    out = neg ? -x : x;
+    return true;
+  };
+  return number(i) && number(j);
+}
+
+} // namespace
+
+// The entry lines of a MatrixMarket file in buf[off, size): comment lines
+// ('%') and lines starting with a line end are skipped, the first nnz entries
+// are kept (in file order, 0-based), an unparsable or out-of-range line
+// before them is an error, and lines after them are not read, as a serial
+// line-by-line reader would. Chunks of whole lines are parsed in parallel.
+static fe_status parse_mtx_entries(const char *buf, size_t off, size_t size, long long rows,
+                                   long long cols, long long nnz, std::vector<int64_t> &u,
+                                   std::vector<int64_t> &v) {
+  const size_t len = size > off && buf ? size - off : 0;
+  if (len == 0) {
+    if (nnz == 0) return FE_OK;
+    set_host_error("fe_read_mtx: fewer entries than announced");
+    return FE_ERR_INVALID_ARG;
+  }
+  const char *base = buf + off;
+  const int nchunks = (int)std::max<size_t>(1, std::min<size_t>(len / (1 << 20) + 1,
+                                                                 (size_t)omp_get_max_threads() * 8));
+  std::vector<size_t> cut((size_t)nchunks + 1, len);
+  cut[0] = 0;
+  for (int c = 1; c < nchunks; ++c) {
+    size_t p = std::max(cut[c - 1], len / nchunks * (size_t)c);
+    while (p < len && base[p - 1] != '\n') ++p;
+    cut[c] = p;
+  }
+  struct Chunk {
+    std::vector<int64_t> u, v;
+    bool bad = false; // stopped at a bad line (its entries are those before it)
+  };
+  std::vector<Chunk> ch((size_t)nchunks);
+  int oom = 0;
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : oom)
+  for (int c = 0; c < nchunks; ++c) {
+    try {
+      Chunk &k = ch[(size_t)c];
+      const char *p = base + cut[c], *e = base + cut[c + 1];
+      k.u.reserve((size_t)(e - p) / 8);
+      k.v.reserve((size_t)(e - p) / 8);
+      while (p < e) {
+        const char *nl = static_cast<const char *>(memchr(p, '\n', (size_t)(e - p)));
+        const char *le = nl ? nl : e;
+        if (*p != '%' && *p != '\n' && *p != '\r') {
+          long long i, j;
+          if (!parse_entry(p, le, i, j) || i < 1 || j < 1 || i > rows || j > cols) {
+            k.bad = true;
+            break;
+          }
+          k.u.push_back(i - 1);
+          k.v.push_back(j - 1);
+        }
+        p = nl ? nl + 1 : e;
+      }
+    } catch (...) {
+      oom = 1;
+    }
+  }
+  if (oom) {
+    set_host_error("fe_read_mtx: out of host memory");
+    return FE_ERR_OOM;
+  }
+  long long total = 0;
+  try {
+    u.reserve((size_t)std::min<long long>(nnz, 1LL << 24));
+    v.reserve((size_t)std::min<long long>(nnz, 1LL << 24));
+    for (const Chunk &k : ch) {
+      const long long take = std::min<long long>((long long)k.u.size(), nnz - total);
+      u.insert(u.end(), k.u.begin(), k.u.begin() + take);
+      v.insert(v.end(), k.v.begin(), k.v.begin() + take);
+      total += take;
+      if (total == nnz) break;
+      if (k.bad) {
+        set_host_error("fe_read_mtx: bad entry line");
+        return FE_ERR_INVALID_ARG;
+      }
+    }
+  } catch (...) {
+    set_host_error("fe_read_mtx: out of host memory");
+    return FE_ERR_OOM;
+  }
+  if (total != nnz) {
+    set_host_error("fe_read_mtx: fewer entries than announced");
+    return FE_ERR_INVALID_ARG;
+  }
+  return FE_OK;
+}
+
 extern "C" fe_status fe_read_mtx(const char *path, fe_graph *out) {
   if (!path || !out) {
     set_host_error("fe_read_mtx: invalid argument");
@@ -416,36 +531,31 @@ extern "C" fe_status fe_read_mtx(const char *path, fe_graph *out) {
                    "size line");
     return FE_ERR_INVALID_ARG;
   }
-  std::vector<int64_t> u, v;
-  long long read = 0;
-  try {
-    // the announced count is only a hint: a bad header must not reserve
-    // more than the entries that follow
-    const size_t hint = (size_t)std::min<long long>(nnz, 1LL << 24);
-    u.reserve(hint);
-    v.reserve(hint);
-    while (read < nnz && fgets(line, sizeof line, f)) {
-      if (line[0] == '%' || line[0] == '\n' || line[0] == '\r') continue;
-      long long i, j;
-      if (sscanf(line, "%lld %lld", &i, &j) != 2 || i < 1 || j < 1 || i > rows || j > cols) {
-        fclose(f);
-        set_host_error("fe_read_mtx: bad entry line");
-        return FE_ERR_INVALID_ARG;
-      }
-      u.push_back(i - 1);
-      v.push_back(j - 1);
-      ++read;
-    }
-  } catch (...) {
+  // the entries: the rest of the file, memory-mapped and parsed in parallel
+  // chunks cut at line ends, merged in file order
+  const long data_off = ftell(f);
+  struct stat stt;
+  if (data_off < 0 || fstat(fileno(f), &stt) != 0) {
     fclose(f);
-    set_host_error("fe_read_mtx: out of host memory");
-    return FE_ERR_OOM;
-  }
-  fclose(f);
-  if (read != nnz) {
-    set_host_error("fe_read_mtx: fewer entries than announced");
+    set_host_error("fe_read_mtx: cannot size the file");
     return FE_ERR_INVALID_ARG;
   }
+  const size_t fsize = (size_t)stt.st_size;
+  const char *map = nullptr;
+  if (fsize > (size_t)data_off) {
+    void *m = mmap(nullptr, fsize, PROT_READ, MAP_PRIVATE, fileno(f), 0);
+    if (m == MAP_FAILED) {
+      fclose(f);
+      set_host_error(std::string("fe_read_mtx: cannot map the file: ") + strerror(errno));
+      return FE_ERR_INVALID_ARG;
+    }
+    map = static_cast<const char *>(m);
+  }
+  fclose(f);
+  std::vector<int64_t> u, v;
+  fe_status st = parse_mtx_entries(map, (size_t)data_off, fsize, rows, cols, nnz, u, v);
+  if (map) munmap(const_cast<char *>(map), fsize);
+  if (st != FE_OK) return st;
   if (symmetric) return build_csr(rows, u.data(), v.data(), (int64_t)u.size(), true, false, out);
   return build_csr(rows, u.data(), v.data(), (int64_t)u.size(), false, false, out);
 }

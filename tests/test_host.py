@@ -167,3 +167,37 @@ def test_host_argument_checks(fe_lib):
     assert fe_lib.fe_gen_rmat(0, 16, 0.57, 0.19, 0.19, 0, C.byref(g)) == _lib.FE_ERR_INVALID_ARG
     assert fe_lib.fe_gen_sbm(10, 20, 0.1, 0.1, 0, C.byref(g)) == _lib.FE_ERR_INVALID_ARG
     assert fe_lib.fe_read_mtx(b"/nonexistent.mtx", C.byref(g)) == _lib.FE_ERR_INVALID_ARG
+
+
+def test_mtx_large_file_parsed_in_chunks(tmp_path):
+    # ~1.2 M entries (tens of chunks): comment lines, CRLF line ends, tabs and
+    # signs, values, and lines past the announced count (never read); the CSR
+    # must equal the one built from the same pairs, duplicates merged
+    rng = np.random.default_rng(7)
+    n, m = 50_000, 1_200_000
+    u = rng.integers(1, n + 1, m)
+    v = rng.integers(1, n + 1, m)
+    lines = []
+    for k in range(m):
+        if k % 9973 == 0:
+            lines.append("% a comment line")
+        sep = "\t" if k % 3 == 0 else " "
+        lines.append(f"{'+' if k % 11 == 0 else ''}{u[k]}{sep}{v[k]} {k * 0.5}" + ("\r" if k % 5 == 0 else ""))
+    lines += ["1 1", "not an entry"]  # past the count: ignored
+    p = tmp_path / "big.mtx"
+    p.write_text("%%MatrixMarket matrix coordinate real general\n% header comment\n"
+                 f"{n} {n} {m}\n" + "\n".join(lines) + "\n")
+    g = api.Graph.read_mtx(p)
+    key = np.unique((u - 1) * n + (v - 1))
+    rows, cols = key // n, key % n
+    assert g.n == n and g.nnz == key.size
+    assert np.array_equal(g.row_ptr, np.concatenate([[0], np.cumsum(np.bincount(rows, minlength=n))]))
+    assert np.array_equal(g.col_idx, cols.astype(np.int32))
+    # a bad line before the count is an error, wherever its chunk falls
+    bad_lines = list(lines)
+    bad_lines[len(bad_lines) * 2 // 3] = "12 x"
+    q = tmp_path / "bad.mtx"
+    q.write_text("%%MatrixMarket matrix coordinate real general\n"
+                 f"{n} {n} {m}\n" + "\n".join(bad_lines) + "\n")
+    with pytest.raises(api.FeError, match="bad entry line"):
+        api.Graph.read_mtx(q)
