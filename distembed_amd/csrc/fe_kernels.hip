@@ -101,6 +101,36 @@ __device__ __forceinline__ void store8_f32_nt(float *p, const float (&o)[8]) {
   __builtin_nontemporal_store(b, (gf4 *)p + 1);
 }
 
+// A row's 8G floats, held 8 per lane by the G lanes of a group (lane q:
+// components [8q, 8q+8)), stored nontemporally so that each of the two store
+// instructions writes one contiguous half of the row: lane q writes
+// components [4q, 4q+4) and then [4G+4q, 4G+4q+4), fetched from lanes q/2
+// and G/2+q/2 of its group. Stored as each lane's own two 16-byte halves,
+// every instruction would write half of each 128-byte line, and a line
+// leaving L2 between the two would reach memory twice. `first` is the
+// group's first lane; every lane of the group must take part.
+__device__ __forceinline__ void store8_f32_nt_lines(float *row, int q, int G, int first,
+                                                    const float (&o)[8]) {
+  if (G < 2) {
+    store8_f32_nt(row, o);
+    return;
+  }
+  const int h = q & 1;
+  const int la = (first + (q >> 1)) << 2, lb = (first + (G >> 1) + (q >> 1)) << 2;
+  f4v a, b;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const float a0 = __int_as_float(__builtin_amdgcn_ds_bpermute(la, __float_as_int(o[i])));
+    const float a1 = __int_as_float(__builtin_amdgcn_ds_bpermute(la, __float_as_int(o[4 + i])));
+    const float b0 = __int_as_float(__builtin_amdgcn_ds_bpermute(lb, __float_as_int(o[i])));
+    const float b1 = __int_as_float(__builtin_amdgcn_ds_bpermute(lb, __float_as_int(o[4 + i])));
+    a[i] = h ? a1 : a0;
+    b[i] = h ? b1 : b0;
+  }
+  __builtin_nontemporal_store(a, (gf4 *)(row + 4 * q));
+  __builtin_nontemporal_store(b, (gf4 *)(row + 4 * G + 4 * q));
+}
+
 typedef __attribute__((address_space(3))) f4v lf4;
 
 __device__ __forceinline__ void store8_lds(float *p, const float (&o)[8]) {
@@ -942,7 +972,7 @@ __device__ __forceinline__ void sum_light_pack(const StepDesc &sd, const RowSrc 
       sum_round<U, MODE>(xb, ok, sd.lr, acc);
     }
   }
-  if (valid) store8_f32_nt(Y + (size_t)r * sd.D + 8 * q, acc);
+  if (valid) store8_f32_nt_lines(Y + (size_t)r * sd.D, q, G, g * G, acc);
 }
 
 template <int U, int MODE, int GT>
@@ -1012,7 +1042,7 @@ __global__ __launch_bounds__(512) void spmm_kernel(const StepDesc *__restrict__ 
           finish = true;
         }
       }
-      if (finish && g == 0) store8_f32_nt(Y + (size_t)r * D + 8 * q, acc);
+      if (finish && g == 0) store8_f32_nt_lines(Y + (size_t)r * D, q, G, 0, acc);
     }
   } else {
     const int W = (nb - nbi) * wpb;
@@ -1024,7 +1054,7 @@ __global__ __launch_bounds__(512) void spmm_kernel(const StepDesc *__restrict__ 
         for (int k = 0; k < 8; ++k) acc[k] = 0.f;
         sum_edges<U, MODE>(sd, src, (int)it0.y, (int)it0.z, G, NG, g, lane, acc);
         combine_groups(acc, G);
-        if (g == 0) store8_f32_nt(Y + (size_t)it0.x * D + 8 * q, acc);
+        if (g == 0) store8_f32_nt_lines(Y + (size_t)it0.x * D, q, G, 0, acc);
       } else {
         const int pk = u - n_wave_items;
         const int idx = pk * NG + g;
