@@ -522,7 +522,7 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
   std::vector<Cut> cuttings;
   for (int ch : {16, 32, 64})
     for (int mc : {64, 1024})
-      for (int li : {0, 4, 8, 16}) cuttings.push_back(Cut{ch, mc, li});
+      for (int li : {0, 4, 8, 16, 32}) cuttings.push_back(Cut{ch, mc, li});
   const size_t xbytes = sizeof(float) * (size_t)j.n * j.d;
   void *backup = nullptr;
   if ((s = hip_ok(c, hipMalloc(&backup, xbytes), "fe_autotune backup")) != FE_OK) return s;

@@ -19,7 +19,7 @@ namespace fe {
 // without waiting for its items.
 constexpr int kItemInts = 8;
 constexpr int kSub = 4;
-constexpr int kLightCols = 16;
+constexpr int kLightCols = 32;
 
 // Row layout of a job's embedding (X and the BF16 shadow Xs). The rows are
 // cut into batches of B; batch k occupies one slot of B rows, and there are

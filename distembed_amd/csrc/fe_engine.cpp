@@ -150,7 +150,7 @@ struct fe_engine {
     hipEvent_t copied = nullptr;
   } stage[kStages];
   int stage_next = 0;
-  size_t stage_cap = 0; // steps
+  size_t stage_cap = 0; // bytes
   char *dev_round = nullptr;
   int64_t rounds_run = 0;
   int64_t units_done = 0;
@@ -173,34 +173,49 @@ static fe_status ehip(fe_engine *e, hipError_t h, const char *what) {
                std::string(what) + ": " + hipGetErrorString(h));
 }
 
-// Round layout in the staging buffers for K steps:
+// Round layout in the staging buffers for K steps and F force blocks:
 //   StepDesc[K] | int state[4K] | int force_prefix[K+1] | int apply_prefix[K+1]
+//   | int block_step[F]
 // state of step k: {epoch, rot, (rot + epoch) mod P, unused} of its job: a
 // fused step reads its job's rotating slot layout and writes the free slot
 // (the apply prefix is kept for the layout of the buffer; no apply runs).
+// block_step[b] is the step of force block b: a block of a round of many
+// jobs finds its descriptor with one load instead of a binary search over
+// the prefix (log2 K dependent loads before any work).
 constexpr int kStateInts = 4;
-static size_t round_bytes(size_t K) {
+static size_t round_bytes(size_t K, size_t F) {
   size_t b = K * sizeof(StepDesc);
-  b += K * kStateInts * sizeof(int) + 2 * (K + 1) * sizeof(int);
+  b += K * kStateInts * sizeof(int) + 2 * (K + 1) * sizeof(int) + F * sizeof(int);
   return (b + 255) / 256 * 256;
 }
 
-static fe_status ensure_stage(fe_engine *e, size_t K) {
-  if (K <= e->stage_cap) return FE_OK;
-  size_t cap = std::max<size_t>(64, e->stage_cap);
-  while (cap < K) cap *= 2;
+// Force blocks of the next round of `run` (as build_round will lay them out).
+static int round_force_blocks(const fe_engine *e, const std::vector<EngineJob *> &run) {
+  const int ipb = e->tun.waves_per_block * e->tun.items_per_wave;
+  int64_t f = 0;
+  for (const EngineJob *ej : run)
+    f += force_blocks_for(ej->job.host_desc[(size_t)ej->job.plan.batches * DESC_FUSED + ej->batch],
+                          ipb);
+  return (int)f;
+}
+
+// Staging capacity in bytes (each pinned stage and the device round buffer).
+static fe_status ensure_stage(fe_engine *e, size_t bytes) {
+  if (bytes <= e->stage_cap) return FE_OK;
+  size_t cap = std::max<size_t>(round_bytes(64, 4096), e->stage_cap);
+  while (cap < bytes) cap *= 2;
   hipStreamSynchronize(e->stream);
   // a failure below leaves no stage usable: the next call allocates again
   e->stage_cap = 0;
   for (auto &s : e->stage) {
     if (s.host) hipHostFree(s.host);
     s.host = nullptr;
-    fe_status st = ehip(e, hipHostMalloc((void **)&s.host, round_bytes(cap)), "pinned staging");
+    fe_status st = ehip(e, hipHostMalloc((void **)&s.host, cap), "pinned staging");
     if (st != FE_OK) return st;
   }
   if (e->dev_round) hipFree(e->dev_round);
   e->dev_round = nullptr;
-  fe_status st = ehip(e, hipMalloc((void **)&e->dev_round, round_bytes(cap)), "round buffer");
+  fe_status st = ehip(e, hipMalloc((void **)&e->dev_round, cap), "round buffer");
   if (st != FE_OK) return st;
   e->stage_cap = cap;
   return FE_OK;
@@ -273,7 +288,7 @@ extern "C" fe_status fe_engine_create(fe_engine **out, int device, int64_t budge
     e->arena.reset(new Arena());
     s = e->arena->init((size_t)budget, g_engine_error);
   }
-  if (s == FE_OK) s = ensure_stage(e, 64);
+  if (s == FE_OK) s = ensure_stage(e, round_bytes(64, 4096));
   if (s != FE_OK) {
     g_engine_error = "fe_engine_create: " + g_engine_error;
     fe_engine_destroy(e);
@@ -476,6 +491,9 @@ static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *
       }
     }
   }
+  int *bs = pa + K + 1;
+  for (int k = 0; k < K; ++k)
+    for (int b = pf[k]; b < pf[k + 1]; ++b) bs[b] = k;
   return FE_OK;
 }
 
@@ -512,7 +530,8 @@ static fe_status run_rounds(fe_engine *e, int rounds) {
     for (auto &jp : e->jobs)
       if (!jp->launched_all && !jp->collected) run.push_back(jp.get());
     if (run.empty()) break;
-    if ((s = ensure_stage(e, run.size())) != FE_OK) return s;
+    const size_t rbytes = round_bytes(run.size(), (size_t)round_force_blocks(e, run));
+    if ((s = ensure_stage(e, rbytes)) != FE_OK) return s;
     // jobs whose uploads are still in flight join once they have landed
     for (EngineJob *ej : run) {
       if (!ej->waited_upload) {
@@ -531,15 +550,15 @@ static fe_status run_rounds(fe_engine *e, int rounds) {
     // from here on the jobs have advanced: a failure ends them all
     const int *hpf = reinterpret_cast<const int *>(stg.host + K * sizeof(StepDesc)) + kStateInts * K;
     const int fblocks = hpf[K];
-    s = ehip(e, hipMemcpyAsync(e->dev_round, stg.host, round_bytes(K), hipMemcpyHostToDevice,
-                               e->stream),
+    s = ehip(e, hipMemcpyAsync(e->dev_round, stg.host, rbytes, hipMemcpyHostToDevice, e->stream),
              "round copy");
     if (s == FE_OK) s = ehip(e, hipEventRecord(stg.copied, e->stream), "event");
     const StepDesc *dsd = reinterpret_cast<const StepDesc *>(e->dev_round);
     const int *dpf = reinterpret_cast<const int *>(e->dev_round + K * sizeof(StepDesc)) + kStateInts * K;
     if (s == FE_OK)
-      s = ehip(e, launch_force(dsd, dpf, K, fblocks, e->tun.waves_per_block, e->tun.items_per_wave,
-                               e->tun.unroll, lds, round_mode(run), round_g(run), e->stream),
+      s = ehip(e, launch_force(dsd, dpf, dpf + 2 * (K + 1), K, fblocks, e->tun.waves_per_block,
+                               e->tun.items_per_wave, e->tun.unroll, lds, round_mode(run),
+                               round_g(run), e->stream),
                "force launch");
     for (EngineJob *ej : finished)
       if (s == FE_OK) s = ehip(e, hipEventRecord(ej->done, e->stream), "event");
@@ -665,7 +684,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
     for (EngineJob *ej : all)
       if (!ej->launched_all) run.push_back(ej);
     if (run.empty()) break;
-    const size_t rb = round_bytes(run.size());
+    const size_t rb = round_bytes(run.size(), (size_t)round_force_blocks(e, run));
     host.resize(total + rb);
     int K = 0;
     size_t lds = 0;
@@ -704,9 +723,9 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
       const RoundInfo &ri = rounds[r];
       const StepDesc *dsd = reinterpret_cast<const StepDesc *>(dsched + ri.offset);
       const int *dpf = reinterpret_cast<const int *>(dsched + ri.offset + ri.K * sizeof(StepDesc)) + kStateInts * ri.K;
-      ls = ehip(e, launch_force(dsd, dpf, ri.K, ri.fblocks, e->tun.waves_per_block,
-                                e->tun.items_per_wave, e->tun.unroll, ri.lds, ri.mode, ri.g_const,
-                                e->stream),
+      ls = ehip(e, launch_force(dsd, dpf, dpf + 2 * (ri.K + 1), ri.K, ri.fblocks,
+                                e->tun.waves_per_block, e->tun.items_per_wave, e->tun.unroll,
+                                ri.lds, ri.mode, ri.g_const, e->stream),
                 "force launch");
     }
     hipGraph_t g = nullptr;
@@ -824,7 +843,7 @@ extern "C" fe_status fe_engine_autotune(fe_engine *e, const fe_job_desc *jobs, i
   fe_tuning best_c{};
   int n = 0;
   std::vector<int> ids;
-  for (int light : {8, 16}) {
+  for (int light : {8, 16, 32}) {
     fe_tuning cut{64, 1024, orig.unroll, orig.waves_per_block, orig.items_per_wave, light};
     if ((s = fe_engine_set_tuning(e, &cut)) != FE_OK) break;
     ids.clear();

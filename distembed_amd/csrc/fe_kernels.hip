@@ -856,13 +856,15 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
 }
 
 // Steps of one or more jobs in one launch (engine rounds): the block finds
-// its step in the block prefix and reads the step's descriptor from memory.
+// its step (one load from the block-to-step table of a round of more than
+// two steps, else a search of the block prefix) and reads the step's
+// descriptor from memory.
 template <int U, int MODE, int GT>
 __global__ __launch_bounds__(512) void force_kernel(
     const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
-    int n_steps, int ipw) {
+    const int *__restrict__ blk_step, int n_steps, int ipw) {
   const int b = blockIdx.x;
-  const int si = find_step(blk_prefix, n_steps, b);
+  const int si = n_steps > 2 ? blk_step[b] : find_step(blk_prefix, n_steps, b);
   force_body<U, MODE, GT>(steps[si], b - blk_prefix[si], blk_prefix[si + 1] - blk_prefix[si]);
   (void)ipw;
 }
@@ -1326,10 +1328,11 @@ hipError_t scratch_address(int device, void **out) {
 // lanes per row G shared by every step of the launch, or 0 if they differ.
 template <int U, int GT>
 static void launch_force_g(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
-                           const StepDesc *steps, const int *blk_prefix, int n_steps, int ipw) {
-  if (mode == 0) hipLaunchKernelGGL((force_kernel<U, 0, GT>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
-  else if (mode == 1) hipLaunchKernelGGL((force_kernel<U, 1, GT>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
-  else hipLaunchKernelGGL((force_kernel<U, 2, GT>), grid, block, lds_bytes, st, steps, blk_prefix, n_steps, ipw);
+                           const StepDesc *steps, const int *blk_prefix, const int *blk_step,
+                           int n_steps, int ipw) {
+  if (mode == 0) hipLaunchKernelGGL((force_kernel<U, 0, GT>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
+  else if (mode == 1) hipLaunchKernelGGL((force_kernel<U, 1, GT>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
+  else hipLaunchKernelGGL((force_kernel<U, 2, GT>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
 }
 
 template <int U, int GT>
@@ -1363,23 +1366,23 @@ hipError_t launch_force1(const StepDesc &sd, int n_blocks, int wpb, int unroll, 
 template <int U>
 static void launch_force_u(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
                            int g_const, const StepDesc *steps, const int *blk_prefix,
-                           int n_steps, int ipw) {
+                           const int *blk_step, int n_steps, int ipw) {
   // compile-time G for the common widths d = 32, 64, 128 (FP32 or BF16)
-  if (mode != 2 && g_const == 16) launch_force_g<U, 16>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw);
-  else if (mode != 2 && g_const == 8) launch_force_g<U, 8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw);
-  else if (mode != 2 && g_const == 4) launch_force_g<U, 4>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw);
-  else launch_force_g<U, 0>(grid, block, lds_bytes, st, mode, steps, blk_prefix, n_steps, ipw);
+  if (mode != 2 && g_const == 16) launch_force_g<U, 16>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+  else if (mode != 2 && g_const == 8) launch_force_g<U, 8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+  else if (mode != 2 && g_const == 4) launch_force_g<U, 4>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+  else launch_force_g<U, 0>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
 }
 
-hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, int n_steps,
-                        int n_blocks, int wpb, int ipw, int unroll, size_t lds_bytes, int mode,
-                        int g_const, hipStream_t st) {
+hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, const int *blk_step,
+                        int n_steps, int n_blocks, int wpb, int ipw, int unroll, size_t lds_bytes,
+                        int mode, int g_const, hipStream_t st) {
   dim3 grid(n_blocks), block(64 * wpb);
   switch (unroll) {
-  case 1: launch_force_u<1>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, n_steps, ipw); break;
-  case 2: launch_force_u<2>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, n_steps, ipw); break;
-  case 8: launch_force_u<8>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, n_steps, ipw); break;
-  default: launch_force_u<4>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, n_steps, ipw); break;
+  case 1: launch_force_u<1>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, blk_step, n_steps, ipw); break;
+  case 2: launch_force_u<2>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, blk_step, n_steps, ipw); break;
+  case 8: launch_force_u<8>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, blk_step, n_steps, ipw); break;
+  default: launch_force_u<4>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, blk_step, n_steps, ipw); break;
   }
   return hipGetLastError();
 }

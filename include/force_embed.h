@@ -76,8 +76,8 @@ typedef struct fe_tuning {
   int unroll;          /* rows in flight per lane group: 1, 2, 4 or 8       */
   int waves_per_block; /* 1, 2, 4 or 8                                      */
   int items_per_wave;  /* wave work units per wave, on average (>= 1)       */
-  int light;           /* rows of degree <= light: one row per lane group,
-                          64/(d/8) rows per wave (rows per wavefront)       */
+  int light;           /* rows of degree <= light (<= 32): one row per lane
+                          group, 64/(d/8) rows per wave (rows per wavefront) */
 } fe_tuning;
 
 typedef struct fe_ctx fe_ctx;
@@ -192,7 +192,7 @@ fe_status fe_set_timing(fe_ctx *ctx, int enable);
  * (1) every launch shape -- unroll {1,2,4} x waves per block {2,4,8} x
  * items per wave {1,2,4} -- on all batches: each batch keeps its fastest;
  * (2) with search_chunking, every cutting -- chunk {16,32,64} x max_chunks
- * {64,1024} x light {0,4,8,16} -- the same way, then (1) again. The
+ * {64,1024} x light {0,4,8,16,32} -- the same way, then (1) again. The
  * context keeps the per-batch result (fe_batch_tuning); *best holds the
  * values most batches chose and *best_ms the resulting ms per epoch. Shapes
  * never change results; the cutting changes the summation order of heavy
@@ -330,7 +330,7 @@ fe_status fe_engine_collect(fe_engine *e, int job_id, float *x_out);
 fe_status fe_engine_set_tuning(fe_engine *e, const fe_tuning *t);
 /* Autotuner of the batching form's tiles (no counterpart in the reference):
  * the n_jobs jobs (their iterations replaced by `epochs`) are submitted once
- * per cutting -- light {8,16} at chunk 64, max_chunks 1024 -- and replayed
+ * per cutting -- light {8,16,32} at chunk 64, max_chunks 1024 -- and replayed
  * through fe_engine_run_graph once per launch shape -- unroll {2,4,8} x
  * waves per block {4,8} x items per wave {1,2}; the fastest replay (device
  * time, HIP events) becomes the engine's tuning. The engine must hold no

@@ -21,12 +21,16 @@ def test_sbm_batches(d):
     g = api.Graph.sbm(2000, 10, 18 / 199, 2 / 1800, seed=0)
     # B=256: 8 batches, the last of 208 rows (sticky repulsive count active)
     _batch_parity(g, d, 256, [(0, 0), (0, 3), (0, 7), (1, 0), (1, 7), (4, 2)])
+    _batch_parity(g, d, 256, [(0, 0), (1, 7)], tuning=dict(light=32))
 
 
 @pytest.mark.parametrize("d", [8, 16, 256, 512])
 def test_other_widths(d):
     g = api.Graph.sbm(700, 7, 0.05, 0.004, seed=3)
     _batch_parity(g, d, 128, [(0, 0), (2, 5)], ns=5 if d <= 128 else 3)
+    # degrees about 15-40: rows up to degree 32 one per lane group
+    g = api.Graph.sbm(700, 7, 0.25, 0.004, seed=3)
+    _batch_parity(g, d, 128, [(1, 2)], ns=5 if d <= 128 else 3, tuning=dict(light=32))
 
 
 def test_hub_rows_split_into_chunks():
@@ -35,7 +39,8 @@ def test_hub_rows_split_into_chunks():
     # last chunk; light rows go one per lane group unless light = 0
     for tun in ({"chunk": 4, "max_chunks": 8}, {"chunk": 32, "max_chunks": 256},
                 {"chunk": 64, "max_chunks": 1}, {"chunk": 32, "light": -1},
-                {"chunk": 64, "max_chunks": 64, "light": 16}):
+                {"chunk": 64, "max_chunks": 64, "light": 16},
+                {"chunk": 64, "max_chunks": 64, "light": 32}):
         _batch_parity(g, 128, 512, [(0, 0), (0, 1), (3, 7)], tuning=tun)
 
 

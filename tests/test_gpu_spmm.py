@@ -45,6 +45,9 @@ def test_spmm_matches_oracle_sbm(d):
     g = api.Graph.sbm(6000, 12, 0.01, 0.0004, seed=5)
     X = api.init_uniform(g.n, d, -0.5, 0.5, seed=2)
     _check(_spmm(g, X, 1000, 0.02), g, X, 0.02, what=f"sbm d={d}")
+    # rows of degree up to 32 one per lane group (columns beside the item)
+    g = api.Graph.sbm(6000, 12, 0.05, 0.0004, seed=5)
+    _check(_spmm(g, X, 1000, 0.02, tuning=dict(light=32)), g, X, 0.02, what=f"sbm d={d} light 32")
 
 
 @pytest.mark.parametrize("quirks", [3, 0])
