@@ -2,6 +2,7 @@
 
   python -m distembed_amd.profile_summary trace  <kernel_stats.csv> <kernel_trace.csv> <out> <command> [<regex>]
   python -m distembed_amd.profile_summary pmc <fetch csv> <write csv> <out> <command> <copyprobe fetch csv> <copyprobe write csv> [<regex>]
+  python -m distembed_amd.profile_summary issue <counter csv> <out> <command> <batches> <form 0|1> [<tcc csv>]
   python -m distembed_amd.profile_summary copyprobe
 
 trace: per-kernel launch count, total and average duration (from the
@@ -12,6 +13,14 @@ pmc: FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of the force kernel (or
 of the kernels <regex> selects, e.g. "spmm_kernel<\d+, 0,"), scaled by what
 the same counters read for a cold 1 GiB device-to-device copy (copyprobe),
 and the resulting HBM bytes per launch (hbm_bytes_per_launch).
+issue: per batch index, for one gather form (force_kernel1<U, 0|1, G>:
+0 FP32, 1 BF16) of a run with one tiling of it, the SQ counters of an
+--pmc run of whole epochs (waves, VALU and vector-memory-read
+instructions per wave) and the VALU busy fraction: VALU instructions x 4
+cycles (a wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles),
+over 1024 SIMDs (256 CUs x 4) and the dispatch's duration at 2.4 GHz (the
+peak engine clock, so the fraction is a lower bound); with a second csv of
+TCC_HIT_sum / TCC_MISS_sum, the L2 hit rate.
 """
 from __future__ import annotations
 
@@ -124,6 +133,64 @@ def copyprobe(nbytes: int = 1 << 30) -> None:
     print(f"copyprobe: copied {nbytes} bytes device to device", flush=True)
 
 
+def issue(csv_path: str, out: str, command: str, batches: int, form: int,
+          tcc_csv: str = "") -> None:
+    def per_form(path):
+        cnt = defaultdict(dict)
+        info = {}
+        for r in csv.DictReader(open(path)):
+            m = re.search(r"force_kernel1<\d+, (\d+), \d+>", r["Kernel_Name"])
+            if not m:
+                continue
+            d = int(r["Dispatch_Id"])
+            cnt[d][r["Counter_Name"]] = float(r["Counter_Value"])
+            info[d] = (int(m.group(1)),
+                       (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
+        forms = defaultdict(list)
+        for d in sorted(cnt):
+            forms[info[d][0]].append((cnt[d], info[d][1]))
+        return forms
+    sq = per_form(csv_path)
+    tcc = per_form(tcc_csv) if tcc_csv else {}
+    simds, clock_ghz = 1024, 2.4
+    lines = ["# rocprofv3 --pmc SQ counters of the force kernel, per batch index",
+             f"command: rocprofv3 --pmc SQ_WAVES SQ_INSTS_VALU SQ_INSTS_VMEM_RD SQ_ACTIVE_INST_VALU "
+             f"SQ_WAIT_INST_ANY SQ_BUSY_CYCLES SQ_WAVE_CYCLES SQ_ACTIVE_INST_ANY -- {command}"]
+    if tcc_csv:
+        lines.append(f"command (L2): rocprofv3 --pmc TCC_HIT_sum TCC_MISS_sum GRBM_GUI_ACTIVE -- {command}")
+    lines += ["valu_busy = SQ_INSTS_VALU x 4 cycles / 1024 SIMDs / (dispatch us x 2400 cycles/us); "
+              "durations are those of the counter run", ""]
+    for name in ("FP32 gathers" if form == 0 else "BF16 gathers",):
+        rows = sq.get(form, [])
+        if not rows:
+            raise SystemExit(f"no force_kernel1 dispatches of form {form} in {csv_path}")
+        n_ep = len(rows) // batches
+        lines.append(f"{name}: {n_ep} epoch(s) of {batches} launches")
+        lines.append("batch  waves   valu/wave  vmem_rd/wave  us      valu_busy" +
+                     ("  l2_hit" if tcc else ""))
+        tot_v = tot_t = 0.0
+        for b in range(batches):
+            sel = [rows[e * batches + b] for e in range(n_ep)]
+            w = statistics.mean(c["SQ_WAVES"] for c, _ in sel)
+            v = statistics.mean(c["SQ_INSTS_VALU"] for c, _ in sel)
+            vm = statistics.mean(c["SQ_INSTS_VMEM_RD"] for c, _ in sel)
+            us = statistics.median(t for _, t in sel)
+            busy = v * 4 / simds / (us * clock_ghz * 1e3)
+            tot_v += v * 4 / simds / (clock_ghz * 1e3)
+            tot_t += us
+            line = f"{b:5d} {w:7.0f} {v / w:10.1f} {vm / w:12.2f} {us:8.1f} {busy:9.2f}"
+            if tcc:
+                tsel = [tcc[form][e * batches + b][0] for e in range(len(tcc[form]) // batches)]
+                h = sum(c["TCC_HIT_sum"] for c in tsel)
+                mi = sum(c["TCC_MISS_sum"] for c in tsel)
+                line += f"  {h / max(h + mi, 1.0):6.2f}"
+            lines.append(line)
+        lines.append(f"epoch: {tot_t:.1f} us of dispatches, VALU {tot_v:.1f} us at the peak clock "
+                     f"({tot_v / tot_t:.2f})")
+        lines.append("")
+    open(out, "w").write("\n".join(lines) + "\n")
+
+
 def calibration(path: str, counter: str, nbytes: int = 1 << 30) -> float:
     """Counter KiB of the last runtime copy kernel / the KiB it copied."""
     vals = []
@@ -141,6 +208,9 @@ if __name__ == "__main__":
               sys.argv[6] if len(sys.argv) > 6 else "")
     elif mode == "copyprobe":
         copyprobe()
+    elif mode == "issue":
+        issue(sys.argv[2], sys.argv[3], sys.argv[4], int(sys.argv[5]), int(sys.argv[6]),
+              sys.argv[7] if len(sys.argv) > 7 else "")
     else:
         pmc(sys.argv[2], sys.argv[3], sys.argv[4], sys.argv[5], sys.argv[6], sys.argv[7],
             sys.argv[8] if len(sys.argv) > 8 else "")
