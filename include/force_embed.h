@@ -137,7 +137,8 @@ fe_status fe_step_delta(fe_ctx *ctx, int epoch, int batch, float *delta_out);
  * 15 MiB of partials (the batch's partial slots x d x 4 bytes) and 262144
  * counters; a batch needing more gets FE_ERR_INVALID_ARG. Every call on a
  * GPU shares that array, so calls whose launches overlap on one GPU (other
- * contexts or streams included) race; callers order them. The context must
+ * contexts or streams included) race; callers order them, and order the
+ * call after this context's own pending work. The context must
  * be prepared: one fe_step, fe_step_delta or fe_train call after the last
  * change of the CSR, X, negative lists, batch size or tuning prepares it;
  * otherwise FE_ERR_STATE. The call launches three kernels (counter fill,
@@ -334,7 +335,8 @@ fe_status fe_engine_set_tuning(fe_engine *e, const fe_tuning *t);
  * through fe_engine_run_graph once per launch shape -- unroll {2,4,8} x
  * waves per block {4,8} x items per wave {1,2}; the fastest replay (device
  * time, HIP events) becomes the engine's tuning. The engine must hold no
- * uncollected job; the trial jobs' results are discarded. Up to max_results
+ * uncollected job, and the n_jobs jobs must fit its budget together
+ * (FE_ERR_OVER_BUDGET otherwise); the trial jobs' results are discarded. Up to max_results
  * candidates go to results / results_ms (replay ms); *best and *best_ms get
  * the winner.                                                             */
 fe_status fe_engine_autotune(fe_engine *e, const fe_job_desc *jobs, int n_jobs,
