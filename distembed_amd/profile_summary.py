@@ -165,6 +165,9 @@ def issue(csv_path: str, out: str, command: str, batches: int, form: int,
         if not rows:
             raise SystemExit(f"no force_kernel1 dispatches of form {form} in {csv_path}")
         n_ep = len(rows) // batches
+        if n_ep < 1:
+            raise SystemExit(f"{len(rows)} dispatches of form {form} in {csv_path}: "
+                             f"fewer than one epoch of {batches}")
         lines.append(f"{name}: {n_ep} epoch(s) of {batches} launches")
         lines.append("batch  waves   valu/wave  vmem_rd/wave  us      valu_busy" +
                      ("  l2_hit" if tcc else ""))
