@@ -1,4 +1,4 @@
-"""Summarise rocprofv3 CSV output into the short text kept under profiles/.
+r"""Summarise rocprofv3 CSV output into the short text kept under profiles/.
 
   python -m distembed_amd.profile_summary trace  <kernel_stats.csv> <kernel_trace.csv> <out> <command> [<regex>]
   python -m distembed_amd.profile_summary pmc <fetch csv> <write csv> <out> <command> <copyprobe fetch csv> <copyprobe write csv> [<regex>]
@@ -31,12 +31,22 @@ import sys
 from collections import defaultdict
 
 
+def _rows(path: str) -> list:
+    with open(path, newline="") as f:
+        return list(csv.DictReader(f))
+
+
+def _write(path: str, lines: list) -> None:
+    with open(path, "w") as f:
+        f.write("\n".join(lines) + "\n")
+
+
 def _short(name: str) -> str:
     return name.split("(")[0].replace("void ", "").strip()
 
 
 def trace(stats_csv: str, trace_csv: str, out: str, command: str, match: str = "") -> None:
-    rows = list(csv.DictReader(open(stats_csv)))
+    rows = list(_rows(stats_csv))
     lines = [f"# rocprofv3 kernel statistics", f"command: {command}", ""]
     lines.append(f"{'kernel':60s} {'calls':>7s} {'total_ms':>10s} {'avg_us':>9s}")
     for r in rows:
@@ -46,7 +56,7 @@ def trace(stats_csv: str, trace_csv: str, out: str, command: str, match: str = "
         avg = float(r.get("AverageNs", 0)) / 1e3
         lines.append(f"{name[:60]:60s} {calls:7d} {tot:10.3f} {avg:9.2f}")
     durs = defaultdict(list)
-    for r in csv.DictReader(open(trace_csv)):
+    for r in _rows(trace_csv):
         durs[_short(r["Kernel_Name"])].append(
             (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
     lines.append("")
@@ -60,12 +70,12 @@ def trace(stats_csv: str, trace_csv: str, out: str, command: str, match: str = "
         lines.append(f"force launches, all instantiations: n={len(allv)} "
                      f"avg_us={statistics.mean(allv):.2f} median_us={statistics.median(allv):.2f} "
                      f"max_us={max(allv):.2f}")
-    open(out, "w").write("\n".join(lines) + "\n")
+    _write(out, lines)
 
 
 def _per_dispatch(path: str, counter: str):
     vals = defaultdict(dict)  # kernel -> dispatch -> value
-    for r in csv.DictReader(open(path)):
+    for r in _rows(path):
         if r.get("Counter_Name") != counter:
             continue
         vals[_short(r["Kernel_Name"])][int(r["Dispatch_Id"])] = float(r["Counter_Value"])
@@ -107,7 +117,7 @@ def pmc(fetch_csv: str, write_csv: str, out: str, command: str, cal_fetch_csv: s
         lines.append(f"hbm_bytes_per_launch = {(tot_f + tot_w) / tot_n:.6e}")
         lines.append(f"hbm_read_bytes_per_launch = {tot_f / tot_n:.6e}")
         lines.append(f"hbm_write_bytes_per_launch = {tot_w / tot_n:.6e}")
-    open(out, "w").write("\n".join(lines) + "\n")
+    _write(out, lines)
 
 
 def copyprobe(nbytes: int = 1 << 30) -> None:
@@ -138,7 +148,7 @@ def issue(csv_path: str, out: str, command: str, batches: int, form: int,
     def per_form(path):
         cnt = defaultdict(dict)
         info = {}
-        for r in csv.DictReader(open(path)):
+        for r in _rows(path):
             m = re.search(r"force_kernel1<\d+, (\d+), \d+>", r["Kernel_Name"])
             if not m:
                 continue
@@ -191,13 +201,13 @@ def issue(csv_path: str, out: str, command: str, batches: int, form: int,
         lines.append(f"epoch: {tot_t:.1f} us of dispatches, VALU {tot_v:.1f} us at the peak clock "
                      f"({tot_v / tot_t:.2f})")
         lines.append("")
-    open(out, "w").write("\n".join(lines) + "\n")
+    _write(out, lines)
 
 
 def calibration(path: str, counter: str, nbytes: int = 1 << 30) -> float:
     """Counter KiB of the last runtime copy kernel / the KiB it copied."""
     vals = []
-    for r in csv.DictReader(open(path)):
+    for r in _rows(path):
         if r.get("Counter_Name") == counter and "copyBuffer" in r["Kernel_Name"]:
             vals.append((int(r["Dispatch_Id"]), float(r["Counter_Value"])))
     vals.sort()
