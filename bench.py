@@ -180,7 +180,7 @@ def run_form(api, g, X0, lists, gather, args, tuning=None):
                 f"over {len(table)} trials")
         tun = ctx.get_tuning()
         per_batch = [[t[k] for k in ("unroll", "waves_per_block", "items_per_wave", "chunk",
-                                     "max_chunks", "light")] for t in ctx.batch_tunings()]
+                                     "max_chunks", "light", "slice")] for t in ctx.batch_tunings()]
         edges, n0, n1 = ctx.epoch_units(ns)
         use_graph = not args.no_graph
         if args.warmup:

@@ -50,7 +50,8 @@ class fe_graph(C.Structure):
 
 class fe_tuning(C.Structure):
     _fields_ = [("chunk", C.c_int), ("max_chunks", C.c_int), ("unroll", C.c_int),
-                ("waves_per_block", C.c_int), ("items_per_wave", C.c_int), ("light", C.c_int)]
+                ("waves_per_block", C.c_int), ("items_per_wave", C.c_int), ("light", C.c_int),
+                ("slice", C.c_int)]
 
 
 class fe_job_desc(C.Structure):

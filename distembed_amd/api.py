@@ -26,11 +26,12 @@ def _ptr(a: np.ndarray, ctype):
     return a.ctypes.data_as(C.POINTER(ctype))
 
 
-def _tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light):
+def _tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light, slice=None):
     """fe_tuning from optional fields: 0 keeps a field, light -1 sets 0."""
     z = lambda v: 0 if v is None else int(v)
     lt = 0 if light is None else (-1 if int(light) == 0 else int(light))
-    return fe_tuning(z(chunk), z(max_chunks), z(unroll), z(waves_per_block), z(items_per_wave), lt)
+    return fe_tuning(z(chunk), z(max_chunks), z(unroll), z(waves_per_block), z(items_per_wave), lt,
+                     z(slice))
 
 
 def _host_err() -> str:
@@ -191,9 +192,10 @@ class ForceEmbed:
         self._check(self._lib.fe_configure(self._ctx, batch_size, lr, max_bound))
 
     def set_tuning(self, chunk=None, max_chunks=None, unroll=None, waves_per_block=None,
-                   items_per_wave=None, light=None) -> None:
-        """Fields left at None keep their value; light=0 turns light rows off."""
-        t = _tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light)
+                   items_per_wave=None, light=None, slice=None) -> None:
+        """Fields left at None keep their value; light=0 turns light rows off;
+        slice is the components of a row per lane (8 or 16)."""
+        t = _tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light, slice)
         self._check(self._lib.fe_set_tuning(self._ctx, C.byref(t)))
 
     def get_tuning(self) -> dict:
@@ -270,9 +272,12 @@ class ForceEmbed:
 
     def set_batch_tunings(self, per_batch) -> None:
         """Per-batch tunings: one (unroll, waves_per_block, items_per_wave,
-        chunk, max_chunks, light) per batch."""
-        for b, (u, w, i, ch, mc, li) in enumerate(per_batch):
-            t = fe_tuning(int(ch), int(mc), int(u), int(w), int(i), -1 if int(li) == 0 else int(li))
+        chunk, max_chunks, light[, slice]) per batch (slice 8 when absent)."""
+        for b, row in enumerate(per_batch):
+            u, w, i, ch, mc, li = row[:6]
+            sl = int(row[6]) if len(row) > 6 else 8
+            t = fe_tuning(int(ch), int(mc), int(u), int(w), int(i), -1 if int(li) == 0 else int(li),
+                          sl)
             self._check(self._lib.fe_set_batch_tuning(self._ctx, b, C.byref(t)))
 
     def set_timing(self, enable) -> None:
@@ -380,9 +385,10 @@ class Engine:
         return x
 
     def set_tuning(self, chunk=None, max_chunks=None, unroll=None, waves_per_block=None,
-                   items_per_wave=None, light=None) -> None:
-        """As ForceEmbed.set_tuning; chunking applies to jobs submitted later."""
-        t = _tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light)
+                   items_per_wave=None, light=None, slice=None) -> None:
+        """As ForceEmbed.set_tuning; chunking applies to jobs submitted later
+        (engine rounds take slices of 8 only)."""
+        t = _tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light, slice)
         self._check(self._lib.fe_engine_set_tuning(self._e, C.byref(t)))
 
     def autotune(self, jobs, epochs: int = 2):

@@ -63,7 +63,7 @@ def _train_timed(g, X0, lists, gather, B, ns, lr, steps, warmup, tuning=None):
             ctx.set_batch_tunings(tuning["per_batch"])
         tun = {"tuning": ctx.get_tuning(),
                "per_batch": [[t[k] for k in ("unroll", "waves_per_block", "items_per_wave",
-                                              "chunk", "max_chunks", "light")]
+                                              "chunk", "max_chunks", "light", "slice")]
                              for t in ctx.batch_tunings()]}
         if warmup:
             ctx.upload_x(X0)

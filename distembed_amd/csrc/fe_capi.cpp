@@ -178,6 +178,7 @@ extern "C" fe_status fe_set_tuning(fe_ctx *c, const fe_tuning *t) {
   if (t->waves_per_block) nt.waves_per_block = t->waves_per_block;
   if (t->items_per_wave) nt.items_per_wave = t->items_per_wave;
   if (t->light) nt.light = t->light < 0 ? 0 : t->light;
+  if (t->slice) nt.slice = t->slice;
   drop_graph(c);
   return from_job(c, job_set_tuning(c->job, nt));
 }
@@ -190,6 +191,7 @@ extern "C" fe_status fe_get_tuning(const fe_ctx *c, fe_tuning *t) {
   t->waves_per_block = c->job.tun.waves_per_block;
   t->items_per_wave = c->job.tun.items_per_wave;
   t->light = c->job.tun.light;
+  t->slice = c->job.tun.slice;
   return FE_OK;
 }
 
@@ -328,8 +330,8 @@ extern "C" fe_status fe_step_delta_noalloc(fe_ctx *c, int epoch, int batch, floa
   const Shape sh = job_shape(j, batch);
   return hip_ok(c,
                 launch_force1(sd, j.force_blocks[batch], sh.waves_per_block, sh.unroll,
-                              force_lds_bytes(j.ns, j.d), j.dtype == FE_GATHER_BF16 ? 1 : 0, j.G,
-                              st),
+                              force_lds_bytes(j.ns, j.d), j.dtype == FE_GATHER_BF16 ? 1 : 0, sd.G,
+                              j.d / sd.G, st),
                 "fe_step_delta_noalloc: force kernel launch");
 }
 
@@ -520,9 +522,12 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
     for (int w : {2, 4, 8})
       for (int ipw : {1, 2, 4}) shapes.push_back(Shape{u, w, ipw});
   std::vector<Cut> cuttings;
-  for (int ch : {16, 32, 64})
-    for (int mc : {64, 1024})
-      for (int li : {0, 4, 8, 16, 32}) cuttings.push_back(Cut{ch, mc, li});
+  for (int sl : {8, 16}) {
+    if (sl > j.d) continue;
+    for (int ch : {16, 32, 64})
+      for (int mc : {64, 1024})
+        for (int li : {0, 4, 8, 16, 32}) cuttings.push_back(Cut{ch, mc, li, sl});
+  }
   const size_t xbytes = sizeof(float) * (size_t)j.n * j.d;
   void *backup = nullptr;
   if ((s = hip_ok(c, hipMalloc(&backup, xbytes), "fe_autotune backup")) != FE_OK) return s;
@@ -545,7 +550,7 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
   auto record = [&](const Shape &sh, const Cut &ct, double ms) {
     if (results && n < max_results) {
       results[n] = fe_tuning{ct.chunk, ct.max_chunks, sh.unroll, sh.waves_per_block,
-                             sh.items_per_wave, ct.light};
+                             sh.items_per_wave, ct.light, ct.slice};
       if (results_ms) results_ms[n] = ms;
     }
     ++n;
@@ -572,7 +577,8 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
     for (const Cut &a : v) {
       int k = 0;
       for (const Cut &b : v)
-        k += a.chunk == b.chunk && a.max_chunks == b.max_chunks && a.light == b.light;
+        k += a.chunk == b.chunk && a.max_chunks == b.max_chunks && a.light == b.light &&
+             a.slice == b.slice;
       if (k > bestc) {
         bestc = k;
         m = a;
@@ -584,7 +590,7 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
   auto apply = [&](const std::vector<Shape> &vs, const std::vector<Cut> &vc) -> fe_status {
     const Shape ms = most_common_shape(vs, Shape{orig.unroll, orig.waves_per_block,
                                                  orig.items_per_wave});
-    const Cut mc = most_common_cut(vc, Cut{orig.chunk, orig.max_chunks, orig.light});
+    const Cut mc = most_common_cut(vc, Cut{orig.chunk, orig.max_chunks, orig.light, orig.slice});
     Tuning t = orig;
     t.unroll = ms.unroll;
     t.waves_per_block = ms.waves_per_block;
@@ -592,6 +598,7 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
     t.chunk = mc.chunk;
     t.max_chunks = mc.max_chunks;
     t.light = mc.light;
+    t.slice = mc.slice;
     fe_status r = from_job(c, job_set_tuning(j, t));
     if (r == FE_OK) r = from_job(c, job_set_shapes(j, vs));
     if (r == FE_OK) r = from_job(c, job_set_cuts(j, vc));
@@ -661,7 +668,8 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
   if (s != FE_OK) return s;
   if (n_results) *n_results = n;
   if (best) *best = fe_tuning{j.tun.chunk, j.tun.max_chunks, j.tun.unroll,
-                              j.tun.waves_per_block, j.tun.items_per_wave, j.tun.light};
+                              j.tun.waves_per_block, j.tun.items_per_wave, j.tun.light,
+                              j.tun.slice};
   if (best_ms) *best_ms = best_t;
   return FE_OK;
 }
@@ -686,7 +694,8 @@ extern "C" fe_status fe_set_batch_tuning(fe_ctx *c, int batch, const fe_tuning *
                    t->items_per_wave ? t->items_per_wave : v[batch].items_per_wave};
   w[batch] = Cut{t->chunk ? t->chunk : w[batch].chunk,
                  t->max_chunks ? t->max_chunks : w[batch].max_chunks,
-                 t->light ? (t->light < 0 ? 0 : t->light) : w[batch].light};
+                 t->light ? (t->light < 0 ? 0 : t->light) : w[batch].light,
+                 t->slice ? t->slice : w[batch].slice};
   drop_graph(c);
   if ((s = from_job(c, job_set_shapes(j, v))) != FE_OK) return s;
   return from_job(c, job_set_cuts(j, w));
@@ -700,7 +709,7 @@ extern "C" fe_status fe_batch_tuning(const fe_ctx *c, int batch, fe_tuning *out)
   const Shape sh = job_shape(j, batch);
   const Cut ct = job_cut(j, batch);
   *out = fe_tuning{ct.chunk, ct.max_chunks, sh.unroll, sh.waves_per_block, sh.items_per_wave,
-                   ct.light};
+                   ct.light, ct.slice};
   return FE_OK;
 }
 

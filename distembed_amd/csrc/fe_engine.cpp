@@ -354,6 +354,8 @@ extern "C" fe_status fe_engine_set_tuning(fe_engine *e, const fe_tuning *t) {
   if (t->waves_per_block) nt.waves_per_block = t->waves_per_block;
   if (t->items_per_wave) nt.items_per_wave = t->items_per_wave;
   if (t->light) nt.light = t->light < 0 ? 0 : t->light;
+  if (t->slice && t->slice != 8)
+    return efail(e, FE_ERR_INVALID_ARG, "fe_engine_set_tuning: engine rounds use slices of 8");
   Job probe;
   if (job_set_tuning(probe, nt) != FE_OK) return efail(e, FE_ERR_INVALID_ARG, probe.err);
   // chunking of already submitted jobs is fixed; launch shape applies to all
@@ -844,7 +846,7 @@ extern "C" fe_status fe_engine_autotune(fe_engine *e, const fe_job_desc *jobs, i
   int n = 0;
   std::vector<int> ids;
   for (int light : {8, 16, 32}) {
-    fe_tuning cut{64, 1024, orig.unroll, orig.waves_per_block, orig.items_per_wave, light};
+    fe_tuning cut{64, 1024, orig.unroll, orig.waves_per_block, orig.items_per_wave, light, 8};
     if ((s = fe_engine_set_tuning(e, &cut)) != FE_OK) break;
     ids.clear();
     for (const fe_job_desc &d : descs) {
@@ -856,7 +858,7 @@ extern "C" fe_status fe_engine_autotune(fe_engine *e, const fe_job_desc *jobs, i
       for (int w : {4, 8}) {
         for (int ipw : {1, 2}) {
           if (s != FE_OK) break;
-          const fe_tuning c{64, 1024, u, w, ipw, light};
+          const fe_tuning c{64, 1024, u, w, ipw, light, 8};
           if ((s = fe_engine_set_tuning(e, &c)) != FE_OK) break;
           rewind_jobs(e, ids);
           if ((s = fe_engine_run_graph(e)) != FE_OK) break;

@@ -40,7 +40,8 @@ struct Plan {
   std::vector<int32_t> light_rows;  // per batch: trailing light rows
   std::vector<int64_t> light_off;   // batches + 1: first light row of batch j
   int light = 0;                    // rows of degree <= light go one per lane group
-  int pack = 1;                     // lane groups per wave (64 / (d / 8))
+  int pack = 1;                     // lane groups per wave (64 / (d / slice)) unless
+                                    // a batch's cut gives its own
   std::vector<int32_t> splits;    // {row, p0, k, 0} per heavy row
   std::vector<int64_t> split_off; // batches + 1, in splits
   std::vector<int32_t> pslots;    // partial slots used by batch j
@@ -50,11 +51,15 @@ struct Plan {
   int64_t edges_eff_total = 0;
 };
 
-// Cutting of one batch's rows (chunk, max_chunks, light as above).
+// Cutting of one batch's rows (chunk, max_chunks, light as above). slice:
+// components per lane (8 or 16), so a row is d / slice lanes and a light
+// pack holds pack = 64 * slice / d rows (0: build_plan's pack).
 struct Cut {
   int chunk;
   int max_chunks;
   int light;
+  int slice = 8;
+  int pack = 0;
 };
 
 // cuts: one per batch, or null for chunk / max_chunks / light everywhere.

@@ -441,7 +441,12 @@ fe_status job_set_tuning(Job &j, const Tuning &t) {
     j.err = "tuning out of range";
     return FE_ERR_INVALID_ARG;
   }
-  if (t.chunk != j.tun.chunk || t.max_chunks != j.tun.max_chunks || t.light != j.tun.light)
+  if ((t.slice != 8 && t.slice != 16) || (j.d > 0 && t.slice > j.d)) {
+    j.err = "tuning: slice must be 8 or 16 components per lane, and at most d";
+    return FE_ERR_INVALID_ARG;
+  }
+  if (t.chunk != j.tun.chunk || t.max_chunks != j.tun.max_chunks || t.light != j.tun.light ||
+      t.slice != j.tun.slice)
     j.plan_dirty = true;
   j.tun = t;
   j.bshape.clear();
@@ -451,19 +456,24 @@ fe_status job_set_tuning(Job &j, const Tuning &t) {
   return FE_OK;
 }
 
+// Light rows per wave of a batch whose lanes hold `slice` components.
+static int pack_of(const Job &j, int slice) { return 64 / (j.d / slice); }
+
 Cut job_cut(const Job &j, int b) {
   if (b >= 0 && b < (int)j.bcut.size()) return j.bcut[b];
-  return Cut{j.tun.chunk, j.tun.max_chunks, j.tun.light};
+  return Cut{j.tun.chunk, j.tun.max_chunks, j.tun.light, j.tun.slice, pack_of(j, j.tun.slice)};
 }
 
 fe_status job_set_cuts(Job &j, const std::vector<Cut> &cuts) {
   for (const Cut &c : cuts) {
-    if (c.chunk < 1 || c.chunk > 64 || c.max_chunks < 1 || c.light < 0 || c.light > 64) {
+    if (c.chunk < 1 || c.chunk > 64 || c.max_chunks < 1 || c.light < 0 || c.light > 64 ||
+        (c.slice != 8 && c.slice != 16) || c.slice > j.d) {
       j.err = "batch cutting out of range";
       return FE_ERR_INVALID_ARG;
     }
   }
   j.bcut = cuts;
+  for (Cut &c : j.bcut) c.pack = pack_of(j, c.slice);
   j.plan_dirty = true;
   j.steps_dirty = true;
   return FE_OK;
@@ -516,7 +526,7 @@ static StepDesc make_desc(const Job &j, int b, DescKind kind) {
   sd.ns = j.ns;
   sd.n_lists = j.n_lists;
   sd.D = j.d;
-  sd.G = j.G;
+  sd.G = j.d / job_cut(j, b).slice;
   sd.bf16 = j.dtype == FE_GATHER_BF16 ? 1 : 0;
   sd.c_limit = (int)j.c_limit;
   sd.lr = j.lr;
@@ -562,9 +572,9 @@ fe_status job_prepare(Job &j, hipStream_t st, bool slots) {
     if (j.pending_B == j.B && !per_batch) {
       p = std::move(j.pending_plan);
     } else {
-      s = build_plan(j.n, j.row_ptr.data(), nullptr, j.B, j.tun.chunk,
-                               j.tun.max_chunks, j.tun.light, 64 / j.G, j.c_limit, &p,
-                               per_batch ? j.bcut.data() : nullptr);
+      s = build_plan(j.n, j.row_ptr.data(), nullptr, j.B, j.tun.chunk, j.tun.max_chunks,
+                     j.tun.light, pack_of(j, j.tun.slice), j.c_limit, &p,
+                     per_batch ? j.bcut.data() : nullptr);
       if (s != FE_OK) {
         j.err = host_error();
         return s;
@@ -701,7 +711,7 @@ fe_status job_launch_force(Job &j, int b, DescKind kind, hipStream_t st, hipEven
   // descriptors are rebuilt (desc_gen)
   const StepDesc &sd = j.host_desc[(size_t)j.plan.batches * kind + b];
   FE_HIP(j, launch_force1(sd, j.force_blocks[b], sh.waves_per_block, sh.unroll, lds,
-                          j.dtype == FE_GATHER_BF16 ? 1 : 0, j.G, st),
+                          j.dtype == FE_GATHER_BF16 ? 1 : 0, sd.G, j.d / sd.G, st),
          "force kernel launch");
   if (ev) FE_HIP(j, record_event(ev[1], st), "event");
   return FE_OK;
@@ -778,6 +788,7 @@ fe_status job_spmm(Job &j, int iterations, float lr, float *y_out, hipStream_t s
   sd.row_begin = 0;
   sd.rows = (int)j.n;
   sd.batch = 0;
+  sd.G = j.G; // the SpMM kernel takes slices of 8 components
   const int wpb = j.tun.waves_per_block;
   const int blocks = force_blocks_for(sd, wpb * j.tun.items_per_wave);
   const int pf[2] = {0, blocks};

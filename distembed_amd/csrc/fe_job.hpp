@@ -20,9 +20,10 @@ namespace fe {
 hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, const int *blk_step,
                         int n_steps, int n_blocks, int wpb, int ipw, int unroll, size_t lds_bytes,
                         int mode, int g_const, hipStream_t st);
-// One step whose descriptor is passed by value (mode 0 or 1).
+// One step whose descriptor is passed by value (mode 0 or 1); slice: the
+// components per lane (8 or 16), sd.G = sd.D / slice.
 hipError_t launch_force1(const StepDesc &sd, int n_blocks, int wpb, int unroll, size_t lds_bytes,
-                         int mode, int g_const, hipStream_t st);
+                         int mode, int g_const, int slice, hipStream_t st);
 hipError_t launch_spmm(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
                        int wpb, int unroll, int mode, int g_const, float *Y, hipStream_t st);
 hipError_t launch_apply(const StepDesc *steps, const int *blk_prefix, int n_steps, int n_blocks,
@@ -51,6 +52,7 @@ struct Tuning {
   int waves_per_block = 4;
   int items_per_wave = 1;
   int light = 16;
+  int slice = 8; // components per lane (8 or 16): rows of d / slice lanes
 };
 
 // Launch shape of one batch's force launch (does not change results).

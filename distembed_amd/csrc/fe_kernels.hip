@@ -58,6 +58,7 @@ typedef unsigned int u4v __attribute__((ext_vector_type(4)));
 typedef __attribute__((address_space(1))) const f4v gcf4;
 typedef __attribute__((address_space(1))) f4v gf4;
 typedef __attribute__((address_space(1))) const u4v gcu4;
+typedef __attribute__((address_space(1))) u4v gu4;
 typedef __attribute__((address_space(1))) const int gcint;
 
 __device__ __forceinline__ int gld_int(const int32_t *p) { return *(gcint *)p; }
@@ -69,36 +70,58 @@ __device__ __forceinline__ int gld_int_nt(const int32_t *p) {
 
 __device__ __forceinline__ u4v gld_u4(const int32_t *p) { return *(gcu4 *)p; }
 
-__device__ __forceinline__ void load8_f32(const float *p, float (&o)[8]) {
-  const f4v a = ((gcf4 *)p)[0];
-  const f4v b = ((gcf4 *)p)[1];
-  o[0] = a.x; o[1] = a.y; o[2] = a.z; o[3] = a.w;
-  o[4] = b.x; o[5] = b.y; o[6] = b.z; o[7] = b.w;
+// A lane's slice of a row: C consecutive components (C = 8 or 16), so a row
+// of D components is G = D / C lanes. The slice helpers below move C floats
+// (C/4 16-byte accesses) or C BF16 values (C/8 16-byte accesses).
+template <int C>
+__device__ __forceinline__ void load_f32(const float *p, float (&o)[C]) {
+#pragma unroll
+  for (int i = 0; i < C / 4; ++i) {
+    const f4v a = ((gcf4 *)p)[i];
+    o[4 * i] = a.x; o[4 * i + 1] = a.y; o[4 * i + 2] = a.z; o[4 * i + 3] = a.w;
+  }
 }
 
-__device__ __forceinline__ void load8_bf16(const uint16_t *p, float (&o)[8]) {
-  const u4v w = *(gcu4 *)p;
-  o[0] = __uint_as_float(w.x << 16); o[1] = __uint_as_float(w.x & 0xFFFF0000u);
-  o[2] = __uint_as_float(w.y << 16); o[3] = __uint_as_float(w.y & 0xFFFF0000u);
-  o[4] = __uint_as_float(w.z << 16); o[5] = __uint_as_float(w.z & 0xFFFF0000u);
-  o[6] = __uint_as_float(w.w << 16); o[7] = __uint_as_float(w.w & 0xFFFF0000u);
+// C BF16 values (C/8 16-byte words) as FP32: the low half of each 32-bit
+// lane word is the even component.
+template <int C>
+__device__ __forceinline__ void unpack_bf16(const u4v (&w)[C / 8], float (&o)[C]) {
+#pragma unroll
+  for (int i = 0; i < C / 8; ++i) {
+    o[8 * i + 0] = __uint_as_float(w[i].x << 16); o[8 * i + 1] = __uint_as_float(w[i].x & 0xFFFF0000u);
+    o[8 * i + 2] = __uint_as_float(w[i].y << 16); o[8 * i + 3] = __uint_as_float(w[i].y & 0xFFFF0000u);
+    o[8 * i + 4] = __uint_as_float(w[i].z << 16); o[8 * i + 5] = __uint_as_float(w[i].z & 0xFFFF0000u);
+    o[8 * i + 6] = __uint_as_float(w[i].w << 16); o[8 * i + 7] = __uint_as_float(w[i].w & 0xFFFF0000u);
+  }
 }
 
-// Global memory only (never an LDS pointer: see store8_lds).
-__device__ __forceinline__ void store8_f32(float *p, const float (&o)[8]) {
-  f4v a = {o[0], o[1], o[2], o[3]};
-  f4v b = {o[4], o[5], o[6], o[7]};
-  ((gf4 *)p)[0] = a;
-  ((gf4 *)p)[1] = b;
+template <int C>
+__device__ __forceinline__ void load_bf16(const uint16_t *p, float (&o)[C]) {
+  u4v w[C / 8];
+#pragma unroll
+  for (int i = 0; i < C / 8; ++i) w[i] = ((gcu4 *)p)[i];
+  unpack_bf16<C>(w, o);
+}
+
+// Global memory only (never an LDS pointer: see store_lds).
+template <int C>
+__device__ __forceinline__ void store_f32(float *p, const float (&o)[C]) {
+#pragma unroll
+  for (int i = 0; i < C / 4; ++i) {
+    f4v a = {o[4 * i], o[4 * i + 1], o[4 * i + 2], o[4 * i + 3]};
+    ((gf4 *)p)[i] = a;
+  }
 }
 
 // Streaming store of a new row (nontemporal): the rows are read back only by
 // later launches, so they need not stay in this launch's L2.
-__device__ __forceinline__ void store8_f32_nt(float *p, const float (&o)[8]) {
-  f4v a = {o[0], o[1], o[2], o[3]};
-  f4v b = {o[4], o[5], o[6], o[7]};
-  __builtin_nontemporal_store(a, (gf4 *)p);
-  __builtin_nontemporal_store(b, (gf4 *)p + 1);
+template <int C>
+__device__ __forceinline__ void store_f32_nt(float *p, const float (&o)[C]) {
+#pragma unroll
+  for (int i = 0; i < C / 4; ++i) {
+    f4v a = {o[4 * i], o[4 * i + 1], o[4 * i + 2], o[4 * i + 3]};
+    __builtin_nontemporal_store(a, (gf4 *)p + i);
+  }
 }
 
 // A row's 8G floats, held 8 per lane by the G lanes of a group (lane q:
@@ -112,7 +135,7 @@ __device__ __forceinline__ void store8_f32_nt(float *p, const float (&o)[8]) {
 __device__ __forceinline__ void store8_f32_nt_lines(float *row, int q, int G, int first,
                                                     const float (&o)[8]) {
   if (G < 2) {
-    store8_f32_nt(row, o);
+    store_f32_nt(row, o);
     return;
   }
   const int h = q & 1;
@@ -133,11 +156,13 @@ __device__ __forceinline__ void store8_f32_nt_lines(float *row, int q, int G, in
 
 typedef __attribute__((address_space(3))) f4v lf4;
 
-__device__ __forceinline__ void store8_lds(float *p, const float (&o)[8]) {
-  f4v a = {o[0], o[1], o[2], o[3]};
-  f4v b = {o[4], o[5], o[6], o[7]};
-  ((lf4 *)p)[0] = a;
-  ((lf4 *)p)[1] = b;
+template <int C>
+__device__ __forceinline__ void store_lds(float *p, const float (&o)[C]) {
+#pragma unroll
+  for (int i = 0; i < C / 4; ++i) {
+    f4v a = {o[4 * i], o[4 * i + 1], o[4 * i + 2], o[4 * i + 3]};
+    ((lf4 *)p)[i] = a;
+  }
 }
 
 // Partials of heavy rows cross CUs (and XCDs, whose L2s are not coherent with
@@ -147,22 +172,24 @@ __device__ __forceinline__ void store8_lds(float *p, const float (&o)[8]) {
 typedef __attribute__((address_space(1))) float gfloat;
 typedef __attribute__((address_space(1))) int gint;
 
-// A heavy-row partial (D = 8G floats) in component-major order: lane q of
-// the row's group keeps its component k at base[k*G + q], so each of the 8
+// A heavy-row partial (D = C*G floats) in component-major order: lane q of
+// the row's group keeps its component k at base[k*G + q], so each of the C
 // device-coherent 4-byte accesses of the group covers one contiguous span
-// of 4G bytes. (The row layout, 8 consecutive floats per lane, would make
-// every such access write 4 bytes out of each 32 and pass the whole row to
-// memory 8 times over.)
-__device__ __forceinline__ void store8_coherent(float *base, int q, int G, const float (&o)[8]) {
+// of 4G bytes. (The row layout, C consecutive floats per lane, would make
+// every such access write 4 bytes out of each 4C and pass the whole row to
+// memory C times over.)
+template <int C>
+__device__ __forceinline__ void store_coherent(float *base, int q, int G, const float (&o)[C]) {
   gfloat *g = (gfloat *)base + q;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) __hip_atomic_store(g + k * G, o[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int k = 0; k < C; ++k) __hip_atomic_store(g + k * G, o[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__device__ __forceinline__ void load8_coherent(const float *base, int q, int G, float (&o)[8]) {
+template <int C>
+__device__ __forceinline__ void load_coherent(const float *base, int q, int G, float (&o)[C]) {
   gfloat *g = (gfloat *)base + q;
 #pragma unroll
-  for (int k = 0; k < 8; ++k) o[k] = __hip_atomic_load(g + k * G, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int k = 0; k < C; ++k) o[k] = __hip_atomic_load(g + k * G, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Counts this wave's chunk partial of a heavy row (just stored by its lanes
@@ -262,26 +289,26 @@ __device__ __forceinline__ uint32_t row_pos(const SlotMap &m, uint32_t c) {
   return m.b_shift >= 0 ? row_pos_t<true>(m, c) : row_pos_t<false>(m, c);
 }
 
-// Row c as 8 FP32 components [8q, 8q+8) (BF16 gathers: the shadow's values).
-template <int MODE>
+// Row c as C FP32 components [Cq, Cq+C) (BF16 gathers: the shadow's values).
+template <int MODE, int C>
 __device__ __forceinline__ void load_row(const StepDesc &sd, const SlotMap &m, int c, int q,
-                                         float (&o)[8]) {
+                                         float (&o)[C]) {
   const size_t pos = row_pos(m, (uint32_t)c);
-  if (use_bf16<MODE>(sd)) load8_bf16(sd.Xs + pos * sd.D + 8 * q, o);
-  else load8_f32(sd.X + pos * sd.D + 8 * q, o);
+  if (use_bf16<MODE>(sd)) load_bf16(sd.Xs + pos * sd.D + C * q, o);
+  else load_f32(sd.X + pos * sd.D + C * q, o);
 }
 
-// A gathered row slice held while its load is in flight: 8 FP32 registers,
-// or with BF16 gathers the 8 packed BF16 values (4 registers), unpacked only
-// when used -- twice the rows in flight for the same registers.
-template <int MODE> struct RowBuf {
-  float f[8];
+// A gathered row slice held while its load is in flight: C FP32 registers,
+// or with BF16 gathers the C packed BF16 values (C/2 registers), unpacked
+// only when used -- twice the rows in flight for the same registers.
+template <int MODE, int C = 8> struct RowBuf {
+  float f[C];
 };
-template <> struct RowBuf<1> {
-  u4v w;
+template <int C> struct RowBuf<1, C> {
+  u4v w[C / 8];
 };
 
-// Where a lane reads gathered rows from: its 8-component slice of X (or of
+// Where a lane reads gathered rows from: its C-component slice of X (or of
 // the BF16 shadow) at the rows' slot positions. The address is arithmetic
 // only (no branch), so the U loads of a round issue back to back.
 struct RowSrc {
@@ -291,17 +318,17 @@ struct RowSrc {
   bool bf;             // gathers read BF16
 };
 
-template <int MODE>
+template <int MODE, int C = 8>
 __device__ __forceinline__ RowSrc make_src(const StepDesc &sd, const SlotMap &m, int q, int G) {
   RowSrc r;
   r.bf = use_bf16<MODE>(sd);
-  // D = 8G floats per row
-  const int dshift = 3 + (31 - __builtin_clz((unsigned)G));
+  // D = C*G floats per row
+  const int dshift = (C == 16 ? 4 : 3) + (31 - __builtin_clz((unsigned)G));
   if (r.bf) {
-    r.xq = reinterpret_cast<const char *>(sd.Xs + 8 * q);
+    r.xq = reinterpret_cast<const char *>(sd.Xs + C * q);
     r.row_shift = dshift + 1;
   } else {
-    r.xq = reinterpret_cast<const char *>(sd.X + 8 * q);
+    r.xq = reinterpret_cast<const char *>(sd.X + C * q);
     r.row_shift = dshift + 2;
   }
   r.m = m;
@@ -309,29 +336,27 @@ __device__ __forceinline__ RowSrc make_src(const StepDesc &sd, const SlotMap &m,
 }
 
 // The lane's slice of the row at slot position pos.
-template <int MODE>
-__device__ __forceinline__ void load_at(const RowSrc &src, uint32_t pos, RowBuf<MODE> &b) {
+template <int MODE, int C>
+__device__ __forceinline__ void load_at(const RowSrc &src, uint32_t pos, RowBuf<MODE, C> &b) {
   const char *p = src.xq + ((uint64_t)pos << src.row_shift);
   if constexpr (MODE == 1) {
-    b.w = *(gcu4 *)p;
+#pragma unroll
+    for (int i = 0; i < C / 8; ++i) b.w[i] = ((gcu4 *)p)[i];
   } else if constexpr (MODE == 0) {
-    load8_f32(reinterpret_cast<const float *>(p), b.f);
+    load_f32(reinterpret_cast<const float *>(p), b.f);
   } else {
-    if (src.bf) load8_bf16(reinterpret_cast<const uint16_t *>(p), b.f);
-    else load8_f32(reinterpret_cast<const float *>(p), b.f);
+    if (src.bf) load_bf16(reinterpret_cast<const uint16_t *>(p), b.f);
+    else load_f32(reinterpret_cast<const float *>(p), b.f);
   }
 }
 
-template <int MODE>
-__device__ __forceinline__ void unpack_buf(const RowBuf<MODE> &b, float (&o)[8]) {
+template <int MODE, int C>
+__device__ __forceinline__ void unpack_buf(const RowBuf<MODE, C> &b, float (&o)[C]) {
   if constexpr (MODE == 1) {
-    o[0] = __uint_as_float(b.w.x << 16); o[1] = __uint_as_float(b.w.x & 0xFFFF0000u);
-    o[2] = __uint_as_float(b.w.y << 16); o[3] = __uint_as_float(b.w.y & 0xFFFF0000u);
-    o[4] = __uint_as_float(b.w.z << 16); o[5] = __uint_as_float(b.w.z & 0xFFFF0000u);
-    o[6] = __uint_as_float(b.w.w << 16); o[7] = __uint_as_float(b.w.w & 0xFFFF0000u);
+    unpack_bf16<C>(b.w, o);
   } else {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) o[k] = b.f[k];
+    for (int k = 0; k < C; ++k) o[k] = b.f[k];
   }
 }
 
@@ -345,38 +370,53 @@ __device__ __forceinline__ void unpack_buf(const RowBuf<MODE> &b, float (&o)[8])
 // Repulsive forces whose clamp binds nowhere in the wave (s d1^2 <= bound^2,
 // so |v_k d1| <= sqrt(s) d1 <= bound, the usual case away from the row
 // itself) take one fused multiply-add per component with lr*d1.
-template <bool ATTR>
-__device__ __forceinline__ void add_force(const float (&v)[8], float s, float lr, float bound,
-                                          float (&acc)[8]) {
+template <bool ATTR, int C>
+__device__ __forceinline__ void add_force(const float (&v)[C], float s, float lr, float bound,
+                                          float (&acc)[C]) {
   if (__builtin_amdgcn_ballot_w64(!__builtin_isfinite(s)) == 0) {
     const float d1 = ATTR ? -2.0f * __builtin_amdgcn_rcpf(1.0f + s)
                           : 2.0f * __builtin_amdgcn_rcpf((s + 1e-6f) * (1.0f + s));
     if (!ATTR && __builtin_amdgcn_ballot_w64(!(s * d1 * d1 <= bound * bound)) == 0) {
       const float c = lr * d1;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) acc[k] = fmaf(c, v[k], acc[k]);
+      for (int k = 0; k < C; ++k) acc[k] = fmaf(c, v[k], acc[k]);
       return;
     }
 #pragma unroll
-    for (int k = 0; k < 8; ++k)
+    for (int k = 0; k < C; ++k)
       acc[k] = fmaf(lr, __builtin_amdgcn_fmed3f(v[k] * d1, -bound, bound), acc[k]);
   } else {
     const float d1 = ATTR ? -2.0f / (1.0f + s) : 2.0f / ((s + 1e-6f) * (1.0f + s));
 #pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] = fmaf(lr, clampf(v[k] * d1, bound), acc[k]);
+    for (int k = 0; k < C; ++k) acc[k] = fmaf(lr, clampf(v[k] * d1, bound), acc[k]);
   }
 }
 
-// One gathered slice against the row's own slice xd.
-template <bool ATTR>
-__device__ __forceinline__ void interact(const float (&xd)[8], const float (&xc)[8], int G,
-                                         float lr, float bound, float (&acc)[8]) {
-  float v[8];
+// v = xd - xc and the lane's part of |v|^2: a serial chain of fused
+// multiply-adds over the lane's C components.
+template <int C>
+__device__ __forceinline__ float diff_sq(const float (&xd)[C], const float (&xc)[C],
+                                         float (&v)[C]) {
 #pragma unroll
-  for (int k = 0; k < 8; ++k) v[k] = xd[k] - xc[k];
+  for (int k = 0; k < C; ++k) v[k] = xd[k] - xc[k];
   float p = v[0] * v[0];
 #pragma unroll
-  for (int k = 1; k < 8; ++k) p = fmaf(v[k], v[k], p);
+  for (int k = 1; k < C; ++k) p = fmaf(v[k], v[k], p);
+  return p;
+}
+
+// One gathered slice against the row's own slice xd. (Written out rather
+// than through diff_sq: the helper made the compiler keep 4 more VGPRs in
+// the repulsion, which cost the two-rows-in-flight BF16 kernel a wave.)
+template <bool ATTR, int C>
+__device__ __forceinline__ void interact(const float (&xd)[C], const float (&xc)[C], int G,
+                                         float lr, float bound, float (&acc)[C]) {
+  float v[C];
+#pragma unroll
+  for (int k = 0; k < C; ++k) v[k] = xd[k] - xc[k];
+  float p = v[0] * v[0];
+#pragma unroll
+  for (int k = 1; k < C; ++k) p = fmaf(v[k], v[k], p);
   add_force<ATTR>(v, group_sum(p, G), lr, bound, acc);
 }
 
@@ -390,34 +430,31 @@ __device__ __forceinline__ void interact(const float (&xd)[8], const float (&xc)
 // (inf * -0), as v*d1 with d1 = -2/inf does; s = NaN gives NaN throughout.
 // FAST is bound >= 2 (uniform per step); a smaller bound takes the exact
 // expressions of add_force.
-template <int U, int MODE, bool FAST>
-__device__ __forceinline__ void attract_round(const float (&xd)[8], const RowBuf<MODE> (&xb)[U],
-                                              int G, float lr, float bound, float (&acc)[8]) {
+template <int U, int MODE, bool FAST, int C>
+__device__ __forceinline__ void attract_round(const float (&xd)[C],
+                                              const RowBuf<MODE, C> (&xb)[U], int G, float lr,
+                                              float bound, float (&acc)[C]) {
   if constexpr (FAST) {
     const float c2 = -2.0f * lr;
 #pragma unroll
     for (int uu = 0; uu < U; ++uu) {
-      float xc[8], v[8];
+      float xc[C], v[C];
       unpack_buf<MODE>(xb[uu], xc);
-#pragma unroll
-      for (int k = 0; k < 8; ++k) v[k] = xd[k] - xc[k];
-      float p = v[0] * v[0];
-#pragma unroll
-      for (int k = 1; k < 8; ++k) p = fmaf(v[k], v[k], p);
+      const float p = diff_sq(xd, xc, v);
       const float c = c2 * __builtin_amdgcn_rcpf(1.0f + group_sum(p, G));
 #pragma unroll
-      for (int k = 0; k < 8; ++k) acc[k] = fmaf(c, v[k], acc[k]);
+      for (int k = 0; k < C; ++k) acc[k] = fmaf(c, v[k], acc[k]);
     }
   } else {
 #pragma unroll
     for (int uu = 0; uu < U; ++uu) {
-      float xc[8], v[8];
+      float xc[C], v[C];
       unpack_buf<MODE>(xb[uu], xc);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) v[k] = xd[k] - xc[k];
+      for (int k = 0; k < C; ++k) v[k] = xd[k] - xc[k];
       float p = v[0] * v[0];
 #pragma unroll
-      for (int k = 1; k < 8; ++k) p = fmaf(v[k], v[k], p);
+      for (int k = 1; k < C; ++k) p = fmaf(v[k], v[k], p);
       add_force<true>(v, group_sum(p, G), lr, bound, acc);
     }
   }
@@ -425,19 +462,30 @@ __device__ __forceinline__ void attract_round(const float (&xd)[8], const RowBuf
 
 // Staging piece p (negative p / G, slice p % G) of the step's negative rows,
 // as FP32 in LDS; id is that negative's row.
-template <int MODE>
+template <int MODE, int C>
 __device__ __forceinline__ void stage_piece(const StepDesc &sd, const SlotMap &m, int p, int id,
                                             int G, float *lds) {
   const int k = p / G;
   const int q = p - k * G;
-  float v[8];
+  float v[C];
   load_row<MODE>(sd, m, id, q, v);
-  store8_lds(lds + (size_t)k * sd.D + 8 * q, v);
+  store_lds(lds + (size_t)k * sd.D + C * q, v);
+}
+
+// The lane's slice of staged negative k (LDS).
+template <int C>
+__device__ __forceinline__ void lds_slice(const float *pn, float (&o)[C]) {
+#pragma unroll
+  for (int i = 0; i < C / 4; ++i) {
+    const float4 a = *reinterpret_cast<const float4 *>(pn + 4 * i);
+    o[4 * i] = a.x; o[4 * i + 1] = a.y; o[4 * i + 2] = a.z; o[4 * i + 3] = a.w;
+  }
 }
 
 // Repulsive forces of the staged negatives on x_r, spread over the groups.
+template <int C>
 __device__ __forceinline__ void repulse(const StepDesc &sd, const float *lds,
-                                        const float (&xr)[8], float (&acc)[8],
+                                        const float (&xr)[C], float (&acc)[C],
                                         int G, int NG, int g, int q) {
   const int ns = sd.ns;
   const int rounds = (ns + NG - 1) / NG;
@@ -446,49 +494,49 @@ __device__ __forceinline__ void repulse(const StepDesc &sd, const float *lds,
     const int k = u * NG + g;
     const bool ok = k < ns;
     // a group without a negative this round meets the row itself: v = 0
-    float xn[8];
+    float xn[C];
     if (ok) {
-      const float *pn = lds + k * sd.D + 8 * q;
-      const float4 a = *reinterpret_cast<const float4 *>(pn);
-      const float4 b = *reinterpret_cast<const float4 *>(pn + 4);
-      xn[0] = a.x; xn[1] = a.y; xn[2] = a.z; xn[3] = a.w;
-      xn[4] = b.x; xn[5] = b.y; xn[6] = b.z; xn[7] = b.w;
+      lds_slice(lds + k * sd.D + C * q, xn);
     } else {
 #pragma unroll
-      for (int k2 = 0; k2 < 8; ++k2) xn[k2] = xr[k2];
+      for (int k2 = 0; k2 < C; ++k2) xn[k2] = xr[k2];
     }
     interact<false>(xr, xn, G, lr, bound, acc);
   }
 }
 
-__device__ __forceinline__ void combine_groups(float (&acc)[8], int G) {
+template <int C>
+__device__ __forceinline__ void combine_groups(float (&acc)[C], int G) {
   for (int m = G; m < 64; m <<= 1) {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) acc[k] += __shfl_xor(acc[k], m);
+    for (int k = 0; k < C; ++k) acc[k] += __shfl_xor(acc[k], m);
   }
 }
 
 // Sum partials p0..p0+k-1: group g takes g, g+NG, ... in order, then the
-// fixed xor tree of combine_groups.
-template <int U>
+// fixed xor tree of combine_groups. UP partials are in flight per group
+// (the order of the additions does not depend on it).
+template <int U, int C>
 __device__ __forceinline__ void sum_partials(const StepDesc &sd, int p0, int k, int NG, int g,
-                                             int q, float (&acc)[8]) {
+                                             int q, float (&acc)[C]) {
+  constexpr int UP = C == 8 ? U : (U > 2 ? 2 : U);
   const int D = sd.D;
+  const int G = D / C;
   const int rounds = (k + NG - 1) / NG;
-  for (int u0 = 0; u0 < rounds; u0 += U) {
-    float pv[U][8];
-    bool ok[U];
+  for (int u0 = 0; u0 < rounds; u0 += UP) {
+    float pv[UP][C];
+    bool ok[UP];
 #pragma unroll
-    for (int uu = 0; uu < U; ++uu) {
+    for (int uu = 0; uu < UP; ++uu) {
       const int idx = (u0 + uu) * NG + g;
       ok[uu] = idx < k;
-      if (ok[uu]) load8_coherent(sd.P + (size_t)(p0 + idx) * D, q, D >> 3, pv[uu]);
+      if (ok[uu]) load_coherent(sd.P + (size_t)(p0 + idx) * D, q, G, pv[uu]);
     }
 #pragma unroll
-    for (int uu = 0; uu < U; ++uu) {
+    for (int uu = 0; uu < UP; ++uu) {
       if (ok[uu]) {
 #pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2) acc[k2] += pv[uu][k2];
+        for (int k2 = 0; k2 < C; ++k2) acc[k2] += pv[uu][k2];
       }
     }
   }
@@ -502,10 +550,10 @@ __device__ __forceinline__ void sum_partials(const StepDesc &sd, int p0, int k, 
 // a slot then fetches the position from lane k (ds_bpermute takes the lane
 // index modulo 64). Lanes k in [m, 64) hold row r; k >= 64 happens only for
 // U > G.
-template <int U, int MODE, bool FAST>
+template <int U, int MODE, bool FAST, int C>
 __device__ __forceinline__ void gather_edges_t(const StepDesc &sd, const RowSrc &src,
-                                             const float (&xd)[8], int r, int e0, int cnt,
-                                             int G, int NG, int g, int lane, float (&acc)[8]) {
+                                             const float (&xd)[C], int r, int e0, int cnt,
+                                             int G, int NG, int g, int lane, float (&acc)[C]) {
   const float lr = sd.lr, bound = sd.bound;
   const int c_limit = sd.c_limit;
   const uint32_t pos_r = row_pos(src.m, (uint32_t)r);
@@ -516,7 +564,7 @@ __device__ __forceinline__ void gather_edges_t(const StepDesc &sd, const RowSrc 
     const int pl = (int)row_pos(src.m, (uint32_t)cl);
     const int rounds = (m + NG - 1) / NG;
     for (int u0 = 0; u0 < rounds; u0 += U) {
-      RowBuf<MODE> xb[U];
+      RowBuf<MODE, C> xb[U];
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
         const int k = (u0 + uu) * NG + g;
@@ -535,57 +583,60 @@ __device__ __forceinline__ void gather_edges_t(const StepDesc &sd, const RowSrc 
 // shadow holds it when gathers read BF16 (no extra traffic), so a row meeting
 // itself as a negative or through a self-loop feels exactly zero force, as
 // in the reference; the update still goes to the FP32 master.
-template <int MODE>
-__device__ __forceinline__ void own_slice(const StepDesc &sd, const float (&xr)[8],
-                                          float (&xd)[8]) {
+template <int MODE, int C>
+__device__ __forceinline__ void own_slice(const StepDesc &sd, const float (&xr)[C],
+                                          float (&xd)[C]) {
   if (use_bf16<MODE>(sd)) {
 #pragma unroll
-    for (int k = 0; k < 8; k += 2) {
+    for (int k = 0; k < C; k += 2) {
       const uint32_t w = bf16x2(xr[k], xr[k + 1]);
       xd[k] = __uint_as_float(w << 16);
       xd[k + 1] = __uint_as_float(w & 0xFFFF0000u);
     }
   } else {
 #pragma unroll
-    for (int k = 0; k < 8; ++k) xd[k] = xr[k];
+    for (int k = 0; k < C; ++k) xd[k] = xr[k];
   }
 }
 
 // Row t of the batch: a fused step writes the new row straight into the
 // free slot (position out + t) of X and, with BF16 gathers, of the shadow;
 // an unfused step writes the new row or the delta to S.
-template <int MODE>
+template <int MODE, int C>
 __device__ __forceinline__ void write_row(const StepDesc &sd, uint32_t out, int t, int q,
-                                          const float (&o)[8]) {
+                                          const float (&o)[C]) {
   if (!sd.fused) {
-    store8_f32(sd.S + (size_t)t * sd.D + 8 * q, o);
+    store_f32(sd.S + (size_t)t * sd.D + C * q, o);
     return;
   }
   const size_t pos = (size_t)(out + (uint32_t)t);
-  store8_f32_nt(sd.Xw + pos * sd.D + 8 * q, o);
+  store_f32_nt(sd.Xw + pos * sd.D + C * q, o);
   if (use_bf16<MODE>(sd)) {
-    u4v w;
-    w.x = bf16x2(o[0], o[1]);
-    w.y = bf16x2(o[2], o[3]);
-    w.z = bf16x2(o[4], o[5]);
-    w.w = bf16x2(o[6], o[7]);
-    __builtin_nontemporal_store(w, (__attribute__((address_space(1))) u4v *)(sd.Xsw + pos * sd.D +
-                                                                          8 * q));
+    gu4 *ps = (gu4 *)(sd.Xsw + pos * sd.D + C * q);
+#pragma unroll
+    for (int i = 0; i < C / 8; ++i) {
+      u4v w;
+      w.x = bf16x2(o[8 * i], o[8 * i + 1]);
+      w.y = bf16x2(o[8 * i + 2], o[8 * i + 3]);
+      w.z = bf16x2(o[8 * i + 4], o[8 * i + 5]);
+      w.w = bf16x2(o[8 * i + 6], o[8 * i + 7]);
+      __builtin_nontemporal_store(w, ps + i);
+    }
   }
 }
 
 // Repulsion (first `rep` rows), group combine, new row (or delta) to S.
-template <int MODE>
+template <int MODE, int C>
 __device__ __forceinline__ void finish_row(const StepDesc &sd, uint32_t out, const float *lds_neg,
-                                           bool repel, int t, const float (&xr)[8],
-                                           const float (&xd)[8], float (&acc)[8], int G, int NG,
+                                           bool repel, int t, const float (&xr)[C],
+                                           const float (&xd)[C], float (&acc)[C], int G, int NG,
                                            int g, int q) {
   if (repel) repulse(sd, lds_neg, xd, acc, G, NG, g, q);
   combine_groups(acc, G);
   if (g == 0) {
-    float o[8];
+    float o[C];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
+    for (int k = 0; k < C; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
     write_row<MODE>(sd, out, t, q, o);
   }
 }
@@ -595,10 +646,10 @@ __device__ __forceinline__ void finish_row(const StepDesc &sd, uint32_t out, con
 // against every staged negative and writes its row. The loops run to the
 // pack's largest degree kmax, so control flow stays uniform; slots past a
 // row's degree read the row itself (no force).
-template <int U, int MODE, bool FAST>
+template <int U, int MODE, bool FAST, int C>
 __device__ __forceinline__ void light_pack_t(const StepDesc &sd, const RowSrc &src, uint32_t out,
                                            u4v ia, int lidx, int pc, int kmax, bool valid,
-                                           const float (&xr)[8], const float *lds_neg,
+                                           const float (&xr)[C], const float *lds_neg,
                                            bool has_neg, int rep, int G, int g, int q,
                                            bool &synced) {
   const int r = (int)ia.x, deg = valid ? (int)ia.z : 0;
@@ -606,10 +657,10 @@ __device__ __forceinline__ void light_pack_t(const StepDesc &sd, const RowSrc &s
   kmax = __builtin_amdgcn_readfirstlane(kmax);
   const float lr = sd.lr, bound = sd.bound;
   const int c_limit = sd.c_limit;
-  float xd[8], acc[8];
+  float xd[C], acc[C];
   own_slice<MODE>(sd, xr, xd);
 #pragma unroll
-  for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+  for (int k = 0; k < C; ++k) acc[k] = 0.f;
   // slot positions as in gather_edges_t: lane q of the group holds the
   // position of edge kb+q (row r past the degree); k - kb >= G only for U > G
   const uint32_t pos_r = row_pos(src.m, (uint32_t)r);
@@ -620,7 +671,7 @@ __device__ __forceinline__ void light_pack_t(const StepDesc &sd, const RowSrc &s
     const int pl = (int)row_pos(src.m, (uint32_t)cl);
     const int kend = min(kmax, kb + G);
     for (int k0 = kb; k0 < kend; k0 += U) {
-      RowBuf<MODE> xb[U];
+      RowBuf<MODE, C> xb[U];
 #pragma unroll
       for (int uu = 0; uu < U; ++uu) {
         const int k = k0 + uu;
@@ -642,42 +693,38 @@ __device__ __forceinline__ void light_pack_t(const StepDesc &sd, const RowSrc &s
     // rows past the repulsive count (or an empty slot) meet themselves: v = 0
     const bool repel = valid && t < rep;
     for (int n = 0; n < sd.ns; ++n) {
-      float xn[8];
+      float xn[C];
       if (repel) {
-        const float *pn = lds_neg + (size_t)n * sd.D + 8 * q;
-        const float4 a = *reinterpret_cast<const float4 *>(pn);
-        const float4 b = *reinterpret_cast<const float4 *>(pn + 4);
-        xn[0] = a.x; xn[1] = a.y; xn[2] = a.z; xn[3] = a.w;
-        xn[4] = b.x; xn[5] = b.y; xn[6] = b.z; xn[7] = b.w;
+        lds_slice(lds_neg + (size_t)n * sd.D + C * q, xn);
       } else {
 #pragma unroll
-        for (int k2 = 0; k2 < 8; ++k2) xn[k2] = xd[k2];
+        for (int k2 = 0; k2 < C; ++k2) xn[k2] = xd[k2];
       }
       interact<false>(xd, xn, G, lr, bound, acc);
     }
   }
   if (valid) {
-    float o[8];
+    float o[C];
 #pragma unroll
-    for (int k = 0; k < 8; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
+    for (int k = 0; k < C; ++k) o[k] = sd.delta_mode ? acc[k] : xr[k] + acc[k];
     write_row<MODE>(sd, out, t, q, o);
   }
 }
 
 // One uniform branch per work unit picks the single-FMA attractive form
 // (clamp bound >= 2).
-template <int U, int MODE>
+template <int U, int MODE, int C>
 __device__ __forceinline__ void gather_edges(const StepDesc &sd, const RowSrc &src,
-                                             const float (&xd)[8], int r, int e0, int cnt,
-                                             int G, int NG, int g, int lane, float (&acc)[8]) {
+                                             const float (&xd)[C], int r, int e0, int cnt,
+                                             int G, int NG, int g, int lane, float (&acc)[C]) {
   if (sd.bound >= 2.0f) gather_edges_t<U, MODE, true>(sd, src, xd, r, e0, cnt, G, NG, g, lane, acc);
   else gather_edges_t<U, MODE, false>(sd, src, xd, r, e0, cnt, G, NG, g, lane, acc);
 }
 
-template <int U, int MODE>
+template <int U, int MODE, int C>
 __device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src, uint32_t out,
                                            u4v ia, int lidx, int pc, bool valid,
-                                           const float (&xr)[8], const float *lds_neg,
+                                           const float (&xr)[C], const float *lds_neg,
                                            bool has_neg, int rep, int G, int g, int q,
                                            bool &synced) {
   const int kmax = (int)ia.w;
@@ -702,10 +749,11 @@ __device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src
 // after its gathers, a wave's after its first unit's gathers, or at its end
 // if it has no unit), so staging overlaps the first gathers. A fused step
 // writes its new rows into the layout's free slot, which no thread of the
-// launch reads. GT > 0: every step of the launch has G = GT lanes per row (a
-// compile-time constant); 0: per step. lb, nb: this block's index among the
-// step's blocks and their count.
-template <int U, int MODE, int GT>
+// launch reads. C: components per lane (the step's G = D / C lanes per row).
+// GT > 0: every step of the launch has G = GT (a compile-time constant);
+// 0: per step. lb, nb: this block's index among the step's blocks and their
+// count.
+template <int U, int MODE, int GT, int C>
 __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
   extern __shared__ float lds[];
   const int D = sd.D;
@@ -741,10 +789,10 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
     if (q < kLightCols) pc = gld_int_nt(sd.lcols + (size_t)li * kLightCols + q);
   }
   // this step's negative ids, one per staging piece (first blockDim pieces)
-  const int total8 = sd.ns * G;
+  const int total_pieces = sd.ns * G;
   const int pz = threadIdx.x;
   int nid = 0;
-  if (pz < total8) nid = gld_int(sd.neg_ids + pz / G);
+  if (pz < total_pieces) nid = gld_int(sd.neg_ids + pz / G);
 
   // slot positions: a fused step reads with the run's rotation and writes
   // the free slot; an unfused step reads the layout at rest
@@ -760,50 +808,50 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
 
   // the own row of this wave's first unit, in flight with the negative rows
   const bool unit0 = lb < nbi || u < n_wave_items + n_packs;
-  float xr0[8];
-  if (unit0) load8_f32(sd.X + (size_t)(pre0.x + own_delta) * D + 8 * q, xr0);
+  float xr0[C];
+  if (unit0) load_f32(sd.X + (size_t)(pre0.x + own_delta) * D + C * q, xr0);
   const int rep = epoch == 0 ? sd.rep0 : sd.rep1;
   const bool has_neg = sd.ns > 0 && epoch + sd.batch < sd.n_lists;
   if (has_neg) {
-    if (pz < total8) stage_piece<MODE>(sd, m, pz, nid, G, lds);
-    for (int p = pz + blockDim.x; p < total8; p += blockDim.x)
-      stage_piece<MODE>(sd, m, p, gld_int(sd.neg_ids + p / G), G, lds);
+    if (pz < total_pieces) stage_piece<MODE, C>(sd, m, pz, nid, G, lds);
+    for (int p = pz + blockDim.x; p < total_pieces; p += blockDim.x)
+      stage_piece<MODE, C>(sd, m, p, gld_int(sd.neg_ids + p / G), G, lds);
   }
-  const RowSrc src = make_src<MODE>(sd, m, q, G);
+  const RowSrc src = make_src<MODE, C>(sd, m, q, G);
 
   if (lb < nbi) {
     const int r = (int)pre0.x, e0 = (int)pre0.y, cnt = (int)pre0.z, pslot = (int)pre0.w;
     const int t = r - sd.row_begin;
-    float xd[8], acc[8];
+    float xd[C], acc[C];
     own_slice<MODE>(sd, xr0, xd);
     float *lds_sub = lds + (has_neg ? sd.ns * D : 0);
     for (int s = wave; s < kSub; s += wpb) {
       const int lo = (int)((int64_t)cnt * s / kSub), hi = (int)((int64_t)cnt * (s + 1) / kSub);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+      for (int k = 0; k < C; ++k) acc[k] = 0.f;
       gather_edges<U, MODE>(sd, src, xd, r, e0 + lo, hi - lo, G, NG, g, lane, acc);
       combine_groups(acc, G);
-      if (g == 0) store8_lds(lds_sub + (size_t)s * D + 8 * q, acc);
+      if (g == 0) store_lds(lds_sub + (size_t)s * D + C * q, acc);
     }
     __syncthreads();
     if (wave == 0) {
       // group 0 holds the sum of the sub-ranges, in sub-range order
 #pragma unroll
-      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+      for (int k = 0; k < C; ++k) acc[k] = 0.f;
       if (g == 0) {
         for (int s = 0; s < kSub; ++s) {
-          const float *p = lds_sub + (size_t)s * D + 8 * q;
+          const float *p = lds_sub + (size_t)s * D + C * q;
 #pragma unroll
-          for (int k = 0; k < 8; ++k) acc[k] += p[k];
+          for (int k = 0; k < C; ++k) acc[k] += p[k];
         }
       }
       bool finish = pslot < 0;
       if (!finish) {
         const int sid = (int)pre1.x, p0 = (int)pre1.y, kparts = (int)pre1.z;
-        if (g == 0) store8_coherent(sd.P + (size_t)pslot * D, q, G, acc);
+        if (g == 0) store_coherent(sd.P + (size_t)pslot * D, q, G, acc);
         if (publish_partial(sd.counters + sid, kparts, lane)) {
 #pragma unroll
-          for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+          for (int k = 0; k < C; ++k) acc[k] = 0.f;
           sum_partials<U>(sd, p0, kparts, NG, g, q, acc);
           if (lane == 0)
             __hip_atomic_store((gint *)(sd.counters + sid), 0, __ATOMIC_RELAXED,
@@ -823,15 +871,15 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
     for (; u < n_wave_items; u += W) {
       if (!have) {
         pre0 = gld_u4(witems + (size_t)kItemInts * u);
-        load8_f32(sd.X + (size_t)(pre0.x + own_delta) * D + 8 * q, xr0);
+        load_f32(sd.X + (size_t)(pre0.x + own_delta) * D + C * q, xr0);
       }
       have = false;
       const int r = (int)pre0.x, e0 = (int)pre0.y, cnt = (int)pre0.z;
       const int t = r - sd.row_begin;
-      float xd[8], acc[8];
+      float xd[C], acc[C];
       own_slice<MODE>(sd, xr0, xd);
 #pragma unroll
-      for (int k = 0; k < 8; ++k) acc[k] = 0.f;
+      for (int k = 0; k < C; ++k) acc[k] = 0.f;
       gather_edges<U, MODE>(sd, src, xd, r, e0, cnt, G, NG, g, lane, acc);
       if (!synced) {
         __syncthreads();
@@ -845,7 +893,7 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
       if (!have) {
         pre0 = gld_u4(lrows + (size_t)kItemInts * li);
         if (q < kLightCols) pc = gld_int_nt(sd.lcols + (size_t)li * kLightCols + q);
-        load8_f32(sd.X + (size_t)(pre0.x + own_delta) * D + 8 * q, xr0);
+        load_f32(sd.X + (size_t)(pre0.x + own_delta) * D + C * q, xr0);
       }
       have = false;
       light_pack<U, MODE>(sd, src, out, pre0, li, pc, idx < n_light, xr0, lds, has_neg, rep, G,
@@ -858,23 +906,23 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
 // Steps of one or more jobs in one launch (engine rounds): the block finds
 // its step (one load from the block-to-step table of a round of more than
 // two steps, else a search of the block prefix) and reads the step's
-// descriptor from memory.
+// descriptor from memory. Slices of 8 components.
 template <int U, int MODE, int GT>
 __global__ __launch_bounds__(512) void force_kernel(
     const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
     const int *__restrict__ blk_step, int n_steps, int ipw) {
   const int b = blockIdx.x;
   const int si = n_steps > 2 ? blk_step[b] : find_step(blk_prefix, n_steps, b);
-  force_body<U, MODE, GT>(steps[si], b - blk_prefix[si], blk_prefix[si + 1] - blk_prefix[si]);
+  force_body<U, MODE, GT, 8>(steps[si], b - blk_prefix[si], blk_prefix[si + 1] - blk_prefix[si]);
   (void)ipw;
 }
 
 // One step (a single graph's minibatch): the descriptor travels in the
 // kernel arguments, so the first loads of the items, the columns and the
-// epoch state do not wait for a descriptor load.
-template <int U, int MODE, int GT>
+// epoch state do not wait for a descriptor load. C: components per lane.
+template <int U, int MODE, int GT, int C>
 __global__ __launch_bounds__(512) void force_kernel1(const StepDesc sd) {
-  force_body<U, MODE, GT>(sd, (int)blockIdx.x, (int)gridDim.x);
+  force_body<U, MODE, GT, C>(sd, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ---- SpMM sibling: SpMMAlgo::algo_spmm at world size 1 ----------------------
@@ -907,7 +955,7 @@ template <int MODE>
 __device__ __forceinline__ void load_or_zero(const RowSrc &src, uint32_t pos, bool ok,
                                              RowBuf<MODE> &b) {
   if constexpr (MODE == 1) {
-    b.w = u4v{0u, 0u, 0u, 0u};
+    b.w[0] = u4v{0u, 0u, 0u, 0u};
   } else {
 #pragma unroll
     for (int k = 0; k < 8; ++k) b.f[k] = 0.0f;
@@ -1016,7 +1064,7 @@ __global__ __launch_bounds__(512) void spmm_kernel(const StepDesc *__restrict__ 
       for (int k = 0; k < 8; ++k) acc[k] = 0.f;
       sum_edges<U, MODE>(sd, src, e0 + lo, hi - lo, G, NG, g, lane, acc);
       combine_groups(acc, G);
-      if (g == 0) store8_lds(lds + (size_t)s * D + 8 * q, acc);
+      if (g == 0) store_lds(lds + (size_t)s * D + 8 * q, acc);
     }
     __syncthreads();
     if (wave == 0) {
@@ -1032,7 +1080,7 @@ __global__ __launch_bounds__(512) void spmm_kernel(const StepDesc *__restrict__ 
       bool finish = pslot < 0;
       if (!finish) {
         const int sid = (int)it1.x, p0 = (int)it1.y, kparts = (int)it1.z;
-        if (g == 0) store8_coherent(sd.P + (size_t)pslot * D, q, G, acc);
+        if (g == 0) store_coherent(sd.P + (size_t)pslot * D, q, G, acc);
         if (publish_partial(sd.counters + sid, kparts, lane)) {
 #pragma unroll
           for (int k = 0; k < 8; ++k) acc[k] = 0.f;
@@ -1207,7 +1255,7 @@ __global__ __launch_bounds__(256) void loss_edges_kernel(LossDesc ld, double *bl
     const int e0 = ld.items[kItemInts * it + 1];
     const int cnt = ld.items[kItemInts * it + 2];
     float xr[8];
-    load8_f32(ld.X + (size_t)row_pos(mp, r) * D + 8 * q, xr);
+    load_f32(ld.X + (size_t)row_pos(mp, r) * D + 8 * q, xr);
     for (int eb = 0; eb < cnt; eb += 64) {
       const int m = min(64, cnt - eb);
       const int cl = lane < m ? ld.col[e0 + eb + lane] : 0;
@@ -1217,7 +1265,7 @@ __global__ __launch_bounds__(256) void loss_edges_kernel(LossDesc ld, double *bl
         const int c = __shfl(cl, k & 63);
         const bool ok = (k < m) && (c < ld.c_limit);
         float xc[8];
-        load8_f32(ld.X + (size_t)row_pos(mp, ok ? c : r) * D + 8 * q, xc);
+        load_f32(ld.X + (size_t)row_pos(mp, ok ? c : r) * D + 8 * q, xc);
         float p = 0.f;
 #pragma unroll
         for (int k2 = 0; k2 < 8; ++k2) {
@@ -1250,8 +1298,8 @@ __global__ __launch_bounds__(256) void loss_negs_kernel(LossDesc ld, double *blo
     const int k = ok ? (int)(p - r * ld.eval_ns) : 0;
     const int64_t m = ok ? eval_negative(ld.eval_seed, r, k, ld.n) : 0;
     float xr[8], xm[8];
-    load8_f32(ld.X + (size_t)row_pos(mp, (uint32_t)r) * D + 8 * q, xr);
-    load8_f32(ld.X + (size_t)row_pos(mp, (uint32_t)m) * D + 8 * q, xm);
+    load_f32(ld.X + (size_t)row_pos(mp, (uint32_t)r) * D + 8 * q, xr);
+    load_f32(ld.X + (size_t)row_pos(mp, (uint32_t)m) * D + 8 * q, xm);
     float s2 = 0.f;
 #pragma unroll
     for (int k2 = 0; k2 < 8; ++k2) {
@@ -1335,30 +1383,45 @@ static void launch_force_g(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t 
   else hipLaunchKernelGGL((force_kernel<U, 2, GT>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
 }
 
-template <int U, int GT>
+template <int U, int GT, int C>
 static void launch_force1_g(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
                             const StepDesc &sd) {
-  if (mode == 1) hipLaunchKernelGGL((force_kernel1<U, 1, GT>), grid, block, lds_bytes, st, sd);
-  else hipLaunchKernelGGL((force_kernel1<U, 0, GT>), grid, block, lds_bytes, st, sd);
+  if (mode == 1) hipLaunchKernelGGL((force_kernel1<U, 1, GT, C>), grid, block, lds_bytes, st, sd);
+  else hipLaunchKernelGGL((force_kernel1<U, 0, GT, C>), grid, block, lds_bytes, st, sd);
 }
 
+// compile-time G for the common widths d = 32, 64, 128 (slices of 8) and
+// d = 64, 128 (slices of 16)
 template <int U>
 static void launch_force1_u(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
-                            int g_const, const StepDesc &sd) {
-  if (g_const == 16) launch_force1_g<U, 16>(grid, block, lds_bytes, st, mode, sd);
-  else if (g_const == 8) launch_force1_g<U, 8>(grid, block, lds_bytes, st, mode, sd);
-  else if (g_const == 4) launch_force1_g<U, 4>(grid, block, lds_bytes, st, mode, sd);
-  else launch_force1_g<U, 0>(grid, block, lds_bytes, st, mode, sd);
+                            int g_const, int slice, const StepDesc &sd) {
+  if constexpr (U <= 4) {
+    if (slice == 16) {
+      if (g_const == 8) launch_force1_g<U, 8, 16>(grid, block, lds_bytes, st, mode, sd);
+      else if (g_const == 4) launch_force1_g<U, 4, 16>(grid, block, lds_bytes, st, mode, sd);
+      else launch_force1_g<U, 0, 16>(grid, block, lds_bytes, st, mode, sd);
+      return;
+    }
+  }
+  if (g_const == 16) launch_force1_g<U, 16, 8>(grid, block, lds_bytes, st, mode, sd);
+  else if (g_const == 8) launch_force1_g<U, 8, 8>(grid, block, lds_bytes, st, mode, sd);
+  else if (g_const == 4) launch_force1_g<U, 4, 8>(grid, block, lds_bytes, st, mode, sd);
+  else launch_force1_g<U, 0, 8>(grid, block, lds_bytes, st, mode, sd);
 }
 
+// Slices of 16 keep at most 4 rows in flight per lane group (8 would spill);
+// the unroll never changes results.
 hipError_t launch_force1(const StepDesc &sd, int n_blocks, int wpb, int unroll, size_t lds_bytes,
-                         int mode, int g_const, hipStream_t st) {
+                         int mode, int g_const, int slice, hipStream_t st) {
   dim3 grid(n_blocks), block(64 * wpb);
   switch (unroll) {
-  case 1: launch_force1_u<1>(grid, block, lds_bytes, st, mode, g_const, sd); break;
-  case 2: launch_force1_u<2>(grid, block, lds_bytes, st, mode, g_const, sd); break;
-  case 8: launch_force1_u<8>(grid, block, lds_bytes, st, mode, g_const, sd); break;
-  default: launch_force1_u<4>(grid, block, lds_bytes, st, mode, g_const, sd); break;
+  case 1: launch_force1_u<1>(grid, block, lds_bytes, st, mode, g_const, slice, sd); break;
+  case 2: launch_force1_u<2>(grid, block, lds_bytes, st, mode, g_const, slice, sd); break;
+  case 8:
+    if (slice == 16) launch_force1_u<4>(grid, block, lds_bytes, st, mode, g_const, slice, sd);
+    else launch_force1_u<8>(grid, block, lds_bytes, st, mode, g_const, slice, sd);
+    break;
+  default: launch_force1_u<4>(grid, block, lds_bytes, st, mode, g_const, slice, sd); break;
   }
   return hipGetLastError();
 }

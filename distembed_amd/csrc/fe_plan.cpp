@@ -30,7 +30,7 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
     p.last = (int)(n - (int64_t)B * (p.batches - 1));
   }
   for (int j = 0; cuts && j < p.batches; ++j) {
-    if (cuts[j].chunk <= 0 || cuts[j].max_chunks <= 0 || cuts[j].light < 0) {
+    if (cuts[j].chunk <= 0 || cuts[j].max_chunks <= 0 || cuts[j].light < 0 || cuts[j].pack < 0) {
       set_host_error("build_plan: invalid cutting");
       return FE_ERR_INVALID_ARG;
     }
@@ -65,6 +65,7 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
       const int chunk_j = cuts ? cuts[j].chunk : chunk;
       const int max_chunks_j = cuts ? cuts[j].max_chunks : max_chunks;
       const int light_j = std::min(std::min(cuts ? cuts[j].light : light, chunk_j), kLightCols);
+      const int pack_j = (cuts && cuts[j].pack > 0) ? cuts[j].pack : pack;
       const int64_t block_edges = (int64_t)kSub * chunk_j;
       const int64_t r0 = (int64_t)j * B;
       const int64_t r1 = std::min<int64_t>(r0 + B, n);
@@ -115,7 +116,7 @@ fe_status build_plan(int64_t n, const int64_t *row_ptr, const int32_t *col, int 
       std::sort(lights.begin(), lights.end(), longer);
       for (size_t i = 0; i < lights.size(); ++i) {
         const int32_t r = lights[i].second;
-        const int32_t pack_max = lights[(i / pack) * pack].first;
+        const int32_t pack_max = lights[(i / pack_j) * pack_j].first;
         // word 7: 1 + light index within the batch (the job-wide index below)
         o.items.insert(o.items.end(),
                        {r, (int32_t)row_ptr[r], lights[i].first, pack_max, 0, 0, 0, 1 + (int32_t)i});

@@ -66,9 +66,9 @@ enum {
 enum { FE_NEG_STRICT = 0, FE_NEG_REDRAW = 1 };
 
 /* Tiles chosen by the autotuner. A zero field keeps the current value
- * (light: a negative value sets 0). chunk, max_chunks and light change how
- * rows are cut into work and therefore the floating-point summation order;
- * unroll, waves_per_block and items_per_wave do not.                       */
+ * (light: a negative value sets 0). chunk, max_chunks, light and slice
+ * change how rows are cut into work and therefore the floating-point
+ * summation order; unroll, waves_per_block and items_per_wave do not.      */
 typedef struct fe_tuning {
   int chunk;           /* rows of degree <= chunk: one wave per row; up to
                           4*chunk: one block per row (<= 64)                */
@@ -77,7 +77,11 @@ typedef struct fe_tuning {
   int waves_per_block; /* 1, 2, 4 or 8                                      */
   int items_per_wave;  /* wave work units per wave, on average (>= 1)       */
   int light;           /* rows of degree <= light (<= 32): one row per lane
-                          group, 64/(d/8) rows per wave (rows per wavefront) */
+                          group, 64/(d/slice) rows per wave (rows per
+                          wavefront)                                        */
+  int slice;           /* components of a row per lane, 8 or 16 (<= d): a
+                          row takes d/slice lanes (the dimension-chunk tile;
+                          engine rounds use 8)                              */
 } fe_tuning;
 
 typedef struct fe_ctx fe_ctx;
