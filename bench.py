@@ -24,7 +24,9 @@ committed rocprofv3 FETCH_SIZE / WRITE_SIZE summary, when present.
 bf16_form: the same workload with BF16 gathers (FP32 master and
 accumulation), its speed-up and its final loss against the FP32 run, and
 against an FP32 run with the same tiling (the tiling alone moves the final
-loss of these chaotic runs; that spread is reported too).
+loss of these chaotic runs; that spread is reported too); loss_over_seeds:
+both forms from 8 further initial embeddings, the mean per-seed relative
+difference of the final losses and its standard error.
 cpu_baseline: the oracle's FP64 restatement of the reference loop (OpenMP,
 schedule(static)) on a bounded sample of the same workload.
 precision: R-MAT scale 20 and 22 at 5 and 10 negatives, FP32 and BF16
@@ -219,6 +221,57 @@ def run_form(api, g, X0, lists, gather, args, tuning=None):
             "loss_parts": (attr, rep), "use_graph": use_graph}
 
 
+def loss_over_seeds(api, g, lists, args, tun32, tun16, seeds):
+    """Final FP64 loss of the FP32 and the BF16 form after args.steps epochs
+    from each of several initial embeddings (seeds 1..seeds, same graph and
+    lists, each form with its tiling). One run's final loss is chaotic (the
+    summation order alone moves it by about a percent); the mean of the
+    per-seed paired differences, with its standard error, shows whether
+    the BF16 shadow shifts the loss."""
+    d, B, ns, lr = args.d, args.batch, args.ns, args.lr
+    ctxs = {}
+    rows = []
+    try:
+        for gather, tun in ((api.FP32, tun32), (api.BF16, tun16)):
+            ctx = api.ForceEmbed(g.n, d, gather=gather)
+            ctxs[gather] = ctx
+            ctx.upload_csr(g)
+            ctx.set_negatives(lists)
+            ctx.configure(B, lr, 5.0)
+            if tun:
+                ctx.set_tuning(**tun.get("tuning", {}))
+        for s in range(1, seeds + 1):
+            x0 = api.init_uniform(g.n, d, -0.5, 0.5, seed=s)
+            row = {"seed": s}
+            for gather, tun in ((api.FP32, tun32), (api.BF16, tun16)):
+                ctx = ctxs[gather]
+                ctx.upload_x(x0)
+                if s == 1 and tun and tun.get("per_batch"):
+                    ctx.set_batch_tunings(tun["per_batch"])
+                ctx.train(args.steps, B, ns, lr)
+                a, r = ctx.loss(0, ns)
+                row["bf16" if gather == api.BF16 else "fp32"] = a + r
+            del x0
+            row["rel_diff"] = (row["bf16"] - row["fp32"]) / row["fp32"]
+            rows.append(row)
+            log(f"loss seed {s}: fp32 {row['fp32']:.6e} bf16 {row['bf16']:.6e} "
+                f"({row['rel_diff'] * 100:+.2f}%)")
+    finally:
+        for ctx in ctxs.values():
+            ctx.close()
+    diffs = np.array([r["rel_diff"] for r in rows])
+    f32 = np.array([r["fp32"] for r in rows])
+    b16 = np.array([r["bf16"] for r in rows])
+    return {
+        "seeds": seeds, "epochs": args.steps,
+        "mean_rel_diff": float(diffs.mean()),
+        "stderr_rel_diff": float(diffs.std(ddof=1) / np.sqrt(len(diffs))) if len(diffs) > 1 else None,
+        "rel_diff_of_means": float((b16.mean() - f32.mean()) / f32.mean()),
+        "fp32_loss_cv_over_seeds": float(f32.std(ddof=1) / f32.mean()) if len(f32) > 1 else None,
+        "per_seed": rows,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -242,6 +295,9 @@ def main() -> int:
                          'waves_per_block, items_per_wave, chunk, max_chunks, light] per '
                          'batch]} (as reported in config.tuning), or @file holding it; '
                          'skips the autotuner')
+    ap.add_argument("--loss-seeds", type=int, default=8,
+                    help="initial embeddings over which the BF16 and FP32 final losses are "
+                         "compared (0: skip)")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-batching", action="store_true")
@@ -383,6 +439,9 @@ def main() -> int:
         out["bf16_form"]["fp32_loss_rel_diff_between_tilings"] = (
             (f32b["loss"] - f32["loss"]) / f32["loss"])
         log(f"fp32 with the bf16 tiling: loss {f32b['loss']:.6e}")
+        if args.loss_seeds > 0:
+            out["bf16_form"]["loss_over_seeds"] = loss_over_seeds(
+                api, g, lists, args, f32["tuning"], b16["tuning"], args.loss_seeds)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, lists, d, B, ns, lr, args.cpu_budget,
                                            args.cpu_epochs)
