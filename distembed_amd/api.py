@@ -387,7 +387,7 @@ class Engine:
     def set_tuning(self, chunk=None, max_chunks=None, unroll=None, waves_per_block=None,
                    items_per_wave=None, light=None, slice=None) -> None:
         """As ForceEmbed.set_tuning; chunking applies to jobs submitted later
-        (engine rounds take slices of 8 only)."""
+        (a new slice needs an engine that holds no job)."""
         t = _tuning(chunk, max_chunks, unroll, waves_per_block, items_per_wave, light, slice)
         self._check(self._lib.fe_engine_set_tuning(self._e, C.byref(t)))
 
@@ -401,7 +401,7 @@ class Engine:
             descs.append(d)
             keep.append(k)
         arr = (_lib.fe_job_desc * len(descs))(*descs)
-        cap = 64
+        cap = 128
         res = (fe_tuning * cap)()
         ms = (C.c_double * cap)()
         n = C.c_int(0)

@@ -249,8 +249,13 @@ static fe_status plan_job(const fe_job_desc *d, const Tuning &tun, Job &tmp, std
     err = "job description: d must be 8 x a power of two in [8, 512]";
     return FE_ERR_INVALID_ARG;
   }
+  if (d->d < tun.slice) {
+    err = "job description: d below the engine's slice (" + std::to_string(tun.slice) +
+          " components per lane)";
+    return FE_ERR_INVALID_ARG;
+  }
   s = build_plan(d->n, d->row_ptr, nullptr, d->batch_size, tun.chunk, tun.max_chunks, tun.light,
-                 64 / (d->d / 8), c_limit, &tmp.plan);
+                 64 / (d->d / tun.slice), c_limit, &tmp.plan);
   if (s != FE_OK) {
     err = host_error();
     return s;
@@ -354,10 +359,17 @@ extern "C" fe_status fe_engine_set_tuning(fe_engine *e, const fe_tuning *t) {
   if (t->waves_per_block) nt.waves_per_block = t->waves_per_block;
   if (t->items_per_wave) nt.items_per_wave = t->items_per_wave;
   if (t->light) nt.light = t->light < 0 ? 0 : t->light;
-  if (t->slice && t->slice != 8)
-    return efail(e, FE_ERR_INVALID_ARG, "fe_engine_set_tuning: engine rounds use slices of 8");
+  if (t->slice) nt.slice = t->slice;
   Job probe;
   if (job_set_tuning(probe, nt) != FE_OK) return efail(e, FE_ERR_INVALID_ARG, probe.err);
+  // one kernel serves a round, so the slice (a cutting field, fixed when a
+  // job is planned) changes only while the engine holds no job
+  if (nt.slice != e->tun.slice) {
+    for (auto &jp : e->jobs)
+      if (!jp->collected)
+        return efail(e, FE_ERR_STATE,
+                     "fe_engine_set_tuning: the slice changes only while the engine holds no job");
+  }
   // chunking of already submitted jobs is fixed; launch shape applies to all
   e->tun = nt;
   return FE_OK;
@@ -443,11 +455,21 @@ static int round_mode(const std::vector<EngineJob *> &run) {
 }
 
 // Lanes per row shared by every job of the round, or 0 if they differ.
+// Lanes per row shared by every step of the round (0 if they differ).
 static int round_g(const std::vector<EngineJob *> &run) {
-  int g = run.empty() ? 0 : run[0]->job.G;
+  auto lanes = [](const EngineJob *ej) { return ej->job.d / ej->job.tun.slice; };
+  int g = run.empty() ? 0 : lanes(run[0]);
   for (const EngineJob *ej : run)
-    if (ej->job.G != g) return 0;
+    if (lanes(ej) != g) return 0;
   return g;
+}
+
+// Components per lane of the round's steps: one kernel serves them all, so
+// every job must have been planned with the engine's slice (0 otherwise).
+static int round_slice(const fe_engine *e, const std::vector<EngineJob *> &run) {
+  for (const EngineJob *ej : run)
+    if (ej->job.tun.slice != e->tun.slice) return 0;
+  return e->tun.slice;
 }
 
 static fe_status build_round(fe_engine *e, std::vector<EngineJob *> &run, char *host, int &K,
@@ -548,6 +570,11 @@ static fe_status run_rounds(fe_engine *e, int rounds) {
     int K = 0;
     size_t lds = 0;
     std::vector<EngineJob *> finished;
+    const int slice = round_slice(e, run);
+    if (slice == 0) {
+      mark_failed(run);
+      return efail(e, FE_ERR_STATE, "fe_engine_run: a job's slice differs from the engine's");
+    }
     build_round(e, run, stg.host, K, lds, finished);
     // from here on the jobs have advanced: a failure ends them all
     const int *hpf = reinterpret_cast<const int *>(stg.host + K * sizeof(StepDesc)) + kStateInts * K;
@@ -560,7 +587,7 @@ static fe_status run_rounds(fe_engine *e, int rounds) {
     if (s == FE_OK)
       s = ehip(e, launch_force(dsd, dpf, dpf + 2 * (K + 1), K, fblocks, e->tun.waves_per_block,
                                e->tun.items_per_wave, e->tun.unroll, lds, round_mode(run),
-                               round_g(run), e->stream),
+                               round_g(run), slice, e->stream),
                "force launch");
     for (EngineJob *ej : finished)
       if (s == FE_OK) s = ehip(e, hipEventRecord(ej->done, e->stream), "event");
@@ -676,6 +703,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
     size_t lds;
     int mode;
     int g_const;
+    int slice;
   };
   std::vector<RoundInfo> rounds;
   std::vector<char> host;
@@ -693,9 +721,14 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
     std::vector<EngineJob *> finished;
     // descriptors point at this round's own epoch slots in the device schedule
     char *h = host.data() + total;
+    const int slice = round_slice(e, run);
+    if (slice == 0) {
+      mark_failed(all);
+      return efail(e, FE_ERR_STATE, "fe_engine_run_graph: a job's slice differs from the engine's");
+    }
     build_round(e, run, h, K, lds, finished);
     const int *hpf = reinterpret_cast<const int *>(h + K * sizeof(StepDesc)) + kStateInts * K;
-    rounds.push_back({total, K, hpf[K], hpf[2 * K + 1], lds, round_mode(run), round_g(run)});
+    rounds.push_back({total, K, hpf[K], hpf[2 * K + 1], lds, round_mode(run), round_g(run), slice});
     for (EngineJob *ej : finished) finish_at.push_back({(int)rounds.size() - 1, ej});
     total += rb;
   }
@@ -727,7 +760,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
       const int *dpf = reinterpret_cast<const int *>(dsched + ri.offset + ri.K * sizeof(StepDesc)) + kStateInts * ri.K;
       ls = ehip(e, launch_force(dsd, dpf, dpf + 2 * (ri.K + 1), ri.K, ri.fblocks,
                                 e->tun.waves_per_block, e->tun.items_per_wave, e->tun.unroll,
-                                ri.lds, ri.mode, ri.g_const, e->stream),
+                                ri.lds, ri.mode, ri.g_const, ri.slice, e->stream),
                 "force launch");
     }
     hipGraph_t g = nullptr;
@@ -818,7 +851,7 @@ static void rewind_jobs(fe_engine *e, const std::vector<int> &ids) {
 // Tiles of the batching form chosen by timing them on the workload itself:
 // a round mixes the batches of many jobs of different sizes, so a tiling
 // that suits one graph alone can lose on the mix. The jobs are submitted
-// once per cutting (light); every launch shape then replays all their
+// once per cutting (slice and light); every launch shape then replays all their
 // epochs through the captured-graph schedule from the rows the previous
 // replay left (the work items, and so the time, do not depend on the
 // values). The fastest replay (device time, HIP events) becomes the
@@ -845,8 +878,15 @@ extern "C" fe_status fe_engine_autotune(fe_engine *e, const fe_job_desc *jobs, i
   fe_tuning best_c{};
   int n = 0;
   std::vector<int> ids;
-  for (int light : {8, 16, 32}) {
-    fe_tuning cut{64, 1024, orig.unroll, orig.waves_per_block, orig.items_per_wave, light, 8};
+  int min_d = 512;
+  for (const fe_job_desc &d : descs) min_d = std::min(min_d, d.d);
+  std::vector<std::pair<int, int>> cuts; // (slice, light)
+  for (int slice : {8, 16})
+    for (int light : {8, 16, 32})
+      if (slice <= min_d) cuts.push_back({slice, light});
+  for (const auto &sl : cuts) {
+    const int slice = sl.first, light = sl.second;
+    fe_tuning cut{64, 1024, orig.unroll, orig.waves_per_block, orig.items_per_wave, light, slice};
     if ((s = fe_engine_set_tuning(e, &cut)) != FE_OK) break;
     ids.clear();
     for (const fe_job_desc &d : descs) {
@@ -858,7 +898,7 @@ extern "C" fe_status fe_engine_autotune(fe_engine *e, const fe_job_desc *jobs, i
       for (int w : {4, 8}) {
         for (int ipw : {1, 2}) {
           if (s != FE_OK) break;
-          const fe_tuning c{64, 1024, u, w, ipw, light, 8};
+          const fe_tuning c{64, 1024, u, w, ipw, light, slice};
           if ((s = fe_engine_set_tuning(e, &c)) != FE_OK) break;
           rewind_jobs(e, ids);
           if ((s = fe_engine_run_graph(e)) != FE_OK) break;

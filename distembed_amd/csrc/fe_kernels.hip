@@ -906,14 +906,14 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
 // Steps of one or more jobs in one launch (engine rounds): the block finds
 // its step (one load from the block-to-step table of a round of more than
 // two steps, else a search of the block prefix) and reads the step's
-// descriptor from memory. Slices of 8 components.
-template <int U, int MODE, int GT>
+// descriptor from memory. C: components per lane, shared by every step.
+template <int U, int MODE, int GT, int C>
 __global__ __launch_bounds__(512) void force_kernel(
     const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
     const int *__restrict__ blk_step, int n_steps, int ipw) {
   const int b = blockIdx.x;
   const int si = n_steps > 2 ? blk_step[b] : find_step(blk_prefix, n_steps, b);
-  force_body<U, MODE, GT, 8>(steps[si], b - blk_prefix[si], blk_prefix[si + 1] - blk_prefix[si]);
+  force_body<U, MODE, GT, C>(steps[si], b - blk_prefix[si], blk_prefix[si + 1] - blk_prefix[si]);
   (void)ipw;
 }
 
@@ -1374,13 +1374,13 @@ hipError_t scratch_address(int device, void **out) {
 // lds_bytes: staged negative rows and sub-range sums (force_lds_bytes);
 // mode: 0 all steps gather FP32, 1 all gather BF16, 2 mixed; g_const: the
 // lanes per row G shared by every step of the launch, or 0 if they differ.
-template <int U, int GT>
+template <int U, int GT, int C>
 static void launch_force_g(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
                            const StepDesc *steps, const int *blk_prefix, const int *blk_step,
                            int n_steps, int ipw) {
-  if (mode == 0) hipLaunchKernelGGL((force_kernel<U, 0, GT>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
-  else if (mode == 1) hipLaunchKernelGGL((force_kernel<U, 1, GT>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
-  else hipLaunchKernelGGL((force_kernel<U, 2, GT>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
+  if (mode == 0) hipLaunchKernelGGL((force_kernel<U, 0, GT, C>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
+  else if (mode == 1) hipLaunchKernelGGL((force_kernel<U, 1, GT, C>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
+  else hipLaunchKernelGGL((force_kernel<U, 2, GT, C>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
 }
 
 template <int U, int GT, int C>
@@ -1428,24 +1428,36 @@ hipError_t launch_force1(const StepDesc &sd, int n_blocks, int wpb, int unroll, 
 
 template <int U>
 static void launch_force_u(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
-                           int g_const, const StepDesc *steps, const int *blk_prefix,
+                           int g_const, int slice, const StepDesc *steps, const int *blk_prefix,
                            const int *blk_step, int n_steps, int ipw) {
-  // compile-time G for the common widths d = 32, 64, 128 (FP32 or BF16)
-  if (mode != 2 && g_const == 16) launch_force_g<U, 16>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
-  else if (mode != 2 && g_const == 8) launch_force_g<U, 8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
-  else if (mode != 2 && g_const == 4) launch_force_g<U, 4>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
-  else launch_force_g<U, 0>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+  // compile-time G for the common widths d = 32, 64, 128 (FP32 or BF16):
+  // slices of 8, or d = 64, 128 in slices of 16 (2 or 4 rows in flight)
+  if constexpr (U == 2 || U == 4) {
+    if (slice == 16) {
+      if (mode != 2 && g_const == 8) launch_force_g<U, 8, 16>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+      else if (mode != 2 && g_const == 4) launch_force_g<U, 4, 16>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+      else launch_force_g<U, 0, 16>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+      return;
+    }
+  }
+  if (mode != 2 && g_const == 16) launch_force_g<U, 16, 8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+  else if (mode != 2 && g_const == 8) launch_force_g<U, 8, 8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+  else if (mode != 2 && g_const == 4) launch_force_g<U, 4, 8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+  else launch_force_g<U, 0, 8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
 }
 
+// Engine rounds in slices of 16 run 2 or 4 rows in flight (1 -> 2, 8 -> 4;
+// the unroll never changes results).
 hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, const int *blk_step,
                         int n_steps, int n_blocks, int wpb, int ipw, int unroll, size_t lds_bytes,
-                        int mode, int g_const, hipStream_t st) {
+                        int mode, int g_const, int slice, hipStream_t st) {
   dim3 grid(n_blocks), block(64 * wpb);
+  if (slice == 16) unroll = unroll <= 2 ? 2 : 4;
   switch (unroll) {
-  case 1: launch_force_u<1>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, blk_step, n_steps, ipw); break;
-  case 2: launch_force_u<2>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, blk_step, n_steps, ipw); break;
-  case 8: launch_force_u<8>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, blk_step, n_steps, ipw); break;
-  default: launch_force_u<4>(grid, block, lds_bytes, st, mode, g_const, steps, blk_prefix, blk_step, n_steps, ipw); break;
+  case 1: launch_force_u<1>(grid, block, lds_bytes, st, mode, g_const, slice, steps, blk_prefix, blk_step, n_steps, ipw); break;
+  case 2: launch_force_u<2>(grid, block, lds_bytes, st, mode, g_const, slice, steps, blk_prefix, blk_step, n_steps, ipw); break;
+  case 8: launch_force_u<8>(grid, block, lds_bytes, st, mode, g_const, slice, steps, blk_prefix, blk_step, n_steps, ipw); break;
+  default: launch_force_u<4>(grid, block, lds_bytes, st, mode, g_const, slice, steps, blk_prefix, blk_step, n_steps, ipw); break;
   }
   return hipGetLastError();
 }

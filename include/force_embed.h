@@ -80,8 +80,8 @@ typedef struct fe_tuning {
                           group, 64/(d/slice) rows per wave (rows per
                           wavefront)                                        */
   int slice;           /* components of a row per lane, 8 or 16 (<= d): a
-                          row takes d/slice lanes (the dimension-chunk tile;
-                          engine rounds use 8)                              */
+                          row takes d/slice lanes (the dimension-chunk tile)
+                                                                            */
 } fe_tuning;
 
 typedef struct fe_ctx fe_ctx;
@@ -331,11 +331,15 @@ fe_status fe_engine_job_done(fe_engine *e, int job_id, int *done);
  * fe_engine_run or fe_engine_run_graph has no result: FE_ERR_STATE, and its
  * memory returns to the budget.                                           */
 fe_status fe_engine_collect(fe_engine *e, int job_id, float *x_out);
-/* Launch shape for later rounds and chunking for later submissions.        */
+/* Launch shape for later rounds and chunking for later submissions. One
+ * kernel serves a round, so every job runs in the engine's slice: a new
+ * slice needs an engine that holds no uncollected job (FE_ERR_STATE), and
+ * a job with d below the slice is refused at submission.                 */
 fe_status fe_engine_set_tuning(fe_engine *e, const fe_tuning *t);
 /* Autotuner of the batching form's tiles (no counterpart in the reference):
  * the n_jobs jobs (their iterations replaced by `epochs`) are submitted once
- * per cutting -- light {8,16,32} at chunk 64, max_chunks 1024 -- and replayed
+ * per cutting -- slice {8,16} (16 when every job has d >= 16) x light
+ * {8,16,32} at chunk 64, max_chunks 1024 -- and replayed
  * through fe_engine_run_graph once per launch shape -- unroll {2,4,8} x
  * waves per block {4,8} x items per wave {1,2}; the fastest replay (device
  * time, HIP events) becomes the engine's tuning. The engine must hold no

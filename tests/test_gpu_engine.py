@@ -229,16 +229,18 @@ def test_submit_and_collect_on_other_threads():
 
 
 def test_engine_autotune_keeps_the_fastest_tiles():
-    """fe_engine_autotune times its 36 candidate tilings on the given jobs
-    through the captured-graph schedule, keeps the fastest as the engine's
-    tuning and leaves no job behind: a job run afterwards equals the same
-    job under an engine given that tuning explicitly."""
+    """fe_engine_autotune times its 72 candidate tilings (slices of 8 and
+    16: every job has d >= 16) on the given jobs through the captured-graph
+    schedule, keeps the fastest as the engine's tuning and leaves no job
+    behind: a job run afterwards equals the same job under an engine given
+    that tuning explicitly."""
     specs = _jobs()
     with api.Engine(2 * GiB) as e:
         best, best_ms, table = e.autotune(
             [(s["g"], s["x0"], s["batch_size"], dict(ns=5, gather=s["gather"])) for s in specs],
             epochs=1)
-        assert len(table) == 36
+        assert len(table) == 72
+        assert {t["slice"] for t, _ in table} == {8, 16}
         assert best_ms == min(ms for _, ms in table) and best_ms > 0
         assert best in [t for t, _ in table]
         assert e.stats()["bytes_in_use"] == 0

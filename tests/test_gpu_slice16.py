@@ -229,3 +229,69 @@ def test_slice_arguments():
         assert ctx.get_tuning()["slice"] == 8
     finally:
         ctx.close()
+
+
+GiB = 1 << 30
+
+
+def _engine_specs():
+    specs = [  # (graph, d, gather, batch, epochs)
+        (api.Graph.rmat(12, 16, seed=1), 128, api.FP32, 512, 3),
+        (api.Graph.sbm(3000, 12, 0.01, 0.0005, seed=2), 64, api.BF16, 1024, 4),
+        (api.Graph.sbm(1200, 4, 0.02, 0.001, seed=3), 32, api.FP32, 256, 3),
+    ]
+    return [dict(g=g, x0=api.init_uniform(g.n, d, seed=20 + k), B=B, T=T, gather=gather)
+            for k, (g, d, gather, B, T) in enumerate(specs)]
+
+
+def test_engine_rounds_in_slices_of_16_equal_contexts():
+    # one round mixes FP32 and BF16 jobs of d = 128, 64 and 32 (the run-time
+    # dtype and width form) in slices of 16; each job equals the single-graph
+    # context with the same tiling, through rounds and the captured schedule
+    specs = _engine_specs()
+    outs = {}
+    for mode in ("rounds", "graph"):
+        with api.Engine(2 * GiB) as e:
+            e.set_tuning(slice=16)
+            ids = [e.submit(s["g"], s["x0"], s["B"], s["T"], gather=s["gather"], ns=5, lr=0.02)
+                   for s in specs]
+            if mode == "rounds":
+                while e.run(16) > 0:
+                    pass
+            else:
+                e.run_graph()
+            outs[mode] = [e.collect(i) for i in ids]
+    for k, s in enumerate(specs):
+        g, x0 = s["g"], s["x0"]
+        with api.ForceEmbed(g.n, x0.shape[1], gather=s["gather"]) as ctx:
+            ctx.upload_csr(g)
+            ctx.upload_x(x0)
+            ctx.set_tuning(slice=16)
+            ctx.train(s["T"], s["B"], 5, 0.02)
+            xc = ctx.download_x()
+        assert np.isfinite(xc).all()
+        np.testing.assert_array_equal(outs["rounds"][k], xc, err_msg=f"job {k}")
+        np.testing.assert_array_equal(outs["graph"][k], xc, err_msg=f"job {k}")
+
+
+def test_engine_slice_guards():
+    g = api.Graph.sbm(600, 3, 0.03, 0.002, seed=4)
+    x8 = api.init_uniform(g.n, 8, seed=1)
+    x64 = api.init_uniform(g.n, 64, seed=1)
+    with api.Engine(GiB) as e:
+        e.set_tuning(slice=16)
+        with pytest.raises(api.FeError):
+            e.submit(g, x8, 128, 2, ns=5, lr=0.02)   # d below the slice
+        i = e.submit(g, x64, 128, 2, ns=5, lr=0.02)
+        with pytest.raises(api.FeError):
+            e.set_tuning(slice=8)                   # a held job keeps its slice
+        e.set_tuning(unroll=4)                      # launch shapes may change
+        while e.run(16) > 0:
+            pass
+        x = e.collect(i)
+        assert np.isfinite(x).all()
+        e.set_tuning(slice=8)
+        i = e.submit(g, x8, 128, 2, ns=5, lr=0.02)
+        while e.run(16) > 0:
+            pass
+        assert np.isfinite(e.collect(i)).all()
