@@ -1,5 +1,6 @@
 """The workloads of the precision, batching and serving forms on the GPU,
-each checked against the oracle.
+each checked against the oracle, and the reference's own CPU workload
+(SBM-10K, 200 epochs) run whole.
 
 precision: R-MAT scale 20 and 22 (a=0.57, b=c=0.19, seed 0), d=128,
   B=16384, 5 and 10 negatives. Per-batch parity of the BF16 shadow against
@@ -64,6 +65,56 @@ def _track_oracle(xg, g, x0, B, T, lists, gather, what):
     assert np.abs(diff).max() <= 1e-4 * np.abs(xr).max(), what
     assert np.linalg.norm(diff) <= (1e-5 * np.linalg.norm(xr)
                                     + 2 * np.linalg.norm(xr.astype(np.float64) - x64)), what
+
+
+# ---- the reference's CPU workload, whole run ---------------------------------------
+
+def test_sbm10k_full_run_tracks_oracle():
+    """SBM with 10,000 vertices in 10 blocks (average intra-degree 18,
+    inter-degree 2), d=128, X0 U(-0.5,0.5) seed 0, B=256 (40 batches, the
+    last of 16 rows: the sticky repulsive count from epoch 1 on), 5
+    reference negatives, lr 0.02, all 200 epochs. The FP32 run tracks the
+    FP32 and FP64 oracles to the bars of _track_oracle: this workload is not
+    chaotic, the FP32 and FP64 oracle runs end 6e-7 apart (max relative).
+    The BF16 form ends within 1 % of the FP32 final loss. Its rows are
+    compared with the oracle's BF16 run at the size of the BF16 effect
+    itself: over 200 epochs a BF16 run moves with the rounding of its FP32
+    sums (one flipped BF16 rounding changes a gathered component by 2^-9).
+    The oracle's BF16 run ends 4.6e-3 (normwise) from its FP32 run, and
+    5.6e-3 from itself started with 1000 components moved by one ulp. So
+    the GPU's BF16 run must stay within twice the oracle's BF16-FP32
+    distance normwise and four times it elementwise."""
+    g = api.Graph.sbm(10_000, 10, 18 / 999, 2 / 9000, seed=0)
+    d, B, T, ns = 128, 256, 200, 5
+    x0 = api.init_uniform(g.n, d, -0.5, 0.5, seed=0)
+    lists = api.reference_negatives(g.n, T + -(-g.n // B) - 1, ns)[0]
+    out, loss = {}, {}
+    for gather in (api.FP32, api.BF16):
+        with api.ForceEmbed(g.n, d, gather=gather) as ctx:
+            ctx.upload_csr(g)
+            ctx.upload_x(x0)
+            ctx.set_negatives(lists)
+            ctx.configure(B, 0.02, 5.0)
+            ctx.train(T, B, ns, 0.02)
+            out[gather] = ctx.download_x()
+            a, r = ctx.loss(0, ns)
+            loss[gather] = a + r
+    _track_oracle(out[api.FP32], g, x0, B, T, lists, api.FP32, "SBM-10K FP32, 200 epochs")
+    x32, x16 = x0.copy(), x0.copy()
+    orc.train(g, x32, B, T, lists)
+    orc.train(g, x16, B, T, lists, bf16=True)
+    norm = lambda a: float(np.linalg.norm(a.astype(np.float64)))
+    effect, effect_max = norm(x16 - x32), float(np.abs(x16 - x32).max())
+    dist, dist_max = norm(out[api.BF16] - x16), float(np.abs(out[api.BF16] - x16).max())
+    print(f"  SBM-10K BF16: GPU to oracle {dist / norm(x16):.2e} normwise, {dist_max:.2e} max; "
+          f"oracle BF16 to FP32 {effect / norm(x16):.2e}, {effect_max:.2e}", flush=True)
+    assert np.isfinite(out[api.BF16]).all()
+    assert dist <= 2.0 * effect, (dist, effect)
+    assert dist_max <= 4.0 * effect_max, (dist_max, effect_max)
+    rel = (loss[api.BF16] - loss[api.FP32]) / loss[api.FP32]
+    print(f"  SBM-10K 200 epochs: FP32 loss {loss[api.FP32]:.6e}, BF16 {loss[api.BF16]:.6e} "
+          f"({rel * 100:+.3f}%)", flush=True)
+    assert abs(rel) <= 0.01, rel
 
 
 # ---- precision form --------------------------------------------------------------
