@@ -425,7 +425,12 @@ fe_status job_configure(Job &j, int B, float lr, float bound) {
     j.err = "fe_configure: batch size must be > 0, lr and max_bound finite, max_bound >= 0";
     return FE_ERR_INVALID_ARG;
   }
-  if (B != j.B) j.plan_dirty = true;
+  if (B != j.B) {
+    // per-batch shapes and cuttings name batches of the old size
+    j.plan_dirty = true;
+    j.bshape.clear();
+    j.bcut.clear();
+  }
   if (B != j.B || lr != j.lr || bound != j.bound) j.steps_dirty = true;
   j.B = B;
   j.lr = lr;
@@ -459,8 +464,9 @@ fe_status job_set_tuning(Job &j, const Tuning &t) {
 // Light rows per wave of a batch whose lanes hold `slice` components.
 static int pack_of(const Job &j, int slice) { return 64 / (j.d / slice); }
 
+// Per-batch values hold only for the plan they were set on (one per batch).
 Cut job_cut(const Job &j, int b) {
-  if (b >= 0 && b < (int)j.bcut.size()) return j.bcut[b];
+  if (b >= 0 && b < (int)j.bcut.size() && (int)j.bcut.size() == j.plan.batches) return j.bcut[b];
   return Cut{j.tun.chunk, j.tun.max_chunks, j.tun.light, j.tun.slice, pack_of(j, j.tun.slice)};
 }
 
@@ -480,7 +486,8 @@ fe_status job_set_cuts(Job &j, const std::vector<Cut> &cuts) {
 }
 
 Shape job_shape(const Job &j, int b) {
-  if (b >= 0 && b < (int)j.bshape.size()) return j.bshape[b];
+  if (b >= 0 && b < (int)j.bshape.size() && (int)j.bshape.size() == j.plan.batches)
+    return j.bshape[b];
   return Shape{j.tun.unroll, j.tun.waves_per_block, j.tun.items_per_wave};
 }
 

@@ -295,3 +295,25 @@ def test_engine_slice_guards():
         while e.run(16) > 0:
             pass
         assert np.isfinite(e.collect(i)).all()
+
+
+def test_new_batch_size_drops_per_batch_tunings():
+    # per-batch cuttings (mixed slices) name batches of one batch size; a new
+    # batch size plans with the context's tuning, as a fresh context does
+    g = api.Graph.rmat(12, 16, seed=5)
+    X = api.init_uniform(g.n, 64, seed=3)
+    lists = api.reference_negatives(g.n, 40, 5)[0]
+    nb = -(-g.n // 512)
+    ctx = _ctx(g, 64, X, 512, lists)
+    ctx.set_batch_tunings([(2, 4, 1, 16, 64, 8, 16 if b % 2 else 8) for b in range(nb)])
+    ctx.step_delta(0, 1, 512)
+    ctx.configure(256, 0.02, 5.0)
+    assert {t["slice"] for t in ctx.batch_tunings()} == {8}
+    got = [ctx.step_delta(0, j, 256) for j in (0, 1, 3)]
+    ctx.close()
+    ref = _ctx(g, 64, X, 256, lists)
+    try:
+        for k, j in enumerate((0, 1, 3)):
+            np.testing.assert_array_equal(got[k], ref.step_delta(0, j, 256))
+    finally:
+        ref.close()
