@@ -201,3 +201,34 @@ def test_mtx_large_file_parsed_in_chunks(tmp_path):
                  f"{n} {n} {m}\n" + "\n".join(bad_lines) + "\n")
     with pytest.raises(api.FeError, match="bad entry line"):
         api.Graph.read_mtx(q)
+
+
+def test_ctypes_structs_match_the_c_header(tmp_path):
+    """The Python binding's structs have the C header's field offsets and
+    sizes (fe_tuning, fe_graph, fe_job_desc), read from a program compiled
+    against include/force_embed.h."""
+    import subprocess
+    from distembed_amd._lib import fe_graph, fe_job_desc, fe_tuning
+    structs = {"fe_tuning": fe_tuning, "fe_graph": fe_graph, "fe_job_desc": fe_job_desc}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "force_embed.h"',
+             'int main(void) {']
+    for name, cls in structs.items():
+        lines.append(f'  printf("{name} size %zu\\n", sizeof({name}));')
+        for field, _ in cls._fields_:
+            lines.append(f'  printf("{name} {field} %zu\\n", offsetof({name}, {field}));')
+    lines += ['  return 0;', '}']
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines) + "\n")
+    exe = tmp_path / "layout"
+    root = Path(__file__).resolve().parent.parent
+    subprocess.run(["gcc", "-std=c11", f"-I{root / 'include'}", str(src), "-o", str(exe)],
+                   check=True)
+    got = {}
+    for line in subprocess.run([str(exe)], check=True, capture_output=True,
+                               text=True).stdout.splitlines():
+        name, key, val = line.split()
+        got[(name, key)] = int(val)
+    for name, cls in structs.items():
+        assert got[(name, "size")] == C.sizeof(cls), name
+        for field, _ in cls._fields_:
+            assert got[(name, field)] == getattr(cls, field).offset, (name, field)
