@@ -189,6 +189,15 @@ static size_t round_bytes(size_t K, size_t F) {
   return (b + 255) / 256 * 256;
 }
 
+// Whether a step of the next round of `run` has block items (a round
+// without any takes the kernel without the block-item path).
+static bool round_blk(const std::vector<EngineJob *> &run) {
+  for (const EngineJob *ej : run)
+    if (ej->job.host_desc[(size_t)ej->job.plan.batches * DESC_FUSED + ej->batch].n_block_items > 0)
+      return true;
+  return false;
+}
+
 // Force blocks of the next round of `run` (as build_round will lay them out).
 static int round_force_blocks(const fe_engine *e, const std::vector<EngineJob *> &run) {
   const int ipb = e->tun.waves_per_block * e->tun.items_per_wave;
@@ -454,7 +463,6 @@ static int round_mode(const std::vector<EngineJob *> &run) {
   return (f && h) ? 2 : (h ? 1 : 0);
 }
 
-// Lanes per row shared by every job of the round, or 0 if they differ.
 // Lanes per row shared by every step of the round (0 if they differ).
 static int round_g(const std::vector<EngineJob *> &run) {
   auto lanes = [](const EngineJob *ej) { return ej->job.d / ej->job.tun.slice; };
@@ -575,6 +583,7 @@ static fe_status run_rounds(fe_engine *e, int rounds) {
       mark_failed(run);
       return efail(e, FE_ERR_STATE, "fe_engine_run: a job's slice differs from the engine's");
     }
+    const bool blk = round_blk(run);
     build_round(e, run, stg.host, K, lds, finished);
     // from here on the jobs have advanced: a failure ends them all
     const int *hpf = reinterpret_cast<const int *>(stg.host + K * sizeof(StepDesc)) + kStateInts * K;
@@ -587,7 +596,7 @@ static fe_status run_rounds(fe_engine *e, int rounds) {
     if (s == FE_OK)
       s = ehip(e, launch_force(dsd, dpf, dpf + 2 * (K + 1), K, fblocks, e->tun.waves_per_block,
                                e->tun.items_per_wave, e->tun.unroll, lds, round_mode(run),
-                               round_g(run), slice, e->stream),
+                               round_g(run), slice, blk, e->stream),
                "force launch");
     for (EngineJob *ej : finished)
       if (s == FE_OK) s = ehip(e, hipEventRecord(ej->done, e->stream), "event");
@@ -704,6 +713,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
     int mode;
     int g_const;
     int slice;
+    bool blk;
   };
   std::vector<RoundInfo> rounds;
   std::vector<char> host;
@@ -726,9 +736,11 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
       mark_failed(all);
       return efail(e, FE_ERR_STATE, "fe_engine_run_graph: a job's slice differs from the engine's");
     }
+    const bool blk = round_blk(run);
     build_round(e, run, h, K, lds, finished);
     const int *hpf = reinterpret_cast<const int *>(h + K * sizeof(StepDesc)) + kStateInts * K;
-    rounds.push_back({total, K, hpf[K], hpf[2 * K + 1], lds, round_mode(run), round_g(run), slice});
+    rounds.push_back(
+        {total, K, hpf[K], hpf[2 * K + 1], lds, round_mode(run), round_g(run), slice, blk});
     for (EngineJob *ej : finished) finish_at.push_back({(int)rounds.size() - 1, ej});
     total += rb;
   }
@@ -760,7 +772,7 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
       const int *dpf = reinterpret_cast<const int *>(dsched + ri.offset + ri.K * sizeof(StepDesc)) + kStateInts * ri.K;
       ls = ehip(e, launch_force(dsd, dpf, dpf + 2 * (ri.K + 1), ri.K, ri.fblocks,
                                 e->tun.waves_per_block, e->tun.items_per_wave, e->tun.unroll,
-                                ri.lds, ri.mode, ri.g_const, ri.slice, e->stream),
+                                ri.lds, ri.mode, ri.g_const, ri.slice, ri.blk, e->stream),
                 "force launch");
     }
     hipGraph_t g = nullptr;

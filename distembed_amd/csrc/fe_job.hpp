@@ -17,10 +17,11 @@ namespace fe {
 // g_const: lanes per row shared by every step of the launch (0: they differ).
 // blk_step[b]: the step of block b (read when n_steps > 2; else the prefix
 // is searched). slice: components per lane of every step (each step's
-// G = D / slice).
+// G = D / slice). blk: some step has block items (false takes the kernel
+// without the block-item path; every step must then have none).
 hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, const int *blk_step,
                         int n_steps, int n_blocks, int wpb, int ipw, int unroll, size_t lds_bytes,
-                        int mode, int g_const, int slice, hipStream_t st);
+                        int mode, int g_const, int slice, bool blk, hipStream_t st);
 // One step whose descriptor is passed by value (mode 0 or 1); slice: the
 // components per lane (8 or 16), sd.G = sd.D / slice.
 hipError_t launch_force1(const StepDesc &sd, int n_blocks, int wpb, int unroll, size_t lds_bytes,

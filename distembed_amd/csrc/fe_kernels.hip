@@ -751,9 +751,11 @@ __device__ __forceinline__ void light_pack(const StepDesc &sd, const RowSrc &src
 // writes its new rows into the layout's free slot, which no thread of the
 // launch reads. C: components per lane (the step's G = D / C lanes per row).
 // GT > 0: every step of the launch has G = GT (a compile-time constant);
-// 0: per step. lb, nb: this block's index among the step's blocks and their
-// count.
-template <int U, int MODE, int GT, int C>
+// 0: per step. BLK false: no step of the launch has block items, and the
+// block-item path is compiled out (its registers cost the other paths a
+// wave per SIMD). lb, nb: this block's index among the step's blocks and
+// their count.
+template <int U, int MODE, int GT, int C, bool BLK>
 __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
   extern __shared__ float lds[];
   const int D = sd.D;
@@ -766,7 +768,7 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
   const int q = lane & (G - 1);
 
   // this wave's first work item, in flight during the staging
-  const int nbi = sd.n_block_items;
+  const int nbi = BLK ? sd.n_block_items : 0;
   const int n_light = sd.n_light;
   const int n_wave_items = sd.n_items - nbi - n_light;
   const int32_t *witems = sd.items + (size_t)kItemInts * nbi;
@@ -774,9 +776,10 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
   const int n_packs = (n_light + NG - 1) / NG;
   // wave units: medium rows [0, n_wave_items), then light packs
   int u = (lb - nbi) * wpb + wave;
+  const bool blk_item = BLK && lb < nbi; // this block takes a block item
   u4v pre0 = {0u, 0u, 0u, 0u}, pre1 = {0u, 0u, 0u, 0u};
   int pc = 0; // a light row's column kb + q = q (its first G columns)
-  if (lb < nbi) {
+  if (blk_item) {
     pre0 = gld_u4(sd.items + (size_t)kItemInts * lb);
     pre1 = gld_u4(sd.items + (size_t)kItemInts * lb + 4);
   } else if (u < n_wave_items) {
@@ -807,7 +810,7 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
   const uint32_t out = (uint32_t)out_sl * (uint32_t)sd.lay.B;
 
   // the own row of this wave's first unit, in flight with the negative rows
-  const bool unit0 = lb < nbi || u < n_wave_items + n_packs;
+  const bool unit0 = blk_item || u < n_wave_items + n_packs;
   float xr0[C];
   if (unit0) load_f32(sd.X + (size_t)(pre0.x + own_delta) * D + C * q, xr0);
   const int rep = epoch == 0 ? sd.rep0 : sd.rep1;
@@ -819,7 +822,7 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
   }
   const RowSrc src = make_src<MODE, C>(sd, m, q, G);
 
-  if (lb < nbi) {
+  if (blk_item) {
     const int r = (int)pre0.x, e0 = (int)pre0.y, cnt = (int)pre0.z, pslot = (int)pre0.w;
     const int t = r - sd.row_begin;
     float xd[C], acc[C];
@@ -906,23 +909,26 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
 // Steps of one or more jobs in one launch (engine rounds): the block finds
 // its step (one load from the block-to-step table of a round of more than
 // two steps, else a search of the block prefix) and reads the step's
-// descriptor from memory. C: components per lane, shared by every step.
-template <int U, int MODE, int GT, int C>
+// descriptor from memory. C: components per lane, shared by every step;
+// BLK: some step has block items.
+template <int U, int MODE, int GT, int C, bool BLK>
 __global__ __launch_bounds__(512) void force_kernel(
     const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
     const int *__restrict__ blk_step, int n_steps, int ipw) {
   const int b = blockIdx.x;
   const int si = n_steps > 2 ? blk_step[b] : find_step(blk_prefix, n_steps, b);
-  force_body<U, MODE, GT, C>(steps[si], b - blk_prefix[si], blk_prefix[si + 1] - blk_prefix[si]);
+  force_body<U, MODE, GT, C, BLK>(steps[si], b - blk_prefix[si],
+                                  blk_prefix[si + 1] - blk_prefix[si]);
   (void)ipw;
 }
 
 // One step (a single graph's minibatch): the descriptor travels in the
 // kernel arguments, so the first loads of the items, the columns and the
-// epoch state do not wait for a descriptor load. C: components per lane.
-template <int U, int MODE, int GT, int C>
+// epoch state do not wait for a descriptor load. C: components per lane;
+// BLK: the step has block items.
+template <int U, int MODE, int GT, int C, bool BLK>
 __global__ __launch_bounds__(512) void force_kernel1(const StepDesc sd) {
-  force_body<U, MODE, GT, C>(sd, (int)blockIdx.x, (int)gridDim.x);
+  force_body<U, MODE, GT, C, BLK>(sd, (int)blockIdx.x, (int)gridDim.x);
 }
 
 // ---- SpMM sibling: SpMMAlgo::algo_spmm at world size 1 ----------------------
@@ -1374,39 +1380,52 @@ hipError_t scratch_address(int device, void **out) {
 // lds_bytes: staged negative rows and sub-range sums (force_lds_bytes);
 // mode: 0 all steps gather FP32, 1 all gather BF16, 2 mixed; g_const: the
 // lanes per row G shared by every step of the launch, or 0 if they differ.
-template <int U, int GT, int C>
+template <int U, int GT, int C, bool BLK>
 static void launch_force_g(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
                            const StepDesc *steps, const int *blk_prefix, const int *blk_step,
                            int n_steps, int ipw) {
-  if (mode == 0) hipLaunchKernelGGL((force_kernel<U, 0, GT, C>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
-  else if (mode == 1) hipLaunchKernelGGL((force_kernel<U, 1, GT, C>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
-  else hipLaunchKernelGGL((force_kernel<U, 2, GT, C>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
+  if (mode == 0) hipLaunchKernelGGL((force_kernel<U, 0, GT, C, BLK>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
+  else if (mode == 1) hipLaunchKernelGGL((force_kernel<U, 1, GT, C, BLK>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
+  else hipLaunchKernelGGL((force_kernel<U, 2, GT, C, BLK>), grid, block, lds_bytes, st, steps, blk_prefix, blk_step, n_steps, ipw);
 }
 
-template <int U, int GT, int C>
+template <int U, int GT, int C, bool BLK>
 static void launch_force1_g(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
                             const StepDesc &sd) {
-  if (mode == 1) hipLaunchKernelGGL((force_kernel1<U, 1, GT, C>), grid, block, lds_bytes, st, sd);
-  else hipLaunchKernelGGL((force_kernel1<U, 0, GT, C>), grid, block, lds_bytes, st, sd);
+  if (mode == 1) hipLaunchKernelGGL((force_kernel1<U, 1, GT, C, BLK>), grid, block, lds_bytes, st, sd);
+  else hipLaunchKernelGGL((force_kernel1<U, 0, GT, C, BLK>), grid, block, lds_bytes, st, sd);
 }
 
-// compile-time G for the common widths d = 32, 64, 128 (slices of 8) and
-// d = 64, 128 (slices of 16)
-template <int U>
-static void launch_force1_u(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
-                            int g_const, int slice, const StepDesc &sd) {
+// Compile-time G for the common widths: d = 32, 64, 128 in slices of 8 and
+// d = 64, 128 in slices of 16. Launches without block items over slices of
+// 8 and up to 4 rows in flight take the kernel without the block-item path.
+template <int U, int GT>
+static void launch_force1_b(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
+                            bool blk, const StepDesc &sd) {
   if constexpr (U <= 4) {
-    if (slice == 16) {
-      if (g_const == 8) launch_force1_g<U, 8, 16>(grid, block, lds_bytes, st, mode, sd);
-      else if (g_const == 4) launch_force1_g<U, 4, 16>(grid, block, lds_bytes, st, mode, sd);
-      else launch_force1_g<U, 0, 16>(grid, block, lds_bytes, st, mode, sd);
+    if (!blk) {
+      launch_force1_g<U, GT, 8, false>(grid, block, lds_bytes, st, mode, sd);
       return;
     }
   }
-  if (g_const == 16) launch_force1_g<U, 16, 8>(grid, block, lds_bytes, st, mode, sd);
-  else if (g_const == 8) launch_force1_g<U, 8, 8>(grid, block, lds_bytes, st, mode, sd);
-  else if (g_const == 4) launch_force1_g<U, 4, 8>(grid, block, lds_bytes, st, mode, sd);
-  else launch_force1_g<U, 0, 8>(grid, block, lds_bytes, st, mode, sd);
+  launch_force1_g<U, GT, 8, true>(grid, block, lds_bytes, st, mode, sd);
+}
+
+template <int U>
+static void launch_force1_u(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
+                            int g_const, int slice, bool blk, const StepDesc &sd) {
+  if constexpr (U <= 4) {
+    if (slice == 16) {
+      if (g_const == 8) launch_force1_g<U, 8, 16, true>(grid, block, lds_bytes, st, mode, sd);
+      else if (g_const == 4) launch_force1_g<U, 4, 16, true>(grid, block, lds_bytes, st, mode, sd);
+      else launch_force1_g<U, 0, 16, true>(grid, block, lds_bytes, st, mode, sd);
+      return;
+    }
+  }
+  if (g_const == 16) launch_force1_b<U, 16>(grid, block, lds_bytes, st, mode, blk, sd);
+  else if (g_const == 8) launch_force1_b<U, 8>(grid, block, lds_bytes, st, mode, blk, sd);
+  else if (g_const == 4) launch_force1_b<U, 4>(grid, block, lds_bytes, st, mode, blk, sd);
+  else launch_force1_b<U, 0>(grid, block, lds_bytes, st, mode, blk, sd);
 }
 
 // Slices of 16 keep at most 4 rows in flight per lane group (8 would spill);
@@ -1414,50 +1433,64 @@ static void launch_force1_u(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t
 hipError_t launch_force1(const StepDesc &sd, int n_blocks, int wpb, int unroll, size_t lds_bytes,
                          int mode, int g_const, int slice, hipStream_t st) {
   dim3 grid(n_blocks), block(64 * wpb);
+  const bool blk = sd.n_block_items > 0;
   switch (unroll) {
-  case 1: launch_force1_u<1>(grid, block, lds_bytes, st, mode, g_const, slice, sd); break;
-  case 2: launch_force1_u<2>(grid, block, lds_bytes, st, mode, g_const, slice, sd); break;
+  case 1: launch_force1_u<1>(grid, block, lds_bytes, st, mode, g_const, slice, blk, sd); break;
+  case 2: launch_force1_u<2>(grid, block, lds_bytes, st, mode, g_const, slice, blk, sd); break;
   case 8:
-    if (slice == 16) launch_force1_u<4>(grid, block, lds_bytes, st, mode, g_const, slice, sd);
-    else launch_force1_u<8>(grid, block, lds_bytes, st, mode, g_const, slice, sd);
+    if (slice == 16) launch_force1_u<4>(grid, block, lds_bytes, st, mode, g_const, slice, blk, sd);
+    else launch_force1_u<8>(grid, block, lds_bytes, st, mode, g_const, slice, blk, sd);
     break;
-  default: launch_force1_u<4>(grid, block, lds_bytes, st, mode, g_const, slice, sd); break;
+  default: launch_force1_u<4>(grid, block, lds_bytes, st, mode, g_const, slice, blk, sd); break;
   }
   return hipGetLastError();
 }
 
+template <int U, int GT>
+static void launch_force_b(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
+                           bool blk, const StepDesc *steps, const int *blk_prefix,
+                           const int *blk_step, int n_steps, int ipw) {
+  if constexpr (U <= 4) {
+    if (!blk) {
+      launch_force_g<U, GT, 8, false>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+      return;
+    }
+  }
+  launch_force_g<U, GT, 8, true>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+}
+
 template <int U>
 static void launch_force_u(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int mode,
-                           int g_const, int slice, const StepDesc *steps, const int *blk_prefix,
-                           const int *blk_step, int n_steps, int ipw) {
+                           int g_const, int slice, bool blk, const StepDesc *steps,
+                           const int *blk_prefix, const int *blk_step, int n_steps, int ipw) {
   // compile-time G for the common widths d = 32, 64, 128 (FP32 or BF16):
   // slices of 8, or d = 64, 128 in slices of 16 (2 or 4 rows in flight)
   if constexpr (U == 2 || U == 4) {
     if (slice == 16) {
-      if (mode != 2 && g_const == 8) launch_force_g<U, 8, 16>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
-      else if (mode != 2 && g_const == 4) launch_force_g<U, 4, 16>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
-      else launch_force_g<U, 0, 16>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+      if (mode != 2 && g_const == 8) launch_force_g<U, 8, 16, true>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+      else if (mode != 2 && g_const == 4) launch_force_g<U, 4, 16, true>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+      else launch_force_g<U, 0, 16, true>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
       return;
     }
   }
-  if (mode != 2 && g_const == 16) launch_force_g<U, 16, 8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
-  else if (mode != 2 && g_const == 8) launch_force_g<U, 8, 8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
-  else if (mode != 2 && g_const == 4) launch_force_g<U, 4, 8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
-  else launch_force_g<U, 0, 8>(grid, block, lds_bytes, st, mode, steps, blk_prefix, blk_step, n_steps, ipw);
+  if (mode != 2 && g_const == 16) launch_force_b<U, 16>(grid, block, lds_bytes, st, mode, blk, steps, blk_prefix, blk_step, n_steps, ipw);
+  else if (mode != 2 && g_const == 8) launch_force_b<U, 8>(grid, block, lds_bytes, st, mode, blk, steps, blk_prefix, blk_step, n_steps, ipw);
+  else if (mode != 2 && g_const == 4) launch_force_b<U, 4>(grid, block, lds_bytes, st, mode, blk, steps, blk_prefix, blk_step, n_steps, ipw);
+  else launch_force_b<U, 0>(grid, block, lds_bytes, st, mode, blk, steps, blk_prefix, blk_step, n_steps, ipw);
 }
 
 // Engine rounds in slices of 16 run 2 or 4 rows in flight (1 -> 2, 8 -> 4;
 // the unroll never changes results).
 hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, const int *blk_step,
                         int n_steps, int n_blocks, int wpb, int ipw, int unroll, size_t lds_bytes,
-                        int mode, int g_const, int slice, hipStream_t st) {
+                        int mode, int g_const, int slice, bool blk, hipStream_t st) {
   dim3 grid(n_blocks), block(64 * wpb);
   if (slice == 16) unroll = unroll <= 2 ? 2 : 4;
   switch (unroll) {
-  case 1: launch_force_u<1>(grid, block, lds_bytes, st, mode, g_const, slice, steps, blk_prefix, blk_step, n_steps, ipw); break;
-  case 2: launch_force_u<2>(grid, block, lds_bytes, st, mode, g_const, slice, steps, blk_prefix, blk_step, n_steps, ipw); break;
-  case 8: launch_force_u<8>(grid, block, lds_bytes, st, mode, g_const, slice, steps, blk_prefix, blk_step, n_steps, ipw); break;
-  default: launch_force_u<4>(grid, block, lds_bytes, st, mode, g_const, slice, steps, blk_prefix, blk_step, n_steps, ipw); break;
+  case 1: launch_force_u<1>(grid, block, lds_bytes, st, mode, g_const, slice, blk, steps, blk_prefix, blk_step, n_steps, ipw); break;
+  case 2: launch_force_u<2>(grid, block, lds_bytes, st, mode, g_const, slice, blk, steps, blk_prefix, blk_step, n_steps, ipw); break;
+  case 8: launch_force_u<8>(grid, block, lds_bytes, st, mode, g_const, slice, blk, steps, blk_prefix, blk_step, n_steps, ipw); break;
+  default: launch_force_u<4>(grid, block, lds_bytes, st, mode, g_const, slice, blk, steps, blk_prefix, blk_step, n_steps, ipw); break;
   }
   return hipGetLastError();
 }

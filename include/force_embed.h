@@ -71,7 +71,9 @@ enum { FE_NEG_STRICT = 0, FE_NEG_REDRAW = 1 };
  * summation order; unroll, waves_per_block and items_per_wave do not.      */
 typedef struct fe_tuning {
   int chunk;           /* rows of degree <= chunk: one wave per row; up to
-                          4*chunk: one block per row (<= 64)                */
+                          4*chunk: one block per row (<= 256). A launch
+                          with no block-row runs a kernel without that path
+                          (more waves per SIMD)                             */
   int max_chunks;      /* most block chunks (partials) one heavy row may use */
   int unroll;          /* rows in flight per lane group: 1, 2, 4 or 8       */
   int waves_per_block; /* 1, 2, 4 or 8                                      */
