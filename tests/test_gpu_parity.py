@@ -44,6 +44,52 @@ def test_hub_rows_split_into_chunks():
         _batch_parity(g, 128, 512, [(0, 0), (0, 1), (3, 7)], tuning=tun)
 
 
+def test_chunks_above_64():
+    # chunk > 64: a wave's row runs past one 64-column block (degree up to
+    # chunk), block items take rows up to 4*chunk
+    g = api.Graph.rmat(12, 16, seed=5)
+    deg = np.diff(g.row_ptr)
+    assert (deg > 64).sum() > 10 and (deg > 256).sum() > 0
+    for tun in ({"chunk": 128, "max_chunks": 8, "light": 16},
+                {"chunk": 256, "max_chunks": 1024, "light": 32},
+                {"chunk": 200, "max_chunks": 3, "light": 0}):
+        _batch_parity(g, 128, 512, [(0, 0), (0, 1), (3, 7)], tuning=tun)
+    _batch_parity(g, 64, 512, [(0, 0), (2, 2)], tuning={"chunk": 256, "max_chunks": 64})
+
+
+def test_block_free_and_block_batches_in_one_epoch():
+    # an SBM graph (degrees far below 64) with one hub in batch 0: batch 0's
+    # launch has block items, the others run the kernel without that path
+    g0 = api.Graph.sbm(2000, 8, 0.02, 0.001, seed=11)
+    deg0 = np.diff(g0.row_ptr)
+    u = np.repeat(np.arange(g0.n), deg0)
+    hub = np.full(300, 5)
+    others = np.arange(700, 1000)
+    g = api.Graph.from_edges(g0.n, np.concatenate([u, hub]),
+                             np.concatenate([g0.col_idx, others]))
+    deg = np.diff(g.row_ptr)
+    B = 256
+    assert deg[:B].max() > 64 and deg[B:].max() <= 64
+    _batch_parity(g, 128, B, [(0, 0), (0, 1), (1, 3), (2, 7)])
+    _batch_parity(g, 64, B, [(0, 0), (1, 4)], tuning={"chunk": 16, "max_chunks": 4})
+    # captured epochs give the bits of stream launches and of single steps
+    X = api.init_uniform(g.n, 128, seed=2)
+    lists = api.reference_negatives(g.n, 3 + 7, 5)[0]
+    res = []
+    for mode in ("graph", "stream", "steps"):
+        ctx = _ctx(g, 128, X, B, lists)
+        if mode == "steps":
+            for i in range(3):
+                for j in range(8):
+                    ctx.step(i, j)
+        else:
+            ctx.train(3, B, 5, 0.02, use_graph=(mode == "graph"))
+        res.append(ctx.download_x())
+        ctx.close()
+    np.testing.assert_array_equal(res[0], res[1])
+    np.testing.assert_array_equal(res[0], res[2])
+
+
 def test_quirks_off_and_no_negatives():
     g = api.Graph.sbm(1000, 4, 0.02, 0.002, seed=9)
     _batch_parity(g, 64, 300, [(0, 0), (2, 3)], quirks=0)

@@ -60,6 +60,15 @@ def test_spmm_hub_rows_and_quirk(quirks):
     _check(y, g, X, 0.37, quirks=quirks, what=f"rmat14 quirks={quirks}")
 
 
+def test_spmm_chunks_above_64():
+    """Rows of degree up to chunk > 64 one per wave (several 64-column
+    blocks), and block items of up to 4*chunk columns."""
+    g = api.Graph.rmat(13, 16, seed=3)
+    X = api.init_uniform(g.n, 128, -0.5, 0.5, seed=4)
+    for cut in (dict(chunk=128, max_chunks=8, light=16), dict(chunk=256, max_chunks=1024)):
+        _check(_spmm(g, X, 2048, 0.37, tuning=cut), g, X, 0.37, what=f"rmat13 {cut}")
+
+
 def test_spmm_independent_of_batch_size_and_launch_shape():
     g = api.Graph.rmat(13, 16, seed=8)
     X = api.init_uniform(g.n, 128, -0.5, 0.5, seed=1)
