@@ -125,6 +125,7 @@ struct EngineJob {
   hipEvent_t uploaded = nullptr; // uploads finished (on the upload stream)
   hipEvent_t done = nullptr;     // the job's last round finished
   bool waited_upload = false;
+  int state_epoch = -1;    // epoch the job's device state ints hold (-1: unset)
   int64_t units = 0;
 };
 
@@ -152,6 +153,7 @@ struct fe_engine {
   int stage_next = 0;
   size_t stage_cap = 0; // bytes
   char *dev_round = nullptr;
+  std::vector<char> one_host; // a round of one job, built on the host only
   int64_t rounds_run = 0;
   int64_t units_done = 0;
   double graph_prepare_ms = 0; // last fe_engine_run_graph: schedule, capture, instantiate
@@ -554,6 +556,49 @@ extern "C" fe_status fe_engine_run(fe_engine *e, int rounds, int *running) {
   return s;
 }
 
+// A round of one job: its step descriptor goes by value in the kernel
+// arguments of the single-step kernel, reading the epoch state from the
+// job's own device ints (set by a one-thread kernel when its epoch changes).
+// No staging copy precedes the launch (that copy and its wait cost about
+// 8 us a round), and the kernel body is the same, so the bits are those of
+// a round of many jobs.
+static fe_status run_one(fe_engine *e, std::vector<EngineJob *> &run, size_t rbytes) {
+  const int mode = round_mode(run), g = round_g(run), slice = round_slice(e, run);
+  if (slice == 0) {
+    mark_failed(run);
+    return efail(e, FE_ERR_STATE, "fe_engine_run: a job's slice differs from the engine's");
+  }
+  EngineJob *ej = run[0];
+  Job &j = ej->job;
+  const int epoch = ej->epoch;
+  fe_status s = FE_OK;
+  if (ej->state_epoch != epoch) {
+    const int shift = (int)(((int64_t)j.rot + epoch) % j.lay.P);
+    if ((s = ehip(e, launch_state(j.epoch, epoch, j.rot, shift, e->stream), "state set")) !=
+        FE_OK) {
+      mark_failed(run);
+      return s;
+    }
+    ej->state_epoch = epoch;
+  }
+  e->one_host.resize(rbytes);
+  int K = 0;
+  size_t lds = 0;
+  std::vector<EngineJob *> finished;
+  build_round(e, run, e->one_host.data(), K, lds, finished);
+  // from here on the job has advanced: a failure ends it
+  StepDesc sd = *reinterpret_cast<const StepDesc *>(e->one_host.data());
+  sd.state = j.epoch;
+  const int *pf = reinterpret_cast<const int *>(e->one_host.data() + sizeof(StepDesc)) + kStateInts;
+  s = ehip(e, launch_force1(sd, pf[1], e->tun.waves_per_block, e->tun.unroll, lds, mode, g, slice,
+                            e->stream),
+           "force launch");
+  for (EngineJob *f : finished)
+    if (s == FE_OK) s = ehip(e, hipEventRecord(f->done, e->stream), "event");
+  if (s != FE_OK) mark_failed(run);
+  return s;
+}
+
 static fe_status run_rounds(fe_engine *e, int rounds) {
   fe_status s = FE_OK;
   int done_rounds = 0;
@@ -563,7 +608,7 @@ static fe_status run_rounds(fe_engine *e, int rounds) {
       if (!jp->launched_all && !jp->collected) run.push_back(jp.get());
     if (run.empty()) break;
     const size_t rbytes = round_bytes(run.size(), (size_t)round_force_blocks(e, run));
-    if ((s = ensure_stage(e, rbytes)) != FE_OK) return s;
+    if (run.size() > 1 && (s = ensure_stage(e, rbytes)) != FE_OK) return s;
     // jobs whose uploads are still in flight join once they have landed
     for (EngineJob *ej : run) {
       if (!ej->waited_upload) {
@@ -571,6 +616,12 @@ static fe_status run_rounds(fe_engine *e, int rounds) {
           return s;
         ej->waited_upload = true;
       }
+    }
+    if (run.size() == 1) {
+      if ((s = run_one(e, run, rbytes)) != FE_OK) return s;
+      ++e->rounds_run;
+      ++done_rounds;
+      continue;
     }
     auto &stg = e->stage[e->stage_next];
     e->stage_next = (e->stage_next + 1) % kStages;

@@ -33,6 +33,8 @@ hipError_t launch_apply(const StepDesc *steps, const int *blk_prefix, int n_step
 // set != 0: every job's epoch counter := value; else += 1. Refreshes negcur.
 hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, int rot,
                         hipStream_t st);
+// One job's state ints {epoch, rot, shift} := the values (no negcur refresh).
+hipError_t launch_state(int *state, int epoch, int rot, int shift, hipStream_t st);
 hipError_t launch_to_bf16(const float *x, uint16_t *xs, int64_t count, hipStream_t st);
 hipError_t launch_fill_lcols(const int32_t *items, int n_items, const int32_t *col,
                              int32_t *lcols, hipStream_t st);

@@ -1501,6 +1501,20 @@ hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, int ro
   return hipGetLastError();
 }
 
+// One job's epoch state {epoch, rot, shift} := the given values.
+__global__ void state_kernel(int *state, int epoch, int rot, int shift) {
+  if (threadIdx.x == 0) {
+    state[0] = epoch;
+    state[1] = rot;
+    state[2] = shift;
+  }
+}
+
+hipError_t launch_state(int *state, int epoch, int rot, int shift, hipStream_t st) {
+  hipLaunchKernelGGL(state_kernel, dim3(1), dim3(64), 0, st, state, epoch, rot, shift);
+  return hipGetLastError();
+}
+
 template <int U, int MODE>
 static void launch_spmm_m(dim3 grid, dim3 block, size_t lds_bytes, hipStream_t st, int g_const,
                           const StepDesc *steps, const int *blk_prefix, int n_steps, float *Y) {
