@@ -225,6 +225,10 @@ void job_free(Job &j) {
   j.device_bytes = 0;
 }
 
+namespace {
+hipError_t h2d_staged(void *dst, const void *src, size_t bytes, hipStream_t st);
+}
+
 fe_status job_upload_csr(Job &j, const int64_t *row_ptr, const int32_t *col, int64_t nnz,
                          hipStream_t st) {
   if (!row_ptr || (nnz > 0 && !col) || nnz < 0 || nnz >= 2147483647LL) {
@@ -262,9 +266,7 @@ fe_status job_upload_csr(Job &j, const int64_t *row_ptr, const int32_t *col, int
   }
   fe_status s = ensure_cap(j, j.col, j.cap_col, (size_t)nnz, "CSR columns");
   if (s != FE_OK) return s;
-  if (nnz > 0)
-    FE_HIP(j, hipMemcpyAsync(j.col, col, sizeof(int32_t) * nnz, hipMemcpyHostToDevice, st),
-           "fe_upload_csr: copy");
+  if (nnz > 0) FE_HIP(j, h2d_staged(j.col, col, sizeof(int32_t) * nnz, st), "fe_upload_csr: copy");
   FE_HIP(j, hipStreamSynchronize(st), "fe_upload_csr: sync");
   j.row_ptr.assign(row_ptr, row_ptr + j.n + 1);
   j.skip_deg.swap(skip);
@@ -282,8 +284,7 @@ fe_status job_upload_x(Job &j, const float *x, hipStream_t st) {
   }
   const size_t count = (size_t)j.n * j.d;
   // the layout at rest with rotation 0: row c at position c
-  FE_HIP(j, hipMemcpyAsync(j.X, x, sizeof(float) * count, hipMemcpyHostToDevice, st),
-         "fe_upload_x: copy");
+  FE_HIP(j, h2d_staged(j.X, x, sizeof(float) * count, st), "fe_upload_x: copy");
   if (j.dtype == FE_GATHER_BF16)
     FE_HIP(j, launch_to_bf16(j.X, j.Xs, (int64_t)count, st), "fe_upload_x: shadow refresh");
   FE_HIP(j, hipMemsetAsync(j.epoch, 0, sizeof(int) * 4, st), "fe_upload_x: state");
@@ -293,10 +294,12 @@ fe_status job_upload_x(Job &j, const float *x, hipStream_t st) {
   return FE_OK;
 }
 
-// Device-to-host copies through pinned memory: two 16 MiB chunks per
-// device (allocated on first use, kept for the process), the DMA of one
-// overlapping the parallel host copy of the other out of the other. Copies
-// on one device share its chunks and take turns.
+// Copies between host and device through pinned memory: two 16 MiB chunks
+// per device and direction (allocated on first use, kept for the process),
+// the DMA of one overlapping the parallel host copy of the other. Copies in
+// one direction on one device share its chunks and take turns; uploads and
+// downloads (an engine's submitting and collecting threads) do not wait for
+// each other.
 namespace {
 struct PinnedStage {
   std::mutex mu;
@@ -305,7 +308,8 @@ struct PinnedStage {
 };
 constexpr size_t kStageChunk = (size_t)16 << 20;
 constexpr int kMaxDevices = 64;
-PinnedStage g_stage[kMaxDevices];
+PinnedStage g_stage[kMaxDevices];    // device to host
+PinnedStage g_stage_up[kMaxDevices]; // host to device
 
 struct CopySeg {
   void *dst;
@@ -324,6 +328,17 @@ void host_copy(void *dst, const void *src, size_t bytes) {
   }
 }
 
+// The current device's chunks of one direction, allocated on first use
+// (the caller holds ps.mu).
+hipError_t stage_ready(PinnedStage &ps) {
+  hipError_t e = hipSuccess;
+  for (int s = 0; s < 2 && e == hipSuccess; ++s) {
+    if (!ps.buf[s]) e = hipHostMalloc((void **)&ps.buf[s], kStageChunk, hipHostMallocDefault);
+    if (e == hipSuccess && !ps.ev[s]) e = hipEventCreateWithFlags(&ps.ev[s], hipEventDisableTiming);
+  }
+  return e;
+}
+
 hipError_t d2h_staged(const std::vector<CopySeg> &segs, hipStream_t st) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
@@ -331,11 +346,7 @@ hipError_t d2h_staged(const std::vector<CopySeg> &segs, hipStream_t st) {
   if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
   PinnedStage &ps = g_stage[dev];
   std::lock_guard<std::mutex> lk(ps.mu);
-  for (int s = 0; s < 2 && e == hipSuccess; ++s) {
-    if (!ps.buf[s]) e = hipHostMalloc((void **)&ps.buf[s], kStageChunk, hipHostMallocDefault);
-    if (e == hipSuccess && !ps.ev[s]) e = hipEventCreateWithFlags(&ps.ev[s], hipEventDisableTiming);
-  }
-  if (e != hipSuccess) return e;
+  if ((e = stage_ready(ps)) != hipSuccess) return e;
   CopySeg held[2] = {{nullptr, nullptr, 0}, {nullptr, nullptr, 0}}; // piece in each chunk
   auto drain = [&](int s) -> hipError_t {
     if (!held[s].bytes) return hipSuccess;
@@ -360,6 +371,42 @@ hipError_t d2h_staged(const std::vector<CopySeg> &segs, hipStream_t st) {
   // the older of the two chunks first
   for (int i = 0; i < 2; ++i) {
     hipError_t r = drain((k + i) & 1);
+    if (e == hipSuccess) e = r;
+  }
+  return e;
+}
+
+// dst (device) := src (host), bytes, on stream st; done when it returns, so
+// the caller may reuse src. Copies below one chunk go straight from src.
+hipError_t h2d_staged(void *dst, const void *src, size_t bytes, hipStream_t st) {
+  if (bytes < kStageChunk) {
+    hipError_t e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, st);
+    return e == hipSuccess ? hipStreamSynchronize(st) : e;
+  }
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  PinnedStage &ps = g_stage_up[dev];
+  std::lock_guard<std::mutex> lk(ps.mu);
+  if ((e = stage_ready(ps)) != hipSuccess) return e;
+  bool used[2] = {false, false};
+  int k = 0;
+  for (size_t off = 0; off < bytes && e == hipSuccess; off += kStageChunk, ++k) {
+    const int s = k & 1;
+    // chunk s is free once its previous DMA has finished
+    if (used[s]) e = hipEventSynchronize(ps.ev[s]);
+    const size_t len = std::min(kStageChunk, bytes - off);
+    if (e == hipSuccess) host_copy(ps.buf[s], static_cast<const char *>(src) + off, len);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(static_cast<char *>(dst) + off, ps.buf[s], len, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(ps.ev[s], st);
+    if (e == hipSuccess) used[s] = true;
+  }
+  // the chunks are handed to the next copy only after their DMAs
+  for (int s = 0; s < 2; ++s) {
+    if (!used[s]) continue;
+    hipError_t r = hipEventSynchronize(ps.ev[s]);
     if (e == hipSuccess) e = r;
   }
   return e;
@@ -615,8 +662,7 @@ fe_status job_prepare(Job &j, hipStream_t st, bool slots) {
     if ((s = ensure_cap(j, j.counters, j.cap_ctr, (size_t)std::max(j.plan.max_splits, 1),
                         "split counters")) != FE_OK)
       return s;
-    FE_HIP(j, hipMemcpyAsync(j.items, j.plan.items.data(), sizeof(int32_t) * j.plan.items.size(),
-                             hipMemcpyHostToDevice, st),
+    FE_HIP(j, h2d_staged(j.items, j.plan.items.data(), sizeof(int32_t) * j.plan.items.size(), st),
            "work items copy");
     if (n_lcols > 0)
       FE_HIP(j, launch_fill_lcols(j.items, (int)(j.plan.items.size() / kItemInts), j.col,
