@@ -110,6 +110,7 @@ struct alignas(16) LossDesc {
 };
 
 constexpr int kMaxLdsFloats = 8192; // staged negative rows: ns * D <= 8192
+constexpr int kRoundBlobBytes = 3584; // round buffers passed as kernel arguments
 constexpr int kMaxNs = 64;
 
 // Scratch array of the step that allocates nothing (fe_step_delta_noalloc):

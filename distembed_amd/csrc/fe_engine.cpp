@@ -153,7 +153,8 @@ struct fe_engine {
   int stage_next = 0;
   size_t stage_cap = 0; // bytes
   char *dev_round = nullptr;
-  std::vector<char> one_host; // a round of one job, built on the host only
+  std::vector<char> one_host; // a round built on the host only (one job, or
+                              // small enough for kernel arguments)
   int64_t rounds_run = 0;
   int64_t units_done = 0;
   double graph_prepare_ms = 0; // last fe_engine_run_graph: schedule, capture, instantiate
@@ -189,6 +190,16 @@ static size_t round_bytes(size_t K, size_t F) {
   size_t b = K * sizeof(StepDesc);
   b += K * kStateInts * sizeof(int) + 2 * (K + 1) * sizeof(int) + F * sizeof(int);
   return (b + 255) / 256 * 256;
+}
+
+// The part of a round's buffer before its block-to-step table, in 16-byte
+// units: a round of at most kSearchSteps jobs runs without the table, its
+// blocks finding their step by a binary search of the prefix (at most three
+// dependent loads).
+constexpr int kSearchSteps = 8;
+static size_t round_head_bytes(size_t K) {
+  const size_t b = K * sizeof(StepDesc) + K * kStateInts * sizeof(int) + 2 * (K + 1) * sizeof(int);
+  return (b + 15) / 16 * 16;
 }
 
 // Whether a step of the next round of `run` has block items (a round
@@ -623,9 +634,20 @@ static fe_status run_rounds(fe_engine *e, int rounds) {
       ++done_rounds;
       continue;
     }
+    // a round of up to kSearchSteps jobs goes to the device without its
+    // block-to-step table (its blocks search the prefix) as the arguments of
+    // a one-block kernel; a larger one, whole, by a DMA copy from a pinned
+    // stage
+    const size_t head = round_head_bytes(run.size());
+    const bool small = run.size() <= (size_t)kSearchSteps && head <= (size_t)kRoundBlobBytes;
     auto &stg = e->stage[e->stage_next];
-    e->stage_next = (e->stage_next + 1) % kStages;
-    if ((s = ehip(e, hipEventSynchronize(stg.copied), "staging reuse")) != FE_OK) return s;
+    if (!small) {
+      e->stage_next = (e->stage_next + 1) % kStages;
+      if ((s = ehip(e, hipEventSynchronize(stg.copied), "staging reuse")) != FE_OK) return s;
+    } else {
+      e->one_host.resize(rbytes);
+    }
+    char *host = small ? e->one_host.data() : stg.host;
     int K = 0;
     size_t lds = 0;
     std::vector<EngineJob *> finished;
@@ -635,19 +657,23 @@ static fe_status run_rounds(fe_engine *e, int rounds) {
       return efail(e, FE_ERR_STATE, "fe_engine_run: a job's slice differs from the engine's");
     }
     const bool blk = round_blk(run);
-    build_round(e, run, stg.host, K, lds, finished);
+    build_round(e, run, host, K, lds, finished);
     // from here on the jobs have advanced: a failure ends them all
-    const int *hpf = reinterpret_cast<const int *>(stg.host + K * sizeof(StepDesc)) + kStateInts * K;
+    const int *hpf = reinterpret_cast<const int *>(host + K * sizeof(StepDesc)) + kStateInts * K;
     const int fblocks = hpf[K];
-    s = ehip(e, hipMemcpyAsync(e->dev_round, stg.host, rbytes, hipMemcpyHostToDevice, e->stream),
-             "round copy");
-    if (s == FE_OK) s = ehip(e, hipEventRecord(stg.copied, e->stream), "event");
+    if (small) {
+      s = ehip(e, launch_blob(host, head, e->dev_round, e->stream), "round upload");
+    } else {
+      s = ehip(e, hipMemcpyAsync(e->dev_round, host, rbytes, hipMemcpyHostToDevice, e->stream),
+               "round copy");
+      if (s == FE_OK) s = ehip(e, hipEventRecord(stg.copied, e->stream), "event");
+    }
     const StepDesc *dsd = reinterpret_cast<const StepDesc *>(e->dev_round);
     const int *dpf = reinterpret_cast<const int *>(e->dev_round + K * sizeof(StepDesc)) + kStateInts * K;
     if (s == FE_OK)
-      s = ehip(e, launch_force(dsd, dpf, dpf + 2 * (K + 1), K, fblocks, e->tun.waves_per_block,
-                               e->tun.items_per_wave, e->tun.unroll, lds, round_mode(run),
-                               round_g(run), slice, blk, e->stream),
+      s = ehip(e, launch_force(dsd, dpf, small ? nullptr : dpf + 2 * (K + 1), K, fblocks,
+                               e->tun.waves_per_block, e->tun.items_per_wave, e->tun.unroll, lds,
+                               round_mode(run), round_g(run), slice, blk, e->stream),
                "force launch");
     for (EngineJob *ej : finished)
       if (s == FE_OK) s = ehip(e, hipEventRecord(ej->done, e->stream), "event");
@@ -821,7 +847,8 @@ extern "C" fe_status fe_engine_run_graph(fe_engine *e) {
       const RoundInfo &ri = rounds[r];
       const StepDesc *dsd = reinterpret_cast<const StepDesc *>(dsched + ri.offset);
       const int *dpf = reinterpret_cast<const int *>(dsched + ri.offset + ri.K * sizeof(StepDesc)) + kStateInts * ri.K;
-      ls = ehip(e, launch_force(dsd, dpf, dpf + 2 * (ri.K + 1), ri.K, ri.fblocks,
+      // rounds of one or two steps search the prefix (one load)
+      ls = ehip(e, launch_force(dsd, dpf, ri.K > 2 ? dpf + 2 * (ri.K + 1) : nullptr, ri.K, ri.fblocks,
                                 e->tun.waves_per_block, e->tun.items_per_wave, e->tun.unroll,
                                 ri.lds, ri.mode, ri.g_const, ri.slice, ri.blk, e->stream),
                 "force launch");

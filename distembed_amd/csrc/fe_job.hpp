@@ -15,8 +15,8 @@ namespace fe {
 // Launch wrappers (fe_kernels.hip).
 // mode: 0 every step gathers FP32, 1 every step gathers BF16, 2 mixed.
 // g_const: lanes per row shared by every step of the launch (0: they differ).
-// blk_step[b]: the step of block b (read when n_steps > 2; else the prefix
-// is searched). slice: components per lane of every step (each step's
+// blk_step[b]: the step of block b (null: blocks binary-search the prefix,
+// log2(n_steps) dependent loads). slice: components per lane of every step (each step's
 // G = D / slice). blk: some step has block items (false takes the kernel
 // without the block-item path; every step must then have none).
 hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, const int *blk_step,
@@ -35,6 +35,9 @@ hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, int ro
                         hipStream_t st);
 // One job's state ints {epoch, rot, shift} := the values (no negcur refresh).
 hipError_t launch_state(int *state, int epoch, int rot, int shift, hipStream_t st);
+// dst (device) := bytes of src (host), bytes <= kRoundBlobBytes and a
+// multiple of 16, by a one-block kernel whose arguments carry the bytes.
+hipError_t launch_blob(const void *src, size_t bytes, void *dst, hipStream_t st);
 hipError_t launch_to_bf16(const float *x, uint16_t *xs, int64_t count, hipStream_t st);
 hipError_t launch_fill_lcols(const int32_t *items, int n_items, const int32_t *col,
                              int32_t *lcols, hipStream_t st);

@@ -28,6 +28,7 @@
 // No MFMA: the step is a sparse gather, not a dense contraction.
 #include <hip/hip_runtime.h>
 #include <algorithm>
+#include <cstring>
 #include <mutex>
 #include "fe_device.hpp"
 
@@ -907,8 +908,8 @@ __device__ __forceinline__ void force_body(const StepDesc &sd, int lb, int nb) {
 }
 
 // Steps of one or more jobs in one launch (engine rounds): the block finds
-// its step (one load from the block-to-step table of a round of more than
-// two steps, else a search of the block prefix) and reads the step's
+// its step (one load from the block-to-step table when the round comes with
+// one, else a binary search of the block prefix) and reads the step's
 // descriptor from memory. C: components per lane, shared by every step;
 // BLK: some step has block items.
 template <int U, int MODE, int GT, int C, bool BLK>
@@ -916,7 +917,7 @@ __global__ __launch_bounds__(512) void force_kernel(
     const StepDesc *__restrict__ steps, const int *__restrict__ blk_prefix,
     const int *__restrict__ blk_step, int n_steps, int ipw) {
   const int b = blockIdx.x;
-  const int si = n_steps > 2 ? blk_step[b] : find_step(blk_prefix, n_steps, b);
+  const int si = blk_step ? blk_step[b] : find_step(blk_prefix, n_steps, b);
   force_body<U, MODE, GT, C, BLK>(steps[si], b - blk_prefix[si],
                                   blk_prefix[si + 1] - blk_prefix[si]);
   (void)ipw;
@@ -1498,6 +1499,26 @@ hipError_t launch_force(const StepDesc *steps, const int *blk_prefix, const int 
 hipError_t launch_epoch(const EpochDesc *jobs, int n, int set, int value, int rot,
                         hipStream_t st) {
   hipLaunchKernelGGL(epoch_kernel, dim3(n), dim3(256), 0, st, jobs, set, value, rot);
+  return hipGetLastError();
+}
+
+// A small round buffer, passed by value and written to device memory by one
+// block: in stream order in front of the round's launch, it costs a kernel
+// boundary instead of a DMA copy and its wait.
+struct RoundBlob {
+  u4v w[kRoundBlobBytes / 16];
+};
+
+__global__ __launch_bounds__(256) void blob_kernel(const RoundBlob b, int n_vec, u4v *dst) {
+  for (int i = threadIdx.x; i < n_vec; i += blockDim.x) dst[i] = b.w[i];
+}
+
+hipError_t launch_blob(const void *src, size_t bytes, void *dst, hipStream_t st) {
+  if (bytes > (size_t)kRoundBlobBytes || bytes % 16 != 0) return hipErrorInvalidValue;
+  RoundBlob b;
+  std::memcpy(&b, src, bytes);
+  hipLaunchKernelGGL(blob_kernel, dim3(1), dim3(256), 0, st, b, (int)(bytes / 16),
+                     static_cast<u4v *>(dst));
   return hipGetLastError();
 }
 
