@@ -65,6 +65,34 @@ def test_alone_equals_mixed_and_graph_schedule():
         np.testing.assert_array_equal(a, g)
 
 
+def test_every_round_size_gives_the_alone_bits():
+    """Eleven jobs of different lengths: rounds of more than eight jobs
+    (descriptors copied with the block-to-step table), of two to eight
+    (passed as kernel arguments, blocks search the prefix) and of one (the
+    single-step kernel) all give each job its alone result."""
+    specs = _jobs()
+    for k in range(7):
+        g = api.Graph.sbm(600 + 300 * k, 3 + k, 0.03, 0.002, seed=20 + k)
+        d = (32, 64, 128)[k % 3]
+        specs.append(dict(g=g, x0=api.init_uniform(g.n, d, seed=30 + k), batch_size=256,
+                          iterations=2 + k, gather=api.BF16 if k % 2 else api.FP32))
+    alone = []
+    for s in specs:
+        with api.Engine(2 * GiB) as e:
+            alone.append(_run_all(e, [_submit(e, s)])[0])
+    with api.Engine(2 * GiB) as e:
+        ids = [_submit(e, s) for s in specs]
+        mixed = _run_all(e, ids)
+    with api.Engine(2 * GiB) as e:
+        ids = [_submit(e, s) for s in specs]
+        e.run_graph()
+        graphed = [e.collect(i) for i in ids]
+    for a, m, g in zip(alone, mixed, graphed):
+        assert np.isfinite(a).all()
+        np.testing.assert_array_equal(a, m)
+        np.testing.assert_array_equal(a, g)
+
+
 def test_engine_job_equals_single_graph_context():
     s = _jobs()[0]
     with api.Engine(2 * GiB) as e:
