@@ -55,6 +55,7 @@ def test_chunks_above_64():
                 {"chunk": 200, "max_chunks": 3, "light": 0}):
         _batch_parity(g, 128, 512, [(0, 0), (0, 1), (3, 7)], tuning=tun)
     _batch_parity(g, 64, 512, [(0, 0), (2, 2)], tuning={"chunk": 256, "max_chunks": 64})
+    bf16_batch_parity(g, 128, 512, [(0, 0), (1, 7)], tuning={"chunk": 128, "max_chunks": 8})
 
 
 def test_block_free_and_block_batches_in_one_epoch():
