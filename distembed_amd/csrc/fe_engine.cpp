@@ -192,8 +192,8 @@ static size_t round_bytes(size_t K, size_t F) {
   return (b + 255) / 256 * 256;
 }
 
-// The part of a round's buffer before its block-to-step table, in 16-byte
-// units: a round of at most kSearchSteps jobs runs without the table, its
+// The bytes of a round's buffer before its block-to-step table, rounded up
+// to 16: a round of at most kSearchSteps jobs runs without the table, its
 // blocks finding their step by a binary search of the prefix (at most three
 // dependent loads).
 constexpr int kSearchSteps = 8;
