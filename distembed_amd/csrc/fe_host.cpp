@@ -560,7 +560,10 @@ extern "C" fe_status fe_read_mtx(const char *path, fe_graph *out) {
   return build_csr(rows, u.data(), v.data(), (int64_t)u.size(), false, false, out);
 }
 
-// Embedding writer (cpp/io/parrallel_IO.hpp:90-146 at world size 1).
+// Embedding writer (cpp/io/parrallel_IO.hpp:90-146 at world size 1): one
+// line per row, "%lu" (1-based id) then " %.5f" per component. Rows are
+// formatted in parallel, a group of chunks at a time, and written in order,
+// so the file is byte for byte what one formatted stream gives.
 extern "C" fe_status fe_write_embedding(const char *path, const float *x, int64_t n, int d) {
   if (!path || (!x && n > 0) || n < 0 || d <= 0) {
     set_host_error("fe_write_embedding: invalid argument");
@@ -571,12 +574,42 @@ extern "C" fe_status fe_write_embedding(const char *path, const float *x, int64_
     set_host_error(std::string("fe_write_embedding: cannot open ") + path);
     return FE_ERR_INVALID_ARG;
   }
-  for (int64_t i = 0; i < n; ++i) {
-    fprintf(f, "%lu", (unsigned long)(i + 1));
-    for (int j = 0; j < d; ++j) fprintf(f, " %.5f", (double)x[i * d + j]);
-    fputc('\n', f);
+  constexpr int64_t kRows = 512;   // rows per chunk
+  constexpr int kGroup = 64;       // chunks formatted before a write
+  const int64_t chunks = (n + kRows - 1) / kRows;
+  std::vector<std::string> buf((size_t)std::min<int64_t>(chunks, kGroup));
+  bool ok = true;
+  int oom = 0; // an exception cannot leave the parallel region
+  for (int64_t c0 = 0; c0 < chunks && ok && !oom; c0 += kGroup) {
+    const int m = (int)std::min<int64_t>(kGroup, chunks - c0);
+#pragma omp parallel for schedule(dynamic, 1) reduction(| : oom)
+    for (int k = 0; k < m; ++k) {
+      try {
+        std::string &s = buf[(size_t)k];
+        s.clear();
+        char tmp[64]; // " %.5f" of the largest float takes 47 characters
+        const int64_t r0 = (c0 + k) * kRows, r1 = std::min(n, r0 + kRows);
+        for (int64_t i = r0; i < r1; ++i) {
+          int len = snprintf(tmp, sizeof tmp, "%lu", (unsigned long)(i + 1));
+          s.append(tmp, (size_t)len);
+          for (int j = 0; j < d; ++j) {
+            len = snprintf(tmp, sizeof tmp, " %.5f", (double)x[i * d + j]);
+            s.append(tmp, (size_t)std::min<int>(len, (int)sizeof tmp - 1));
+          }
+          s.push_back('\n');
+        }
+      } catch (...) {
+        oom = 1;
+      }
+    }
+    for (int k = 0; k < m && ok && !oom; ++k)
+      ok = fwrite(buf[(size_t)k].data(), 1, buf[(size_t)k].size(), f) == buf[(size_t)k].size();
   }
-  const bool ok = fclose(f) == 0;
+  ok = (fclose(f) == 0) && ok;
+  if (oom) {
+    set_host_error("fe_write_embedding: out of host memory");
+    return FE_ERR_OOM;
+  }
   if (!ok) {
     set_host_error("fe_write_embedding: write failed");
     return FE_ERR_INVALID_ARG;

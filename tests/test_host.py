@@ -149,6 +149,24 @@ def test_embedding_writer_format(tmp_path):
     assert p.read_text() == "1 0.50000 -1.25000\n2 0.00000 3.00000\n"
 
 
+def test_embedding_writer_matches_one_formatted_stream(tmp_path):
+    # more rows than one group of parallel chunks (64 x 512), widths of
+    # every size up to the largest float, and infinities
+    rng = np.random.default_rng(7)
+    n, d = 40_000, 3
+    x = (rng.standard_normal((n, d)) * 10.0 ** rng.integers(-8, 8, (n, d))).astype(np.float32)
+    x[5, 1] = np.finfo(np.float32).max
+    x[6, 2] = -np.finfo(np.float32).max
+    x[7, 0] = np.inf
+    x[8, 1] = -np.inf
+    x[9, 2] = -0.0
+    p = tmp_path / "emb.txt"
+    api.write_embedding(p, x)
+    want = "".join(f"{i + 1}" + "".join(" %.5f" % float(v) for v in row) + "\n"
+                   for i, row in enumerate(x))
+    assert p.read_text() == want
+
+
 def test_init_uniform_range_and_determinism():
     a = api.init_uniform(1000, 64, -0.5, 0.5, seed=0)
     b = api.init_uniform(1000, 64, -0.5, 0.5, seed=0)
