@@ -589,6 +589,7 @@ extern "C" fe_status fe_write_embedding(const char *path, const float *x, int64_
         s.clear();
         char tmp[64]; // " %.5f" of the largest float takes 47 characters
         const int64_t r0 = (c0 + k) * kRows, r1 = std::min(n, r0 + kRows);
+        s.reserve((size_t)(r1 - r0) * ((size_t)d * 10 + 9)); // " -x.xxxxx" per component
         for (int64_t i = r0; i < r1; ++i) {
           int len = snprintf(tmp, sizeof tmp, "%lu", (unsigned long)(i + 1));
           s.append(tmp, (size_t)len);
