@@ -16,7 +16,9 @@
 // Additional flags:
 //   -rmat SCALE [-edge_factor 16] [-seed S]     R-MAT graph (a=0.57, b=c=0.19)
 //   -sbm N [-blocks 10] [-p_in P] [-p_out P] [-seed S]   stochastic block model
-//   -dim D (128; 8..512, multiple of 8; the reference fixes 128 at compile time)
+//   -dim D (128; one of 2, 3, 8, 16, 32, 64, 100, 128, 256, 512: each is a
+//           compile-time instantiation, as the reference's dimension is; the
+//           library itself takes any d in [1, 512])
 //   -gather fp32|bf16 (fp32)  -device I (0)  -quirks Q (3 = reference behaviour)
 //   -init_seed S (0)
 // Outputs, as the reference: DIR/embedding.txt (parrallel_IO.hpp:90-146
@@ -272,15 +274,19 @@ int main(int argc, char **argv) {
     }
     std::cout << " rank 0 gROWs  " << mat.gRows << "gCols" << mat.gCols << std::endl;
     switch (o.dim) {
+    case 2: return run<2>(o, &mat);
+    case 3: return run<3>(o, &mat);
     case 8: return run<8>(o, &mat);
     case 16: return run<16>(o, &mat);
     case 32: return run<32>(o, &mat);
     case 64: return run<64>(o, &mat);
+    case 100: return run<100>(o, &mat);
     case 128: return run<128>(o, &mat);
     case 256: return run<256>(o, &mat);
     case 512: return run<512>(o, &mat);
     default:
-      std::cerr << "dist_embed: -dim must be one of 8, 16, 32, 64, 128, 256, 512" << std::endl;
+      std::cerr << "dist_embed: -dim must be one of 2, 3, 8, 16, 32, 64, 100, 128, 256, 512"
+                << std::endl;
       return 2;
     }
   } catch (const std::exception &e) {

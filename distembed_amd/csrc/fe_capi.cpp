@@ -283,9 +283,11 @@ extern "C" fe_status fe_step_delta(fe_ctx *c, int epoch, int batch, float *delta
   if ((s = from_job(c, job_set_epoch(j, epoch, c->stream))) != FE_OK)
     return s;
   if ((s = from_job(c, job_launch_force(j, batch, DESC_DELTA, c->stream, nullptr))) != FE_OK) return s;
-  const size_t count = (size_t)rows_of_batch(j, batch) * j.d;
-  if ((s = hip_ok(c, hipMemcpyAsync(delta_out, j.S, sizeof(float) * count, hipMemcpyDeviceToHost,
-                                    c->stream),
+  // the batch's rows of d floats, as the caller's rows of dx
+  const size_t rows = (size_t)rows_of_batch(j, batch);
+  if ((s = hip_ok(c, hipMemcpy2DAsync(delta_out, sizeof(float) * j.dx, j.S, sizeof(float) * j.d,
+                                      sizeof(float) * j.dx, rows, hipMemcpyDeviceToHost,
+                                      c->stream),
                   "delta copy")) != FE_OK)
     return s;
   return hip_ok(c, hipStreamSynchronize(c->stream), "fe_step_delta sync");
@@ -297,6 +299,11 @@ extern "C" fe_status fe_step_delta_noalloc(fe_ctx *c, int epoch, int batch, floa
   if (!delta_dev) return fail(c, FE_ERR_INVALID_ARG, "fe_step_delta_noalloc: NULL delta_dev");
   if (!c->scratch) return fail(c, FE_ERR_HIP, "fe_step_delta_noalloc: " + c->scratch_err);
   Job &j = c->job;
+  if (j.dx != j.d)
+    return fail(c, FE_ERR_INVALID_ARG,
+                "fe_step_delta_noalloc: d = " + std::to_string(j.dx) +
+                    " is held padded to " + std::to_string(j.d) +
+                    " components; this entry point needs d = 8 x a power of two");
   // preparing the plan or the descriptors would allocate: a prepared
   // context is required
   if (j.nnz < 0 || !j.x_ready || j.B <= 0 || j.plan_dirty || j.steps_dirty || !j.edesc)

@@ -260,24 +260,25 @@ static fe_status validate_desc(const fe_job_desc *d, std::string &err) {
 static fe_status plan_job(const fe_job_desc *d, const Tuning &tun, Job &tmp, std::string &err) {
   fe_status s = validate_desc(d, err);
   if (s != FE_OK) return s;
+  if (d->d < 1 || d->d > 512) {
+    err = "job description: d must be in [1, 512]";
+    return FE_ERR_INVALID_ARG;
+  }
   tmp.n = d->n;
-  tmp.d = d->d;
+  tmp.dx = d->d;
+  tmp.d = padded_width(d->d); // the device width, as job_init sets it
   tmp.dtype = d->gather_dtype;
   tmp.nnz = d->nnz;
   tmp.ns = d->ns;
   tmp.tun = tun;
   const int64_t c_limit = (d->quirks & FE_QUIRK_SKIP_LAST_COL) ? d->n - 1 : d->n;
-  if (d->d < 8 || d->d > 512 || d->d % 8 != 0 || ((d->d / 8) & (d->d / 8 - 1)) != 0) {
-    err = "job description: d must be 8 x a power of two in [8, 512]";
-    return FE_ERR_INVALID_ARG;
-  }
-  if (d->d < tun.slice) {
+  if (tmp.d < tun.slice) {
     err = "job description: d below the engine's slice (" + std::to_string(tun.slice) +
           " components per lane)";
     return FE_ERR_INVALID_ARG;
   }
   s = build_plan(d->n, d->row_ptr, nullptr, d->batch_size, tun.chunk, tun.max_chunks, tun.light,
-                 64 / (d->d / tun.slice), c_limit, &tmp.plan);
+                 64 / (tmp.d / tun.slice), c_limit, &tmp.plan);
   if (s != FE_OK) {
     err = host_error();
     return s;
@@ -969,7 +970,8 @@ extern "C" fe_status fe_engine_autotune(fe_engine *e, const fe_job_desc *jobs, i
   int n = 0;
   std::vector<int> ids;
   int min_d = 512;
-  for (const fe_job_desc &d : descs) min_d = std::min(min_d, d.d);
+  for (const fe_job_desc &d : descs)
+    min_d = std::min(min_d, (d.d >= 1 && d.d <= 512) ? padded_width(d.d) : d.d);
   std::vector<std::pair<int, int>> cuts; // (slice, light)
   for (int slice : {8, 16})
     for (int light : {8, 16, 32})

@@ -172,8 +172,8 @@ fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks, int slo
     j.err = "n must be in [1, 2^31-1)";
     return FE_ERR_INVALID_ARG;
   }
-  if (d < 8 || d > 512 || d % 8 != 0 || ((d / 8) & (d / 8 - 1)) != 0) {
-    j.err = "d must be 8 x a power of two in [8, 512]";
+  if (d < 1 || d > 512) {
+    j.err = "d must be in [1, 512]";
     return FE_ERR_INVALID_ARG;
   }
   if (dtype != FE_GATHER_FP32 && dtype != FE_GATHER_BF16) {
@@ -185,6 +185,8 @@ fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks, int slo
     return FE_ERR_INVALID_ARG;
   }
   j.n = n;
+  j.dx = d;
+  d = padded_width(d);
   j.d = d;
   j.G = d / 8;
   j.dtype = dtype;
@@ -227,6 +229,8 @@ void job_free(Job &j) {
 
 namespace {
 hipError_t h2d_staged(void *dst, const void *src, size_t bytes, hipStream_t st);
+hipError_t h2d_rows(void *dst, const void *src, int64_t rows, size_t src_row, size_t dst_row,
+                    hipStream_t st);
 }
 
 fe_status job_upload_csr(Job &j, const int64_t *row_ptr, const int32_t *col, int64_t nnz,
@@ -283,8 +287,10 @@ fe_status job_upload_x(Job &j, const float *x, hipStream_t st) {
     return FE_ERR_INVALID_ARG;
   }
   const size_t count = (size_t)j.n * j.d;
-  // the layout at rest with rotation 0: row c at position c
-  FE_HIP(j, h2d_staged(j.X, x, sizeof(float) * count, st), "fe_upload_x: copy");
+  // the layout at rest with rotation 0: row c at position c; the caller's
+  // rows of dx floats, zero-padded to d
+  FE_HIP(j, h2d_rows(j.X, x, j.n, sizeof(float) * (size_t)j.dx, sizeof(float) * (size_t)j.d, st),
+         "fe_upload_x: copy");
   if (j.dtype == FE_GATHER_BF16)
     FE_HIP(j, launch_to_bf16(j.X, j.Xs, (int64_t)count, st), "fe_upload_x: shadow refresh");
   FE_HIP(j, hipMemsetAsync(j.epoch, 0, sizeof(int) * 4, st), "fe_upload_x: state");
@@ -311,10 +317,15 @@ constexpr int kMaxDevices = 64;
 PinnedStage g_stage[kMaxDevices];    // device to host
 PinnedStage g_stage_up[kMaxDevices]; // host to device
 
+// bytes of device rows from src (device) to dst (host); with src_row > 0,
+// rows of src_row bytes of which the first dst_row go to consecutive rows
+// of dst_row bytes (the padding of the device rows is dropped)
 struct CopySeg {
   void *dst;
   const void *src;
   size_t bytes;
+  size_t src_row = 0;
+  size_t dst_row = 0;
 };
 
 void host_copy(void *dst, const void *src, size_t bytes) {
@@ -325,6 +336,18 @@ void host_copy(void *dst, const void *src, size_t bytes) {
     const size_t off = (size_t)b * block;
     std::memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off,
                 std::min(block, bytes - off));
+  }
+}
+
+// `rows` rows of row_bytes each, from src (a row every src_stride bytes) to
+// dst (a row every dst_stride bytes); zero_tail > 0 clears that many bytes
+// after each dst row (the padding of a device row)
+void host_copy_rows(char *dst, size_t dst_stride, const char *src, size_t src_stride,
+                    size_t row_bytes, int64_t rows, size_t zero_tail) {
+#pragma omp parallel for schedule(static) if (rows > 4096)
+  for (int64_t r = 0; r < rows; ++r) {
+    std::memcpy(dst + (size_t)r * dst_stride, src + (size_t)r * src_stride, row_bytes);
+    if (zero_tail) std::memset(dst + (size_t)r * dst_stride + row_bytes, 0, zero_tail);
   }
 }
 
@@ -351,20 +374,30 @@ hipError_t d2h_staged(const std::vector<CopySeg> &segs, hipStream_t st) {
   auto drain = [&](int s) -> hipError_t {
     if (!held[s].bytes) return hipSuccess;
     hipError_t r = hipEventSynchronize(ps.ev[s]);
-    if (r == hipSuccess) host_copy(held[s].dst, ps.buf[s], held[s].bytes);
+    const CopySeg &h = held[s];
+    if (r == hipSuccess) {
+      if (h.src_row && h.dst_row != h.src_row)
+        host_copy_rows(static_cast<char *>(h.dst), h.dst_row, ps.buf[s], h.src_row, h.dst_row,
+                       (int64_t)(h.bytes / h.src_row), 0);
+      else
+        host_copy(h.dst, ps.buf[s], h.bytes);
+    }
     held[s].bytes = 0;
     return r;
   };
   int k = 0;
   for (const CopySeg &sg : segs) {
+    // a chunk (16 MiB) holds whole rows of any padded width (8 x 2^k floats)
+    const bool strided = sg.src_row && sg.dst_row != sg.src_row;
     for (size_t off = 0; off < sg.bytes && e == hipSuccess; off += kStageChunk) {
       const int s = k & 1;
       e = drain(s);
       const size_t len = std::min(kStageChunk, sg.bytes - off);
       const char *src = static_cast<const char *>(sg.src) + off;
+      char *dst = static_cast<char *>(sg.dst) + (strided ? off / sg.src_row * sg.dst_row : off);
       if (e == hipSuccess) e = hipMemcpyAsync(ps.buf[s], src, len, hipMemcpyDeviceToHost, st);
       if (e == hipSuccess) e = hipEventRecord(ps.ev[s], st);
-      if (e == hipSuccess) held[s] = {static_cast<char *>(sg.dst) + off, src, len};
+      if (e == hipSuccess) held[s] = {dst, src, len, sg.src_row, sg.dst_row};
       ++k;
     }
   }
@@ -411,6 +444,43 @@ hipError_t h2d_staged(void *dst, const void *src, size_t bytes, hipStream_t st) 
   }
   return e;
 }
+
+// `rows` rows of src_row bytes (host) into rows of dst_row >= src_row bytes
+// (device), the rest of each device row zeroed, through the pinned chunks;
+// done when it returns. Equal widths take h2d_staged.
+hipError_t h2d_rows(void *dst, const void *src, int64_t rows, size_t src_row, size_t dst_row,
+                    hipStream_t st) {
+  if (src_row == dst_row) return h2d_staged(dst, src, (size_t)rows * src_row, st);
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  PinnedStage &ps = g_stage_up[dev];
+  std::lock_guard<std::mutex> lk(ps.mu);
+  if ((e = stage_ready(ps)) != hipSuccess) return e;
+  const int64_t per = (int64_t)(kStageChunk / dst_row); // rows per chunk
+  bool used[2] = {false, false};
+  int k = 0;
+  for (int64_t r0 = 0; r0 < rows && e == hipSuccess; r0 += per, ++k) {
+    const int s = k & 1;
+    if (used[s]) e = hipEventSynchronize(ps.ev[s]);
+    const int64_t m = std::min(per, rows - r0);
+    if (e == hipSuccess)
+      host_copy_rows(ps.buf[s], dst_row, static_cast<const char *>(src) + (size_t)r0 * src_row,
+                     src_row, src_row, m, dst_row - src_row);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(static_cast<char *>(dst) + (size_t)r0 * dst_row, ps.buf[s],
+                         (size_t)m * dst_row, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventRecord(ps.ev[s], st);
+    if (e == hipSuccess) used[s] = true;
+  }
+  for (int s = 0; s < 2; ++s) {
+    if (!used[s]) continue;
+    hipError_t r = hipEventSynchronize(ps.ev[s]);
+    if (e == hipSuccess) e = r;
+  }
+  return e;
+}
 } // namespace
 
 fe_status job_download_x(Job &j, float *x, hipStream_t st) {
@@ -422,17 +492,20 @@ fe_status job_download_x(Job &j, float *x, hipStream_t st) {
     j.err = "fe_download_x: X was never uploaded";
     return FE_ERR_STATE;
   }
+  // device rows of d floats become the caller's rows of dx (the padding
+  // is dropped)
+  const size_t srow = sizeof(float) * (size_t)j.d, drow = sizeof(float) * (size_t)j.dx;
   std::vector<CopySeg> segs;
   if (j.rot == 0) {
-    segs.push_back({x, j.X, sizeof(float) * (size_t)j.n * j.d});
+    segs.push_back({x, j.X, srow * (size_t)j.n, srow, drow});
   } else {
     // one piece per batch, from its slot
     const Layout L = make_layout(j.lay_B, (int)((j.n + j.lay_B - 1) / j.lay_B));
     for (int k = 0; k < L.P - 1; ++k) {
       const int64_t r0 = (int64_t)k * L.B;
       const int64_t rows = std::min<int64_t>(r0 + L.B, j.n) - r0;
-      segs.push_back({x + r0 * j.d, j.X + rest_pos(j, L, j.rot, r0) * j.d,
-                      sizeof(float) * (size_t)rows * j.d});
+      segs.push_back({x + r0 * j.dx, j.X + rest_pos(j, L, j.rot, r0) * j.d, srow * (size_t)rows,
+                      srow, drow});
     }
   }
   FE_HIP(j, d2h_staged(segs, st), "fe_download_x: copy");
@@ -856,7 +929,11 @@ fe_status job_spmm(Job &j, int iterations, float lr, float *y_out, hipStream_t s
                           j.dtype == FE_GATHER_BF16 ? 1 : 0, j.G, j.Y, st),
            "SpMM kernel launch");
   if (iterations > 0)
-    FE_HIP(j, d2h_staged({{y_out, j.Y, sizeof(float) * (size_t)j.n * j.d}}, st), "fe_spmm: copy");
+    FE_HIP(j,
+           d2h_staged({{y_out, j.Y, sizeof(float) * (size_t)j.n * j.d, sizeof(float) * (size_t)j.d,
+                        sizeof(float) * (size_t)j.dx}},
+                      st),
+           "fe_spmm: copy");
   return FE_OK;
 }
 

@@ -88,7 +88,9 @@ struct Job {
   DeviceAllocator *allocator = nullptr; // null: hipMalloc / hipFree
   // shape
   int64_t n = 0;
-  int d = 0;
+  int d = 0;  // device row width: dx rounded up to 8 x a power of two
+  int dx = 0; // row width of the caller's arrays (the d it asked for); the
+              // components [dx, d) of every device row hold zeros
   int G = 0;
   int dtype = FE_GATHER_FP32;
   uint32_t quirks = FE_QUIRKS_REFERENCE;
@@ -160,6 +162,14 @@ struct Job {
 // All return FE_OK or an error with job.err set. `st` is the stream used.
 // slot_B > 0: X (and the shadow) get every slot of the layout of batch size
 // slot_B at once, so a fused run later needs no second allocation.
+// The device row width for an embedding of d components (1 <= d <= 512):
+// the smallest 8 x 2^k >= d. Zero components add exactly nothing to a
+// distance and get exactly zero force, so the padding changes no result.
+inline int padded_width(int d) {
+  int w = 8;
+  while (w < d) w *= 2;
+  return w;
+}
 fe_status job_init(Job &j, int64_t n, int d, int dtype, uint32_t quirks, int slot_B = 0);
 void job_free(Job &j);
 fe_status job_upload_csr(Job &j, const int64_t *row_ptr, const int32_t *col, int64_t nnz,

@@ -229,8 +229,7 @@ using core::Process3DGrid;
 using core::SpMat;
 
 template <typename INDEX_TYPE, typename VALUE_TYPE, size_t embedding_dim> class EmbeddingAlgo {
-  static_assert(embedding_dim % 8 == 0 && embedding_dim >= 8 && embedding_dim <= 512,
-                "embedding_dim must be a multiple of 8 in [8, 512]");
+  static_assert(embedding_dim >= 1 && embedding_dim <= 512, "embedding_dim must be in [1, 512]");
 
 protected:
   DenseMat<INDEX_TYPE, VALUE_TYPE, embedding_dim> *dense_local;
@@ -347,8 +346,7 @@ public:
 // dense_local and left unchanged; A' skips the columns the reference's
 // dst_id < dst_end_index skips (column N-1).
 template <typename INDEX_TYPE, typename VALUE_TYPE, size_t embedding_dim> class SpMMAlgo {
-  static_assert(embedding_dim % 8 == 0 && embedding_dim >= 8 && embedding_dim <= 512,
-                "embedding_dim must be a multiple of 8 in [8, 512]");
+  static_assert(embedding_dim >= 1 && embedding_dim <= 512, "embedding_dim must be in [1, 512]");
 
   DenseMat<INDEX_TYPE, VALUE_TYPE, embedding_dim> *dense_local_output;
   DenseMat<INDEX_TYPE, VALUE_TYPE, embedding_dim> *dense_local;

@@ -90,8 +90,12 @@ typedef struct fe_ctx fe_ctx;
 
 /* ---- lifecycle -------------------------------------------------------- */
 
-/* Creates a context on HIP device `device` for an N x d embedding.
- * d must be a multiple of 8 in [8, 512]; 0 < n < 2^31.                     */
+/* Creates a context on HIP device `device` for an N x d embedding,
+ * 1 <= d <= 512, 0 < n < 2^31. Every host array of rows (X, deltas, SpMM
+ * output) has rows of d floats. On the device a row is held at the next
+ * width 8 x 2^k >= d, its extra components zero: a zero component adds
+ * exactly nothing to a distance and gets exactly zero force, so the results
+ * are those of width d (the bytes moved are those of the wider rows).      */
 fe_status fe_create(fe_ctx **out, int device, int64_t n, int d,
                     int gather_dtype, uint32_t quirks);
 void fe_destroy(fe_ctx *ctx);
@@ -148,7 +152,8 @@ fe_status fe_step_delta(fe_ctx *ctx, int epoch, int batch, float *delta_out);
  * be prepared: one fe_step, fe_step_delta or fe_train call after the last
  * change of the CSR, X, negative lists, batch size or tuning prepares it;
  * otherwise FE_ERR_STATE. The call launches three kernels (counter fill,
- * epoch state, force) and nothing else.                                    */
+ * epoch state, force) and nothing else. It needs d = 8 x 2^k (a context of
+ * another d holds its rows padded: FE_ERR_INVALID_ARG).                    */
 fe_status fe_step_delta_noalloc(fe_ctx *ctx, int epoch, int batch, float *delta_dev,
                                 void *stream);
 /* algo_force2_vec_ns(iterations, batch_size, ns, lr) (cpp/algo/algo.hpp:74).

@@ -52,6 +52,18 @@ def test_driver_sbm_matches_oracle(tmp_path):
     assert st["Memory usage"] > 0 and st["Achieved GB/s"] > 0 and st["Edges+Negatives/s"] > 0
 
 
+@pytest.mark.parametrize("d", [2, 100])
+def test_driver_widths_below_and_between_powers_of_two(tmp_path, d):
+    # the reference's dimension is any compile-time constant: a 2-D embedding
+    # (visualisation) and d = 100 (held padded to 8 and 128 on the device)
+    n, B, T = 900, 128, 3
+    got, _ = _run(tmp_path, "-sbm", str(n), "-dim", str(d), "-batch", str(B), "-iter", str(T),
+                  "-nsamples", "5", "-lr", "0.02")
+    assert got.shape == (n, d)
+    want = _oracle_x(_sbm(n), d, B, T, 5)
+    np.testing.assert_allclose(got, want, rtol=0, atol=5e-6 + 1e-4 * np.abs(want).max())
+
+
 def test_driver_mtx_input_equals_generated_graph(tmp_path):
     n, d, B, T = 700, 32, 100, 3
     g = _sbm(n)

@@ -69,11 +69,12 @@ def test_every_round_size_gives_the_alone_bits():
     """Eleven jobs of different lengths: rounds of more than eight jobs
     (descriptors copied with the block-to-step table), of two to eight
     (passed as kernel arguments, blocks search the prefix) and of one (the
-    single-step kernel) all give each job its alone result."""
+    single-step kernel) all give each job its alone result. Some widths are
+    held padded on the device (3, 100, 12, 2)."""
     specs = _jobs()
     for k in range(7):
         g = api.Graph.sbm(600 + 300 * k, 3 + k, 0.03, 0.002, seed=20 + k)
-        d = (32, 64, 128)[k % 3]
+        d = (32, 3, 100, 64, 128, 12, 2)[k]
         specs.append(dict(g=g, x0=api.init_uniform(g.n, d, seed=30 + k), batch_size=256,
                           iterations=2 + k, gather=api.BF16 if k % 2 else api.FP32))
     alone = []

@@ -385,8 +385,52 @@ def test_bad_arguments_return_errors():
     with pytest.raises(api.FeError):
         ctx.step(0, 4)  # batch out of range
     ctx.close()
-    with pytest.raises(api.FeError):
-        api.ForceEmbed(100, 24)  # d/8 not a power of two
+    for d in (0, 513):
+        with pytest.raises(api.FeError):
+            api.ForceEmbed(100, d)
+
+
+@pytest.mark.parametrize("d", [2, 3, 12, 100, 200])
+def test_any_width_against_oracle(d):
+    # rows held zero-padded to 8 x 2^k on the device; the caller sees width d
+    g = api.Graph.sbm(700, 7, 0.05, 0.004, seed=3)
+    _batch_parity(g, d, 128, [(0, 0), (2, 5)])
+    g = api.Graph.rmat(11, 16, seed=2)
+    _batch_parity(g, d, 256, [(0, 0), (1, 3)], tuning={"chunk": 16, "max_chunks": 8})
+    bf16_batch_parity(g, d, 256, [(0, 0), (1, 5)])
+
+
+@pytest.mark.parametrize("d,gather", [(12, api.FP32), (3, api.FP32), (100, api.BF16)])
+def test_padded_width_equals_zero_columns(d, gather):
+    # a run at width d gives the bits of the run at the padded width whose
+    # extra columns are zero, and those columns stay zero
+    g = api.Graph.rmat(11, 16, seed=4)
+    D = 8
+    while D < d:
+        D *= 2
+    X = api.init_uniform(g.n, d, seed=5)
+    XD = np.zeros((g.n, D), dtype=np.float32)
+    XD[:, :d] = X
+    lists = api.reference_negatives(g.n, 4 + 8, 5)[0]
+    out = []
+    for x, w in ((X, d), (XD, D)):
+        ctx = api.ForceEmbed(g.n, w, gather=gather)
+        ctx.upload_csr(g)
+        ctx.upload_x(x)
+        ctx.set_negatives(lists)
+        ctx.configure(256, 0.02, 5.0)
+        ctx.train(4, 256, 5, 0.02)
+        out.append(ctx.download_x())
+        if w == d:
+            y = ctx.spmm(1, 256, 0.3)
+        else:
+            yD = ctx.spmm(1, 256, 0.3)
+        ctx.close()
+    np.testing.assert_array_equal(out[0], out[1][:, :d])
+    assert not out[1][:, d:].any()
+    np.testing.assert_array_equal(y, yD[:, :d])
+    assert not yD[:, d:].any()
+
 
 
 # ---- full-size workload (R-MAT scale 20, d = 128, B = 16384) ----------------------

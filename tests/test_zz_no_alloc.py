@@ -133,3 +133,18 @@ def test_noalloc_step_rejects_unprepared_context_and_oversized_batch():
     finally:
         hip.h.hipFree(C.c_void_p(p))
         ctx.close()
+    # a width held padded (d = 12 on rows of 16): refused before any launch
+    # (the buffer has room for padded rows all the same)
+    g = api.Graph.rmat(10, 8, seed=6)
+    lists = api.reference_negatives(g.n, 20, 5)[0]
+    ctx = _ctx(g, 12, 128, lists)
+    p = hip.malloc(128 * 16 * 4)
+    try:
+        ctx.step_delta(0, 0, 128)
+        with pytest.raises(api.FeError) as e:
+            ctx.step_delta_noalloc(0, 0, p)
+        assert e.value.status == _lib.FE_ERR_INVALID_ARG
+        assert "padded" in str(e.value)
+    finally:
+        hip.h.hipFree(C.c_void_p(p))
+        ctx.close()
