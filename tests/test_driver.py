@@ -22,7 +22,7 @@ using namespace distblas::core;
 using INDEX_TYPE = uint64_t;
 using VALUE_TYPE = double;
 int main(int argc, char **argv) {
-  const int dimension = 128;
+  const int dimension = DIMENSION;
   int world_size = 1, batch_size = 16384, iterations = 30, ns = 5;
   double alpha = 0, beta = 0.25, lr = 0.02;
   bool col_major = false, sync_comm = false;
@@ -50,9 +50,11 @@ int main(int argc, char **argv) {
 """
 
 
-def test_reference_driver_shape_compiles_against_mirror(tmp_path):
+@pytest.mark.parametrize("dim", [128, 2, 100])
+def test_reference_driver_shape_compiles_against_mirror(tmp_path, dim):
+    # the reference's dimension is a compile-time constant of any value
     src = tmp_path / "shape.cpp"
-    src.write_text(REFERENCE_SHAPED)
+    src.write_text(REFERENCE_SHAPED.replace("DIMENSION", str(dim)))
     r = subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only",
                         f"-I{ROOT / 'include'}", str(src)], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
