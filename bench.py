@@ -32,7 +32,8 @@ schedule(static)) on a bounded sample of the same workload.
 precision: R-MAT scale 20 and 22 at 5 and 10 negatives, FP32 and BF16
 (each autotuned): epoch times, BF16 speed-ups, final losses (also like for
 like, at the BF16 run's tiling) and per-step gradient errors of the BF16
-form against the FP32 kernel.
+form against the FP32 kernel; for R-MAT-22 at 5 negatives, the final
+losses from 4 initial embeddings as in loss_over_seeds (about 10 s each).
 serving: Poisson arrivals of R-MAT embedding jobs (2e4-2e7 edges, d from
 {32, 64, 128}, 50 epochs) at 4, 8 and 16 jobs/s under an 8 GiB budget:
 p50/p99 job latency (arrival to result on the host) and edges+negatives/s.
@@ -221,57 +222,6 @@ def run_form(api, g, X0, lists, gather, args, tuning=None):
             "loss_parts": (attr, rep), "use_graph": use_graph}
 
 
-def loss_over_seeds(api, g, lists, args, tun32, tun16, seeds):
-    """Final FP64 loss of the FP32 and the BF16 form after args.steps epochs
-    from each of several initial embeddings (seeds 1..seeds, same graph and
-    lists, each form with its tiling). One run's final loss is chaotic (the
-    summation order alone moves it by about a percent); the mean of the
-    per-seed paired differences, with its standard error, shows whether
-    the BF16 shadow shifts the loss."""
-    d, B, ns, lr = args.d, args.batch, args.ns, args.lr
-    ctxs = {}
-    rows = []
-    try:
-        for gather, tun in ((api.FP32, tun32), (api.BF16, tun16)):
-            ctx = api.ForceEmbed(g.n, d, gather=gather)
-            ctxs[gather] = ctx
-            ctx.upload_csr(g)
-            ctx.set_negatives(lists)
-            ctx.configure(B, lr, 5.0)
-            if tun:
-                ctx.set_tuning(**tun.get("tuning", {}))
-        for s in range(1, seeds + 1):
-            x0 = api.init_uniform(g.n, d, -0.5, 0.5, seed=s)
-            row = {"seed": s}
-            for gather, tun in ((api.FP32, tun32), (api.BF16, tun16)):
-                ctx = ctxs[gather]
-                ctx.upload_x(x0)
-                if s == 1 and tun and tun.get("per_batch"):
-                    ctx.set_batch_tunings(tun["per_batch"])
-                ctx.train(args.steps, B, ns, lr)
-                a, r = ctx.loss(0, ns)
-                row["bf16" if gather == api.BF16 else "fp32"] = a + r
-            del x0
-            row["rel_diff"] = (row["bf16"] - row["fp32"]) / row["fp32"]
-            rows.append(row)
-            log(f"loss seed {s}: fp32 {row['fp32']:.6e} bf16 {row['bf16']:.6e} "
-                f"({row['rel_diff'] * 100:+.2f}%)")
-    finally:
-        for ctx in ctxs.values():
-            ctx.close()
-    diffs = np.array([r["rel_diff"] for r in rows])
-    f32 = np.array([r["fp32"] for r in rows])
-    b16 = np.array([r["bf16"] for r in rows])
-    return {
-        "seeds": seeds, "epochs": args.steps,
-        "mean_rel_diff": float(diffs.mean()),
-        "stderr_rel_diff": float(diffs.std(ddof=1) / np.sqrt(len(diffs))) if len(diffs) > 1 else None,
-        "rel_diff_of_means": float((b16.mean() - f32.mean()) / f32.mean()),
-        "fp32_loss_cv_over_seeds": float(f32.std(ddof=1) / f32.mean()) if len(f32) > 1 else None,
-        "per_seed": rows,
-    }
-
-
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -443,8 +393,9 @@ def main() -> int:
                 (f32b["loss"] - f32["loss"]) / f32["loss"])
             log(f"fp32 with the bf16 tiling: loss {f32b['loss']:.6e}")
         if args.loss_seeds > 0:
-            out["bf16_form"]["loss_over_seeds"] = loss_over_seeds(
-                api, g, lists, args, f32["tuning"], b16["tuning"], args.loss_seeds)
+            from distembed_amd import bench_forms as bf
+            out["bf16_form"]["loss_over_seeds"] = bf.loss_over_seeds(
+                g, lists, d, B, ns, lr, args.steps, f32["tuning"], b16["tuning"], args.loss_seeds)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g, lists, d, B, ns, lr, args.cpu_budget,
                                            args.cpu_epochs)

@@ -81,6 +81,56 @@ def _train_timed(g, X0, lists, gather, B, ns, lr, steps, warmup, tuning=None):
             "tuning": tun}
 
 
+def loss_over_seeds(g, lists, d, B, ns, lr, steps, tun32, tun16, seeds):
+    """Final FP64 loss of the FP32 and the BF16 form after `steps` epochs
+    from each of several initial embeddings (seeds 1..seeds, same graph and
+    lists, each form with its tiling: {"tuning": ..., "per_batch": ...}).
+    One run's final loss is chaotic (the summation order alone moves it by
+    about a percent); the mean of the per-seed paired differences, with its
+    standard error, shows whether the BF16 shadow shifts the loss."""
+    ctxs = {}
+    rows = []
+    try:
+        for gather, tun in ((api.FP32, tun32), (api.BF16, tun16)):
+            ctx = api.ForceEmbed(g.n, d, gather=gather)
+            ctxs[gather] = ctx
+            ctx.upload_csr(g)
+            ctx.set_negatives(lists)
+            ctx.configure(B, lr, 5.0)
+            if tun:
+                ctx.set_tuning(**tun.get("tuning", {}))
+        for s in range(1, seeds + 1):
+            x0 = api.init_uniform(g.n, d, -0.5, 0.5, seed=s)
+            row = {"seed": s}
+            for gather, tun in ((api.FP32, tun32), (api.BF16, tun16)):
+                ctx = ctxs[gather]
+                ctx.upload_x(x0)
+                if s == 1 and tun and tun.get("per_batch"):
+                    ctx.set_batch_tunings(tun["per_batch"])
+                ctx.train(steps, B, ns, lr)
+                a, r = ctx.loss(0, ns)
+                row["bf16" if gather == api.BF16 else "fp32"] = a + r
+            del x0
+            row["rel_diff"] = (row["bf16"] - row["fp32"]) / row["fp32"]
+            rows.append(row)
+            log(f"loss seed {s}: fp32 {row['fp32']:.6e} bf16 {row['bf16']:.6e} "
+                f"({row['rel_diff'] * 100:+.2f}%)")
+    finally:
+        for ctx in ctxs.values():
+            ctx.close()
+    diffs = np.array([r["rel_diff"] for r in rows])
+    f32 = np.array([r["fp32"] for r in rows])
+    b16 = np.array([r["bf16"] for r in rows])
+    return {
+        "seeds": seeds, "epochs": steps,
+        "mean_rel_diff": float(diffs.mean()),
+        "stderr_rel_diff": float(diffs.std(ddof=1) / np.sqrt(len(diffs))) if len(diffs) > 1 else None,
+        "rel_diff_of_means": float((b16.mean() - f32.mean()) / f32.mean()),
+        "fp32_loss_cv_over_seeds": float(f32.std(ddof=1) / f32.mean()) if len(f32) > 1 else None,
+        "per_seed": rows,
+    }
+
+
 def _gradient_error(g, X0, lists, B, ns, lr, batches):
     """||delta_bf16 - delta_fp32|| / ||delta_fp32|| of single steps on the
     same X and lists (the BF16 form against the plain FP32 kernel)."""
@@ -106,13 +156,16 @@ def _gradient_error(g, X0, lists, B, ns, lr, batches):
     return out
 
 
-def precision(steps: int, warmup: int, cases=((20, 5), (20, 10), (22, 5), (22, 10))) -> dict:
+def precision(steps: int, warmup: int, cases=((20, 5), (20, 10), (22, 5), (22, 10)),
+              seed_cases=((22, 5),), seeds: int = 4) -> dict:
     """R-MAT at each (scale, ns) of `cases`, d=128, B=16384: FP32 and BF16
     epoch times (each autotuned), the BF16 speed-up, the final FP64 loss of
     both and of an FP32 run with the BF16 run's tiling (the tiling alone,
     i.e. the summation order, moves the final loss of these chaotic runs),
     and the per-step gradient error of the BF16 form against the FP32 kernel
-    on the first, second and last batch."""
+    on the first, second and last batch. For the cases in `seed_cases`, the
+    loss difference over `seeds` initial embeddings as well
+    (loss_over_seeds)."""
     out = []
     graphs = {}
     d, B, lr = 128, 16384, 0.02
@@ -132,6 +185,10 @@ def precision(steps: int, warmup: int, cases=((20, 5), (20, 10), (22, 5), (22, 1
         row["bf16"] = _train_timed(g, X0, lists, api.BF16, B, ns, lr, steps, warmup)
         same = _train_timed(g, X0, lists, api.FP32, B, ns, lr, steps, warmup,
                             tuning=row["bf16"]["tuning"])
+        if (scale, ns) in seed_cases and seeds > 0:
+            row["loss_over_seeds"] = loss_over_seeds(g, lists, d, B, ns, lr, steps,
+                                                     row["fp32"]["tuning"], row["bf16"]["tuning"],
+                                                     seeds)
         for k in ("fp32", "bf16"):
             row[k].pop("tuning")
         row["bf16_speedup"] = row["fp32"]["ms_per_epoch"] / row["bf16"]["ms_per_epoch"]
