@@ -133,7 +133,8 @@ def write_embedding(path: str, x: np.ndarray) -> None:
 
 
 class ForceEmbed:
-    """One graph's device-resident embedding state (fe_ctx)."""
+    """One graph's device-resident embedding state (fe_ctx): n rows of any
+    width d in [1, 512] (held zero-padded to 8 x 2^k on the device)."""
 
     def __init__(self, n: int, d: int, gather: int = FP32, quirks: int = QUIRKS_REFERENCE,
                  device: int = 0):
