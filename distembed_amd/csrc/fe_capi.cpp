@@ -531,7 +531,7 @@ extern "C" fe_status fe_autotune(fe_ctx *c, int epochs, int search_chunking, fe_
   std::vector<Cut> cuttings;
   for (int sl : {8, 16}) {
     if (sl > j.d) continue;
-    for (int ch : {16, 32, 64})
+    for (int ch : {16, 32, 64, 128, 256, 512})
       for (int mc : {64, 1024})
         for (int li : {0, 4, 8, 16, 32}) cuttings.push_back(Cut{ch, mc, li, sl});
   }

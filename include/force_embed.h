@@ -71,7 +71,7 @@ enum { FE_NEG_STRICT = 0, FE_NEG_REDRAW = 1 };
  * summation order; unroll, waves_per_block and items_per_wave do not.      */
 typedef struct fe_tuning {
   int chunk;           /* rows of degree <= chunk: one wave per row; up to
-                          4*chunk: one block per row (<= 256). A launch
+                          4*chunk: one block per row (<= 1024). A launch
                           with no block-row runs a kernel without that path
                           (more waves per SIMD)                             */
   int max_chunks;      /* most block chunks (partials) one heavy row may use */
@@ -203,8 +203,9 @@ fe_status fe_set_timing(fe_ctx *ctx, int enable);
  * depends only on its own tiling, so each trial serves every batch:
  * (1) every launch shape -- unroll {1,2,4} x waves per block {2,4,8} x
  * items per wave {1,2,4} -- on all batches: each batch keeps its fastest;
- * (2) with search_chunking, every cutting -- chunk {16,32,64} x max_chunks
- * {64,1024} x light {0,4,8,16,32} -- the same way, then (1) again. The
+ * (2) with search_chunking, every cutting -- slice {8,16} (16 when
+ * d >= 16) x chunk {16,32,64,128,256,512} x max_chunks {64,1024} x light
+ * {0,4,8,16,32} -- the same way, then (1) again. The
  * context keeps the per-batch result (fe_batch_tuning); *best holds the
  * values most batches chose and *best_ms the resulting ms per epoch. Shapes
  * never change results; the cutting changes the summation order of heavy
