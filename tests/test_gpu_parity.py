@@ -345,6 +345,44 @@ def test_training_tracks_oracle():
     assert rel < 1e-4, rel
 
 
+@pytest.mark.parametrize("B", [1, 7, 300, 301, 4096])
+def test_batch_size_extremes(B):
+    """B = 1 (one launch per row, 300 per epoch), B = n and B > n (a single
+    batch of all n rows: last = n, cpp/algo/algo.hpp:80-88), two epochs each
+    against the oracle."""
+    n, d, T = 300, 64, 2
+    g = api.Graph.sbm(n, 3, 0.05, 0.01, seed=11)
+    X = api.init_uniform(n, d, seed=4)
+    lists = api.reference_negatives(n, T + -(-n // B) - 1, 5)[0]
+    ctx = _ctx(g, d, X, B, lists)
+    ctx.train(T, B, 5, 0.02)
+    Xg = ctx.download_x()
+    ctx.close()
+    Xr = X.copy()
+    orc.train(g, Xr, B, T, lists)
+    rel = np.abs(Xg - Xr).max() / np.abs(Xr).max()
+    assert rel < 1e-4, rel
+
+
+@pytest.mark.parametrize("gather", [api.FP32, api.BF16])
+def test_graph_without_edges(gather):
+    """No CSR entries at all: every row feels only its negatives."""
+    n, d, B, T = 500, 32, 128, 3
+    g = api.Graph.from_edges(n, np.array([], dtype=np.int64), np.array([], dtype=np.int64))
+    assert g.row_ptr[-1] == 0
+    X = api.init_uniform(n, d, seed=6)
+    lists = api.reference_negatives(n, T + -(-n // B) - 1, 5)[0]
+    ctx = _ctx(g, d, X, B, lists, gather=gather)
+    ctx.train(T, B, 5, 0.02)
+    Xg = ctx.download_x()
+    ctx.close()
+    Xr = X.copy()
+    orc.train(g, Xr, B, T, lists, bf16=(gather == api.BF16))
+    assert not np.array_equal(Xr, X)  # the repulsion moved the rows
+    rel = np.abs(Xg - Xr).max() / np.abs(Xr).max()
+    assert rel < (1e-3 if gather == api.BF16 else 1e-4), rel
+
+
 def test_loss_matches_fp64_oracle():
     g = api.Graph.rmat(12, 8, seed=1)
     X = api.init_uniform(g.n, 128, seed=3)
