@@ -94,6 +94,39 @@ def test_every_round_size_gives_the_alone_bits():
         np.testing.assert_array_equal(a, g)
 
 
+def test_engine_batch_size_extremes():
+    """A job of batch size 1 (one row per round) beside a BF16 job whose
+    batch size exceeds its row count (one batch of all rows): each equals
+    its alone run, mixed and through the captured-graph schedule, and the
+    single-graph context."""
+    g1 = api.Graph.sbm(200, 2, 0.05, 0.01, seed=40)
+    g2 = api.Graph.sbm(300, 3, 0.05, 0.01, seed=41)
+    specs = [dict(g=g1, x0=api.init_uniform(g1.n, 32, seed=42), batch_size=1, iterations=2,
+                  gather=api.FP32),
+             dict(g=g2, x0=api.init_uniform(g2.n, 64, seed=43), batch_size=4096, iterations=3,
+                  gather=api.BF16)]
+    alone = []
+    for s in specs:
+        with api.Engine(1 * GiB) as e:
+            alone.append(_run_all(e, [_submit(e, s)])[0])
+    with api.Engine(1 * GiB) as e:
+        ids = [_submit(e, s) for s in specs]
+        mixed = _run_all(e, ids)
+    with api.Engine(1 * GiB) as e:
+        ids = [_submit(e, s) for s in specs]
+        e.run_graph()
+        graphed = [e.collect(i) for i in ids]
+    for s, a, m, gr in zip(specs, alone, mixed, graphed):
+        np.testing.assert_array_equal(a, m)
+        np.testing.assert_array_equal(a, gr)
+        g = s["g"]
+        with api.ForceEmbed(g.n, s["x0"].shape[1], gather=s["gather"]) as ctx:
+            ctx.upload_csr(g)
+            ctx.upload_x(s["x0"])
+            ctx.train(s["iterations"], s["batch_size"], 5, 0.02)
+            np.testing.assert_array_equal(a, ctx.download_x())
+
+
 def test_engine_job_equals_single_graph_context():
     s = _jobs()[0]
     with api.Engine(2 * GiB) as e:
