@@ -934,7 +934,7 @@ __global__ __launch_bounds__(512) void force_kernel1(const StepDesc sd) {
 
 // ---- SpMM sibling: SpMMAlgo::algo_spmm at world size 1 ----------------------
 // Y[r] = sum over the CSR entries c of row r with c < c_limit of lr * X[c]
-// (cpp/algo/spmm.hpp:341-412: dst_id < dst_end_index skips column N-1 like
+// (cpp/algo/spmm.hpp:341-395: dst_id < dst_end_index skips column N-1 like
 // the force step; :415-431 assigns the batch's rows to the output). X is
 // read through the slot map of the layout at rest and never written, so the
 // batches are independent. The work decomposition is the force step's: block

@@ -341,7 +341,7 @@ public:
   }
 };
 
-// SpMMAlgo (cpp/algo/spmm.hpp:11-52): dense_local_output = lr * A' X at
+// SpMMAlgo (cpp/algo/spmm.hpp:13-52): dense_local_output = lr * A' X at
 // world size 1, the SpMM sibling of the force step (fe_spmm). X is read from
 // dense_local and left unchanged; A' skips the columns the reference's
 // dst_id < dst_end_index skips (column N-1).
@@ -364,7 +364,7 @@ template <typename INDEX_TYPE, typename VALUE_TYPE, size_t embedding_dim> class 
   std::map<std::string, double> stats;
 
 public:
-  // per-row time of the reference's row loop (cpp/algo/spmm.hpp:398-406); the
+  // per-row time of the reference's row loop (cpp/algo/spmm.hpp:378-388); the
   // device kernel has no per-row clock, so its time is spread evenly
   std::vector<double> timing_info;
 

@@ -166,7 +166,7 @@ fe_status fe_train(fe_ctx *ctx, int iterations, int batch_size, int ns,
 fe_status fe_synchronize(fe_ctx *ctx);
 
 /* SpMMAlgo::algo_spmm(iterations, batch_size, lr) at world size 1
- * (cpp/algo/spmm.hpp:52-165; row kernel :341-412, assignment :415-431):
+ * (cpp/algo/spmm.hpp:54-159; row kernel :341-395, assignment :397-412):
  * Y[r] = sum of lr * X[c] over the CSR entries c of row r, in CSR order,
  * skipping columns c >= N-1 under FE_QUIRK_SKIP_LAST_COL as the reference's
  * dst_id < dst_end_index does. y_out (n x d, row-major) receives Y; X is not

@@ -13,7 +13,7 @@
 //   cpp/algo/algo.hpp:651-666    update X[batch] += prev
 //   cpp/algo/algo.hpp:65-72      scale() clamp
 //   cpp/core/common.cpp:37-53    generate_random_numbers (orc_std_sampler)
-//   cpp/algo/spmm.hpp:341-431    SpMMAlgo row kernel and assignment (orc_spmm_*)
+//   cpp/algo/spmm.hpp:341-412    SpMMAlgo row kernel and assignment (orc_spmm_*)
 // The arithmetic keeps the reference's expressions: the squared distance is a
 // serial sum in VALUE_TYPE, d1 is computed from double literals and converted
 // to VALUE_TYPE, and each component accumulates sequentially (edges in CSR
@@ -28,7 +28,7 @@
 //
 // SpMM (orc_spmm_*): out[r][k] = sum over e in row r (CSR order) with
 // col[e] < c_limit of lr * X[col[e]][k], accumulated serially in VALUE_TYPE
-// from zero (cpp/algo/spmm.hpp:358-363, 395-405), then assigned (:415-431).
+// from zero (cpp/algo/spmm.hpp:122-127, 379-385), then assigned (:397-412).
 // Pinned against scipy.sparse's CSR product and a closed form.
 //
 // Pinning (tests/test_oracle_*.py): the sampler against std:: compiled with

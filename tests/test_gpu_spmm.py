@@ -1,5 +1,5 @@
 """GPU parity of the SpMM sibling (fe_spmm, SpMMAlgo::algo_spmm at world size
-1, cpp/algo/spmm.hpp:341-431) against the oracle, through the C ABI.
+1, cpp/algo/spmm.hpp:341-412) against the oracle, through the C ABI.
 
 Bar (FP32), elementwise on Y = lr * A' X:
     |Y_gpu - Y_32| <= 1e-5 (|Y_32| + rms(Y_32)) + 2 |Y_32 - Y_64|,

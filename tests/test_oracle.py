@@ -360,7 +360,7 @@ def test_eval_negative_same_in_library_and_oracle():
         assert api.eval_negative(seed, r, k, n) == orc.eval_negative(seed, r, k, n)
 
 
-# ---- SpMM sibling (SpMMAlgo::algo_spmm, cpp/algo/spmm.hpp:341-431) ----------
+# ---- SpMM sibling (SpMMAlgo::algo_spmm, cpp/algo/spmm.hpp:341-412) ----------
 
 def _masked_csr(g, quirks):
     import scipy.sparse as sp
