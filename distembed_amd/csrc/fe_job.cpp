@@ -562,7 +562,7 @@ fe_status job_set_tuning(Job &j, const Tuning &t) {
   auto pow2 = [](int v) { return v > 0 && (v & (v - 1)) == 0; };
   if (t.chunk < 1 || t.chunk > 1024 || t.max_chunks < 1 || !pow2(t.unroll) || t.unroll > 8 ||
       !pow2(t.waves_per_block) || t.waves_per_block > 8 || t.items_per_wave < 1 ||
-      t.items_per_wave > 64 || t.light < 0 || t.light > 64) {
+      t.items_per_wave > 64 || t.light < 0 || t.light > kLightCols) {
     j.err = "tuning out of range";
     return FE_ERR_INVALID_ARG;
   }
@@ -592,7 +592,7 @@ Cut job_cut(const Job &j, int b) {
 
 fe_status job_set_cuts(Job &j, const std::vector<Cut> &cuts) {
   for (const Cut &c : cuts) {
-    if (c.chunk < 1 || c.chunk > 1024 || c.max_chunks < 1 || c.light < 0 || c.light > 64 ||
+    if (c.chunk < 1 || c.chunk > 1024 || c.max_chunks < 1 || c.light < 0 || c.light > kLightCols ||
         (c.slice != 8 && c.slice != 16) || c.slice > j.d) {
       j.err = "batch cutting out of range";
       return FE_ERR_INVALID_ARG;

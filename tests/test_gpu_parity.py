@@ -422,6 +422,10 @@ def test_bad_arguments_return_errors():
         ctx.step(5, 1)  # list 6 does not exist
     with pytest.raises(api.FeError):
         ctx.step(0, 4)  # batch out of range
+    for bad in (dict(light=33), dict(chunk=1025), dict(slice=4)):
+        with pytest.raises(api.FeError):
+            ctx.set_tuning(**bad)
+    ctx.set_tuning(light=32, chunk=1024)
     ctx.close()
     for d in (0, 513):
         with pytest.raises(api.FeError):
