@@ -77,7 +77,7 @@ typedef struct fe_tuning {
   int max_chunks;      /* most block chunks (partials) one heavy row may use */
   int unroll;          /* rows in flight per lane group: 1, 2, 4 or 8       */
   int waves_per_block; /* 1, 2, 4 or 8                                      */
-  int items_per_wave;  /* wave work units per wave, on average (>= 1)       */
+  int items_per_wave;  /* wave work units per wave, on average (1 to 64)    */
   int light;           /* rows of degree <= light (<= 32): one row per lane
                           group, 64/(d/slice) rows per wave (rows per
                           wavefront)                                        */
